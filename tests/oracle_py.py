@@ -1,0 +1,286 @@
+"""ctypes binding of the CPU oracle (oracle/libdgh_oracle.so).
+
+Test infrastructure: the checker for the HIP path. Imported only by tests/,
+``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB_PATH = os.path.join(_ROOT, "oracle", "libdgh_oracle.so")
+_lib = None
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(_LIB_PATH)
+        L.orc_error.restype = C.c_char_p
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        for name in dir(L):
+            pass
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Oracle:
+    """One reference ``Problem`` plus lazily built operators (CPU, FP64)."""
+
+    def __init__(self, args):
+        L = lib()
+        argv = ["oracle"] + [str(a) for a in args]
+        arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+        self.h = L.orc_create(len(argv), arr)
+        if not self.h:
+            raise OracleError(L.orc_error().decode())
+        dims = np.zeros(8, dtype=np.int64)
+        L.orc_dims(C.c_void_p(self.h), dims.ctypes.data_as(C.POINTER(C.c_longlong)))
+        self.size, self.trace_size, self.cells, self.n, self.points, self.nx, self.ny, self.faces = [int(x) for x in dims]
+        self.m = 4 * self.n * self.n
+        geo = np.zeros(5)
+        L.orc_geometry(C.c_void_p(self.h), _p(geo))
+        self.hx, self.hy, self.lx, self.ly = geo[:4]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def _call(self, name, *args):
+        L = lib()
+        rc = getattr(L, name)(C.c_void_p(self.h), *args)
+        if rc != 0:
+            raise OracleError(L.orc_error().decode())
+
+    # ---- state / setup
+    def initial(self):
+        u = np.zeros(self.size)
+        self._call("orc_initial", _p(u))
+        return u
+
+    def seeded(self, seed):
+        u = np.zeros(self.size)
+        self._call("orc_seeded", C.c_ulonglong(seed), _p(u))
+        return u
+
+    def basis(self):
+        n = self.n
+        out = np.zeros(4 * n + n * n)
+        lib().orc_basis(C.c_void_p(self.h), _p(out))
+        return dict(x=out[:n], w=out[n:2 * n], left=out[2 * n:3 * n], right=out[3 * n:4 * n],
+                    wd=out[4 * n:].reshape(n, n, order="F"))
+
+    def freeze(self, u):
+        rate = C.c_double()
+        self._call("orc_freeze", _p(np.ascontiguousarray(u)), C.byref(rate))
+        return rate.value
+
+    def speeds(self):
+        out = np.zeros(2 * self.faces * self.n)
+        lib().orc_speeds(C.c_void_p(self.h), _p(out))
+        return out.reshape(-1, 2)
+
+    def set_step(self, tau, old=None):
+        self._call("orc_set_step", C.c_double(tau), _p(np.ascontiguousarray(old)) if old is not None else None)
+
+    def dg_residual(self, u):
+        r = np.zeros(self.size)
+        self._call("orc_dg_residual", _p(np.ascontiguousarray(u)), _p(r))
+        return r
+
+    def admissible(self, u):
+        ok = C.c_int()
+        self._call("orc_admissible", _p(np.ascontiguousarray(u)), C.byref(ok))
+        return bool(ok.value)
+
+    def residual_norm(self, r):
+        out = C.c_double()
+        self._call("orc_residual_norm", _p(np.ascontiguousarray(r)), C.byref(out))
+        return out.value
+
+    def integral(self, u):
+        out = np.zeros(4)
+        self._call("orc_integral", _p(np.ascontiguousarray(u)), _p(out))
+        return out
+
+    def error_l2(self, u, t):
+        out = np.zeros(4)
+        self._call("orc_error_l2", _p(np.ascontiguousarray(u)), C.c_double(t), _p(out))
+        return out
+
+    def trace(self, u):
+        lam = np.zeros(self.trace_size)
+        self._call("orc_trace", _p(np.ascontiguousarray(u)), _p(lam))
+        return lam
+
+    def hybrid_residual(self, u, lam):
+        r = np.zeros(self.size)
+        g = np.zeros(self.trace_size)
+        self._call("orc_hybrid_residual", _p(np.ascontiguousarray(u)), _p(np.ascontiguousarray(lam)), _p(r), _p(g))
+        return r, g
+
+    # ---- DG linearisation
+    def dg_refresh(self, u):
+        self._call("orc_dg_refresh", _p(np.ascontiguousarray(u)))
+
+    def dg_apply(self, x):
+        y = np.zeros(self.size)
+        self._call("orc_dg_apply", _p(np.ascontiguousarray(x)), _p(y))
+        return y
+
+    def dg_apply_transpose(self, w):
+        z = np.zeros(self.size)
+        self._call("orc_dg_apply_transpose", _p(np.ascontiguousarray(w)), _p(z))
+        return z
+
+    def dg_block(self, row, col):
+        out = np.zeros(self.m * self.m)
+        self._call("orc_dg_block", C.c_int(row), C.c_int(col), _p(out))
+        return out.reshape(self.m, self.m, order="F")
+
+    def volume_cache(self):
+        out = np.zeros(self.cells * 3 * self.points * self.points)
+        lib().orc_volume_cache(C.c_void_p(self.h), _p(out))
+        return out
+
+    def bj_refresh(self):
+        self._call("orc_bj_refresh")
+
+    def bj_apply(self, r):
+        z = np.zeros(self.size)
+        self._call("orc_bj_apply", _p(np.ascontiguousarray(r)), _p(z))
+        return z
+
+    def bj_factor(self, block):
+        lu = np.zeros(self.m * self.m)
+        piv = np.zeros(self.m, dtype=np.int32)
+        self._call("orc_bj_factor", C.c_int(block), _p(lu), piv.ctypes.data_as(_I))
+        return lu.reshape(self.m, self.m, order="F"), piv
+
+    def dg_gmres(self, rhs, restart, limit, tol):
+        x = np.zeros(self.size)
+        res = np.zeros(self.size)
+        its = C.c_int()
+        rel = C.c_double()
+        self._call("orc_dg_gmres", _p(np.ascontiguousarray(rhs)), _p(x), C.c_int(restart), C.c_int(limit),
+                   C.c_double(tol), C.byref(its), C.byref(rel), _p(res))
+        return x, its.value, rel.value, res
+
+    def dense_jacobian(self):
+        out = np.zeros(self.size * self.size)
+        self._call("orc_dense_jacobian", _p(out))
+        return out.reshape(self.size, self.size, order="F")
+
+    # ---- HDG
+    def hdg_refresh_projected(self, u, r):
+        self._call("orc_hdg_refresh_projected", _p(np.ascontiguousarray(u)), _p(np.ascontiguousarray(r)))
+
+    def hdg_rhs(self):
+        out = np.zeros(self.trace_size)
+        lib().orc_hdg_rhs(C.c_void_p(self.h), _p(out))
+        return out
+
+    def hdg_apply(self, x):
+        y = np.zeros(self.trace_size)
+        self._call("orc_hdg_apply", _p(np.ascontiguousarray(x)), _p(y))
+        return y
+
+    def hdg_block(self, row, col):
+        fw = 4 * self.n
+        out = np.zeros(fw * fw)
+        self._call("orc_hdg_block", C.c_int(row), C.c_int(col), _p(out))
+        return out.reshape(fw, fw, order="F")
+
+    def hdg_recover(self, dl):
+        out = np.zeros(self.size)
+        self._call("orc_hdg_recover", _p(np.ascontiguousarray(dl)), _p(out))
+        return out
+
+    def hdg_recovered_residual(self, e):
+        out = np.zeros(self.size)
+        self._call("orc_hdg_recovered_residual", _p(np.ascontiguousarray(e)), _p(out))
+        return out
+
+    def hdg_factors(self):
+        t = 16 * self.n
+        x = np.zeros(self.cells * self.m * t)
+        s = np.zeros(self.cells * t * t)
+        y = np.zeros(self.cells * self.m)
+        lib().orc_hdg_factors(C.c_void_p(self.h), _p(x), _p(s), _p(y))
+        return x, s, y
+
+    def hdg_bj_refresh(self):
+        self._call("orc_hdg_bj_refresh")
+
+    def hdg_bj_apply(self, r):
+        z = np.zeros(self.trace_size)
+        self._call("orc_hdg_bj_apply", _p(np.ascontiguousarray(r)), _p(z))
+        return z
+
+    # ---- whole run
+    def advance(self, method, u, cap=100000):
+        u = np.array(u, dtype=np.float64, copy=True)
+        run = np.zeros(12)
+        upd = np.zeros((cap, 13))
+        count = C.c_int()
+        self._call("orc_advance", method.encode(), _p(u), _p(run), _p(upd), C.c_int(cap), C.byref(count))
+        keys = ["final_time", "conservation_drift", "seconds", "max_linear_residual", "max_dg_linear_residual",
+                "minimum_step", "maximum_step", "measured_newton_seconds", "steps", "newton_iterations",
+                "krylov_iterations", "measured_updates"]
+        stats = dict(zip(keys, run.tolist()))
+        ukeys = ["step", "stage", "iteration", "krylov_iterations", "time", "stage_time", "dt", "cfl",
+                 "residual_physical_rms", "damping", "linear_relative_residual", "dg_linear_relative_residual",
+                 "iteration_s"]
+        stats["updates"] = [dict(zip(ukeys, row.tolist())) for row in upd[:min(count.value, cap)]]
+        return u, stats
+
+
+def gauss(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    rc = lib().orc_gauss(C.c_int(n), _p(x), _p(w))
+    if rc:
+        raise OracleError(lib().orc_error().decode())
+    return x, w
+
+
+def lu_solve(a, b):
+    a = np.asfortranarray(a, dtype=np.float64)
+    n = a.shape[0]
+    x = np.zeros(n)
+    rc = lib().orc_lu_solve(C.c_int(n), _p(a.ravel(order="F").copy()), _p(np.ascontiguousarray(b, dtype=np.float64)),
+                            _p(x))
+    if rc:
+        raise OracleError(lib().orc_error().decode())
+    return x
+
+
+def uniform(seed, count):
+    out = np.zeros(count)
+    lib().orc_uniform(C.c_ulonglong(seed), C.c_longlong(count), _p(out))
+    return out
+
+
+def normal(seed, count):
+    out = np.zeros(count)
+    lib().orc_normal(C.c_ulonglong(seed), C.c_longlong(count), _p(out))
+    return out
