@@ -1,0 +1,861 @@
+// Host side of libdgh_device.so: context set-up, the device-resident Newton /
+// GMRES drivers and the extern "C" entry points of include/dgh/dgh_device.h.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dgh/dgh_device.h"
+#include "dgh_internal.hpp"
+#include "dgh_host.hpp"
+
+using namespace dgh;
+
+namespace {
+thread_local std::string g_create_error;
+
+void require(bool condition, const char *message) {
+    if (!condition) throw std::runtime_error(message);
+}
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+#define HIPCHECK(call) hip_check((call), #call)
+
+using Clock = std::chrono::steady_clock;
+double elapsed(Clock::time_point start) { return std::chrono::duration<double>(Clock::now() - start).count(); }
+
+// ------------------------------------------------------------- host basis
+// Gauss-Legendre nodes/weights by Newton iteration on P_n, ascending, exactly
+// symmetric, middle node 0 for odd n (nodal_basis.hpp:13-25 conventions).
+void gauss_rule(int n, std::vector<double> &x, std::vector<double> &w) {
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    const double pi = std::acos(-1.0);
+    auto legendre = [n](double z, double &pn, double &dpn) {
+        double p0 = 1.0, p1 = z;
+        for (int l = 2; l <= n; ++l) {
+            const double p2 = ((2.0 * l - 1.0) * z * p1 - (l - 1.0) * p0) / l;
+            p0 = p1;
+            p1 = p2;
+        }
+        if (n == 1) { p0 = 1.0; p1 = z; }
+        pn = p1;
+        dpn = n * (z * p1 - p0) / (z * z - 1.0);
+    };
+    for (int k = 1; k <= (n + 1) / 2; ++k) {
+        double z = std::cos(pi * (k - 0.25) / (n + 0.5)), pn, dpn;
+        for (int it = 0; it < 100; ++it) {
+            legendre(z, pn, dpn);
+            const double dz = pn / dpn;
+            z -= dz;
+            if (std::abs(dz) <= 1e-16 * std::max(1.0, std::abs(z)) && it > 2) break;
+        }
+        legendre(z, pn, dpn);
+        x[k - 1] = -z;
+        x[n - k] = z;
+        w[k - 1] = w[n - k] = 2.0 / ((1.0 - z * z) * dpn * dpn);
+    }
+    if (n % 2) x[n / 2] = 0;
+}
+
+std::vector<double> lagrange_at(const std::vector<double> &x, double z) {
+    const int n = static_cast<int>(x.size());
+    std::vector<double> v(n, 1.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            if (i != j) v[i] *= (z - x[j]) / (x[i] - x[j]);
+    return v;
+}
+
+// nodal_basis.hpp:34-65 for the collocated rule (points == nodes).
+Basis make_basis(int n, double hx, double hy) {
+    Basis b{};
+    std::vector<double> x, w;
+    gauss_rule(n, x, w);
+    std::vector<double> bary(n, 1.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            if (i != j) bary[i] /= x[i] - x[j];
+    std::vector<double> D(n * n, 0.0);  // D(i, j) at [i + n*j]
+    for (int i = 0; i < n; ++i) {
+        double s = 0;
+        for (int j = 0; j < n; ++j)
+            if (i != j) D[i + n * j] = bary[j] / (bary[i] * (x[i] - x[j]));
+        for (int j = 0; j < n; ++j) s += D[i + n * j];
+        D[i + n * i] = -s;
+    }
+    for (int q = 0; q < n; ++q)
+        for (int i = 0; i < n; ++i) b.wd[q + n * i] = w[q] * D[q + n * i];
+    for (int i = 0; i < n; ++i) { b.w[i] = w[i]; b.x[i] = x[i]; }
+    const auto l = lagrange_at(x, -1.0), r = lagrange_at(x, 1.0);
+    for (int i = 0; i < n; ++i) { b.left[i] = l[i]; b.right[i] = r[i]; }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            b.mass[i + n * j] = .25 * hx * hy * w[i] * w[j];
+            b.inv_mass[i + n * j] = 1.0 / b.mass[i + n * j];
+        }
+    const int eq = std::max(20, 2 * n + 3);
+    std::vector<double> ex, ew;
+    gauss_rule(eq, ex, ew);
+    for (int q = 0; q < eq; ++q) {
+        b.ex[q] = ex[q];
+        b.ew[q] = ew[q];
+        const auto e = lagrange_at(x, ex[q]);
+        for (int i = 0; i < n; ++i) b.ev[q + eq * i] = e[i];
+    }
+    for (int i = 0; i < n; ++i)
+        for (int q = 0; q < eq; ++q) b.proj[i + n * q] = ((1.0 / w[i]) * b.ev[q + eq * i]) * ew[q];
+    return b;
+}
+
+// ------------------------------------------------------------- device memory
+template <class T> struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count, size_t &counter) {
+        if (n >= count && p) return;
+        release(counter);
+        HIPCHECK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+        n = count;
+        counter += count * sizeof(T);
+    }
+    void release(size_t &counter) {
+        if (p) {
+            (void)hipFree(p);
+            counter -= n * sizeof(T);
+        }
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct Timer {
+    double ms = 0;
+    long long launches = 0;
+};
+
+}  // namespace
+
+// ----------------------------------------------------------------- context
+struct dgh_ctx {
+    dgh_config cfg{};
+    Geo geo{};
+    Basis hbasis{};
+    int n = 0, P = 0, m = 0, cells = 0, faces = 0, fw = 0;
+    long long D = 0, T = 0;
+    double tau = 0, inv_step = 0;
+    bool refreshed = false, frozen = false, bj_ready = false;
+    hipStream_t stream = nullptr;
+    std::string error;
+    size_t bytes = 0;
+    bool timing = false;
+    std::map<std::string, Timer> timers;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    DevBuf<Basis> basis;
+    DevBuf<double> old, speeds, tr_u, tr_x, fq, gtr, vcache, fcache, partials, scalars;
+    DevBuf<int> flags;
+    DevBuf<unsigned long long> rate_bits;
+    DevBuf<double> factors;
+    DevBuf<float> factors32;
+    DevBuf<int> rowsrc;
+    // GMRES
+    DevBuf<double> kbasis, kw, kz, kwork, kres, ky;
+    // Newton
+    DevBuf<double> state, trial, residual, trial_residual, direction, rhs, equation_error, step_state, stage_base,
+        slopes;
+
+    explicit dgh_ctx(const dgh_config &c) : cfg(c) {
+        n = c.degree + 1;
+        P = n * n;
+        m = 4 * P;
+        cells = c.nx * c.ny;
+        faces = 2 * cells;
+        fw = 4 * n;
+        D = static_cast<long long>(cells) * m;
+        T = static_cast<long long>(faces) * fw;
+        const double pi = std::acos(-1.0);
+        const double cosine = std::cos(45. * pi / 180.), sine = std::sin(45. * pi / 180.);
+        geo.nx = c.nx;
+        geo.ny = c.ny;
+        geo.n = n;
+        geo.cells = cells;
+        geo.eq = std::max(20, 2 * n + 3);
+        geo.testcase = c.testcase;
+        geo.lx = c.testcase == DGH_DIAGONAL ? 2. / cosine : 2.;
+        geo.ly = c.testcase == DGH_DIAGONAL ? 2. / sine : 2.;
+        geo.hx = geo.lx / c.nx;
+        geo.hy = geo.ly / c.ny;
+        geo.gamma = 3.0;
+        geo.safety = c.safety;
+        geo.cosine = cosine;
+        geo.sine = sine;
+        hbasis = make_basis(n, geo.hx, geo.hy);
+        HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreate(&ev0));
+        HIPCHECK(hipEventCreate(&ev1));
+        basis.alloc(1, bytes);
+        HIPCHECK(hipMemcpy(basis.p, &hbasis, sizeof(Basis), hipMemcpyHostToDevice));
+        old.alloc(D, bytes);
+        speeds.alloc(2LL * faces * n, bytes);
+        tr_u.alloc(16LL * cells * n, bytes);
+        tr_x.alloc(16LL * cells * n, bytes);
+        fq.alloc(static_cast<size_t>(T), bytes);
+        gtr.alloc(2 * static_cast<size_t>(T), bytes);
+        vcache.alloc(3LL * cells * P, bytes);
+        fcache.alloc(static_cast<size_t>(F_NFIELDS) * faces * n, bytes);
+        partials.alloc(REDUCE_BLOCKS * 32, bytes);
+        scalars.alloc(256, bytes);
+        flags.alloc(8, bytes);
+        rate_bits.alloc(1, bytes);
+        HIPCHECK(hipMemsetAsync(old.p, 0, D * sizeof(double), stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+    }
+    ~dgh_ctx() {
+        DevBuf<double> *bufs[] = {&old, &speeds, &tr_u, &tr_x, &fq, &gtr, &vcache, &fcache, &partials, &scalars,
+                                  &factors, &kbasis, &kw, &kz, &kwork, &kres, &ky, &state, &trial, &residual,
+                                  &trial_residual, &direction, &rhs, &equation_error, &step_state, &stage_base,
+                                  &slopes};
+        for (auto *b : bufs) b->release(bytes);
+        factors32.release(bytes);
+        rowsrc.release(bytes);
+        flags.release(bytes);
+        rate_bits.release(bytes);
+        basis.release(bytes);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // ---- timing of kernel groups on the context stream
+    template <class F> void timed(const char *name, F &&launch) {
+        if (!timing) {
+            launch();
+            return;
+        }
+        HIPCHECK(hipEventRecord(ev0, stream));
+        launch();
+        HIPCHECK(hipEventRecord(ev1, stream));
+        HIPCHECK(hipEventSynchronize(ev1));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        auto &t = timers[name];
+        t.ms += ms;
+        t.launches += 1;
+    }
+    void sync() { HIPCHECK(hipStreamSynchronize(stream)); }
+
+    // ---- reductions
+    double reduce_sq(const double *x, long long count, int mode) {
+        HIPCHECK(launch_weighted_sq(n, x, count, mode, partials.p, basis.p, stream));
+        HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 1, scalars.p + 200, stream));
+        double out = 0;
+        HIPCHECK(hipMemcpyAsync(&out, scalars.p + 200, sizeof(double), hipMemcpyDeviceToHost, stream));
+        sync();
+        return out;
+    }
+    double dual_norm(const double *x) { return std::sqrt(reduce_sq(x, D, 0)); }
+    double state_norm(const double *x) { return std::sqrt(reduce_sq(x, D, 1)); }
+    double gmres_norm(const double *x) { return std::sqrt(std::max(0.0, reduce_sq(x, D, 0))); }
+    std::array<double, 4> integral(const double *u) {
+        HIPCHECK(launch_integral(n, u, cells, partials.p, basis.p, stream));
+        HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 4, scalars.p + 200, stream));
+        std::array<double, 4> out{};
+        HIPCHECK(hipMemcpyAsync(out.data(), scalars.p + 200, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        sync();
+        return out;
+    }
+    int read_flag(int idx) {
+        int f = 0;
+        HIPCHECK(hipMemcpyAsync(&f, flags.p + idx, sizeof(int), hipMemcpyDeviceToHost, stream));
+        sync();
+        return f;
+    }
+    void set_flag(int idx, int value) { HIPCHECK(hipMemsetD32Async(flags.p + idx, value, 1, stream)); }
+    void copy(double *dst, const double *src, long long count) {
+        HIPCHECK(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    }
+    void axpby(double a, const double *x, double b, const double *y, double *out, long long count) {
+        HIPCHECK(launch_axpby(a, x, b, y, out, count, stream));
+    }
+
+    // ---- Problem
+    void initial(double *u) {
+        set_flag(3, 0);
+        if (cfg.testcase == DGH_SEEDED) HIPCHECK(launch_seeded(n, u, cfg.seed, flags.p + 3, basis.p, geo, stream));
+        else HIPCHECK(launch_initial(n, u, flags.p + 3, basis.p, geo, stream));
+        require(read_flag(3) == 0, "Nonphysical cell average");
+        require(admissible(u), "Invalid projected initial state");
+    }
+    void seeded(double *u, unsigned long long seed) {
+        set_flag(3, 0);
+        HIPCHECK(launch_seeded(n, u, seed, flags.p + 3, basis.p, geo, stream));
+        require(read_flag(3) == 0, "Nonphysical cell average");
+        require(admissible(u), "Invalid projected initial state");
+    }
+    double freeze(const double *u) {
+        set_flag(0, 0);
+        HIPCHECK(hipMemsetAsync(rate_bits.p, 0, sizeof(unsigned long long), stream));
+        HIPCHECK(launch_restrict(n, u, tr_u.p, basis.p, geo, stream));
+        HIPCHECK(launch_freeze_speeds(n, tr_u.p, speeds.p, flags.p, geo, stream));
+        HIPCHECK(launch_freeze_rate(n, u, speeds.p, rate_bits.p, flags.p, geo, stream));
+        copy(old.p, u, D);
+        unsigned long long bits = 0;
+        HIPCHECK(hipMemcpyAsync(&bits, rate_bits.p, sizeof(bits), hipMemcpyDeviceToHost, stream));
+        const int f = read_flag(0);
+        require(!(f & 1), "Invalid frozen face state");
+        require(!(f & 2), "Invalid CFL volume state");
+        frozen = true;
+        refreshed = false;
+        double rate;
+        std::memcpy(&rate, &bits, sizeof(rate));
+        return rate;
+    }
+    void residual_into(const double *u, double *r) {
+        require(frozen, "Freeze the wave speeds before evaluating the residual");
+        timed("residual", [&] {
+            HIPCHECK(launch_restrict(n, u, tr_u.p, basis.p, geo, stream));
+            HIPCHECK(launch_residual_flux(n, tr_u.p, speeds.p, fq.p, basis.p, geo, stream));
+            HIPCHECK(launch_element_residual(n, u, old.p, fq.p, r, tau, basis.p, geo, stream));
+        });
+    }
+    bool admissible(const double *u) {
+        set_flag(1, 1);
+        HIPCHECK(launch_admissible(n, u, flags.p + 1, basis.p, geo, stream));
+        return read_flag(1) != 0;
+    }
+    std::array<double, 4> error_l2(const double *u, double time) {
+        HIPCHECK(launch_error_partials(n, u, time, partials.p, REDUCE_BLOCKS, basis.p, geo, stream));
+        HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 4, scalars.p + 200, stream));
+        std::array<double, 4> out{};
+        HIPCHECK(hipMemcpyAsync(out.data(), scalars.p + 200, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        sync();
+        for (auto &v : out) v = std::sqrt(v);
+        return out;
+    }
+
+    // ---- DG linearisation
+    void dg_refresh(const double *u) {
+        require(tau > 0, "Set the time step before refreshing the Jacobian");
+        require(frozen, "Freeze the wave speeds before refreshing the Jacobian");
+        inv_step = 1 / tau;
+        HIPCHECK(launch_volume_cache(n, u, vcache.p, geo, stream));
+        HIPCHECK(launch_restrict(n, u, tr_u.p, basis.p, geo, stream));
+        HIPCHECK(launch_face_cache(n, tr_u.p, speeds.p, fcache.p, basis.p, geo, stream));
+        refreshed = true;
+        bj_ready = false;
+    }
+    void dg_apply(const double *x, double *y) {
+        require(refreshed, "Refresh the volume Jacobian before applying it");
+        timed("jv", [&] {
+            HIPCHECK(launch_restrict(n, x, tr_x.p, basis.p, geo, stream));
+            HIPCHECK(launch_jv_flux(n, tr_x.p, fcache.p, fq.p, geo, stream));
+            HIPCHECK(launch_element_jv(n, x, vcache.p, fq.p, y, inv_step, basis.p, geo, stream));
+        });
+    }
+    void dg_apply_transpose(const double *w, double *z) {
+        require(refreshed, "Refresh the volume Jacobian before applying it");
+        timed("jtv", [&] {
+            HIPCHECK(launch_restrict(n, w, tr_x.p, basis.p, geo, stream));
+            HIPCHECK(launch_adjoint_face(n, tr_x.p, fcache.p, gtr.p, geo, stream));
+            HIPCHECK(launch_element_jtv(n, w, vcache.p, gtr.p, z, inv_step, basis.p, geo, stream));
+        });
+    }
+    void bj_refresh() {
+        require(refreshed, "Refresh the volume Jacobian before assembling a block");
+        factors.alloc(static_cast<size_t>(cells) * m * m, bytes);
+        rowsrc.alloc(static_cast<size_t>(cells) * m, bytes);
+        set_flag(2, 0);
+        timed("assemble", [&] {
+            HIPCHECK(launch_assemble_blocks(n, vcache.p, fcache.p, factors.p, inv_step, basis.p, geo, 0, cells, stream));
+        });
+        timed("lu", [&] { HIPCHECK(launch_lu(m, factors.p, rowsrc.p, cells, flags.p + 2, stream)); });
+        require(read_flag(2) == 0, "Singular block Jacobi factor");
+        bj_ready = true;
+    }
+    void bj_apply(const double *r, double *z) {
+        require(bj_ready, "Refresh the block Jacobi preconditioner first");
+        timed("bj_apply", [&] { HIPCHECK(launch_bj_apply(m, factors.p, rowsrc.p, r, z, cells, stream)); });
+    }
+
+    // ---- GMRES (gmres.cpp:17-86), A = DG Jv, M = block Jacobi, metric M^-1
+    struct Linear { int iterations; double relative; };
+    Linear gmres(const double *rhs, double *x, int limit, double tolerance) {
+        const int restart = static_cast<int>(std::min<long long>(D, cfg.restart));
+        require(restart + 1 <= 32, "GMRES restart above 31 is not supported on the device");
+        kbasis.alloc(static_cast<size_t>(restart + 1) * D, bytes);
+        kw.alloc(D, bytes);
+        kz.alloc(D, bytes);
+        kwork.alloc(D, bytes);
+        kres.alloc(D, bytes);
+        ky.alloc(64, bytes);
+        std::vector<const double *> V(restart + 1);
+        for (int k = 0; k <= restart; ++k) V[k] = kbasis.p + static_cast<size_t>(k) * D;
+        auto Vk = [&](int k) { return kbasis.p + static_cast<size_t>(k) * D; };
+        HIPCHECK(hipMemsetAsync(x, 0, D * sizeof(double), stream));
+        copy(kres.p, rhs, D);
+        const double rhs_norm = gmres_norm(rhs);
+        if (rhs_norm == 0) return {0, 0};
+        std::vector<double> H((restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), y(restart);
+        auto Hm = [&](int i, int j) -> double & { return H[i + (restart + 1) * j]; };
+        int iterations = 0;
+        double host_scalars[65];
+        while (iterations < limit) {
+            const double beta = gmres_norm(kres.p);
+            if (beta <= tolerance * rhs_norm) return {iterations, beta / rhs_norm};
+            scale_div(kres.p, beta, Vk(0));  // basis_.col(0) = residual_ / beta
+            std::fill(H.begin(), H.end(), 0.0);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            int count = 0;
+            for (int j = 0; j < restart && iterations < limit; ++j) {
+                bj_apply(Vk(j), kz.p);
+                dg_apply(kz.p, kw.p);
+                timed("gmres_ortho", [&] {
+                    HIPCHECK(launch_multi_dot(n, V.data(), j + 1, kw.p, nullptr, D, partials.p, basis.p, stream));
+                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p, stream));
+                    HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p, kw.p, nullptr, D, partials.p, 0,
+                                               basis.p, stream));
+                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p + 32, stream));
+                    HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p + 32, kw.p, nullptr, D, partials.p, 1,
+                                               basis.p, stream));
+                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 1, scalars.p + 64, stream));
+                    HIPCHECK(hipMemcpyAsync(host_scalars, scalars.p, 65 * sizeof(double), hipMemcpyDeviceToHost,
+                                            stream));
+                    sync();
+                });
+                for (int k = 0; k <= j; ++k) Hm(k, j) = (0.0 + host_scalars[k]) + host_scalars[32 + k];
+                const double next = std::sqrt(std::max(0.0, host_scalars[64]));
+                Hm(j + 1, j) = next;
+                if (next > 0) scale_div(kw.p, next, Vk(j + 1));
+                for (int i = 0; i < j; ++i) {
+                    const double a = Hm(i, j), b = Hm(i + 1, j);
+                    Hm(i, j) = cs[i] * a + sn[i] * b;
+                    Hm(i + 1, j) = -sn[i] * a + cs[i] * b;
+                }
+                const double d = std::hypot(Hm(j, j), Hm(j + 1, j));
+                require(std::isfinite(d) && d > 0, "GMRES Arnoldi breakdown");
+                cs[j] = Hm(j, j) / d;
+                sn[j] = Hm(j + 1, j) / d;
+                Hm(j, j) = d;
+                Hm(j + 1, j) = 0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] *= cs[j];
+                ++iterations;
+                count = j + 1;
+                if (std::abs(g[j + 1]) <= tolerance * rhs_norm || next <= std::numeric_limits<double>::min()) break;
+            }
+            for (int i = count - 1; i >= 0; --i) {
+                double value = g[i];
+                for (int k = i + 1; k < count; ++k) value -= Hm(i, k) * y[k];
+                y[i] = value / Hm(i, i);
+            }
+            HIPCHECK(hipMemcpyAsync(ky.p, y.data(), count * sizeof(double), hipMemcpyHostToDevice, stream));
+            HIPCHECK(launch_combine(V.data(), count, ky.p, kwork.p, D, stream));
+            bj_apply(kwork.p, kw.p);
+            axpby(1.0, x, 1.0, kw.p, x, D);
+            dg_apply(x, kwork.p);
+            axpby(1.0, rhs, -1.0, kwork.p, kres.p, D);
+            const double relative = gmres_norm(kres.p) / rhs_norm;
+            require(std::isfinite(relative), "Nonfinite GMRES residual");
+            if (relative <= tolerance) return {iterations, relative};
+        }
+        throw std::runtime_error("GMRES iteration limit reached");
+    }
+    void scale_div(const double *src, double d, double *dst) { HIPCHECK(launch_scale_div(src, d, dst, D, stream)); }
+
+    // ---- TimeIntegrator::advance (time_integrator.cpp:48-213), DG method
+    void advance(double *solution, dgh_run_stats &stats, std::vector<dgh_update_stats> &updates) {
+        const auto &c = cfg;
+        require(c.method != DGH_METHOD_HDG, "HDG method is not available on the device path yet");
+        require(!c.direct, "--linear direct is a host reference and is not available on the device path");
+        require(admissible(solution), "Invalid initial solution");
+        const double shock = (c.testcase == DGH_WAVE || c.testcase == DGH_SEEDED)
+                                 ? std::numeric_limits<double>::infinity()
+                                 : 4 / ((geo.gamma + 1) * std::acos(-1.));
+        require(c.steps > 0 || c.final_time < shock, "Final time reaches simple-wave shock");
+        constexpr double alpha = .24169426078821, beta = alpha / 4., eta = .12915286960590;
+        constexpr double a[4][4] = {{alpha, 0, 0, 0}, {-alpha, alpha, 0, 0}, {0, 1 - alpha, alpha, 0},
+                                    {beta, eta, .5 - beta - eta - alpha, alpha}};
+        constexpr double b[4] = {0, 1. / 6, 1. / 6, 2. / 3}, stage_c[4] = {alpha, 0, 1, .5};
+        const int stages = c.integrator == 0 ? 1 : 4;
+        std::memset(&stats, 0, sizeof(stats));
+        freeze(solution);
+        const auto initial_integral = integral(solution);
+        state.alloc(D, bytes);
+        trial.alloc(D, bytes);
+        residual.alloc(D, bytes);
+        trial_residual.alloc(D, bytes);
+        direction.alloc(D, bytes);
+        rhs.alloc(D, bytes);
+        equation_error.alloc(D, bytes);
+        step_state.alloc(D, bytes);
+        stage_base.alloc(D, bytes);
+        if (stages > 1) slopes.alloc(static_cast<size_t>(4) * D, bytes);
+        auto slope = [&](int s) { return slopes.p + static_cast<size_t>(s) * D; };
+        const auto begin = Clock::now();
+        while (c.steps > 0 ? stats.steps < c.steps : stats.final_time < c.final_time) {
+            const double rate = freeze(solution);
+            double step = c.cfl / rate;
+            if (c.steps == 0) {
+                const double remaining = c.final_time - stats.final_time;
+                step = std::min(step, remaining);
+                if (remaining - step <= 16 * std::numeric_limits<double>::epsilon() * std::max(remaining, c.final_time))
+                    step = remaining;
+            }
+            require(std::isfinite(step) && step > 0 && stats.final_time + step > stats.final_time, "Invalid CFL step");
+            require(stats.final_time + step < shock, "Step reaches simple-wave shock");
+            copy(step_state.p, solution, D);
+            copy(state.p, solution, D);
+            for (int stage = 0; stage < stages; ++stage) {
+                const double diagonal = stages == 1 ? 1. : a[stage][stage];
+                copy(stage_base.p, step_state.p, D);
+                for (int j = 0; j < stage; ++j) axpby(1.0, stage_base.p, step * a[stage][j], slope(j), stage_base.p, D);
+                if (stage > 0) freeze(state.p);
+                tau = diagonal * step;
+                copy(old.p, stage_base.p, D);
+                refreshed = false;
+                bj_ready = false;
+                const double old_norm = state_norm(stage_base.p);
+                const int limit = c.newton_steps > 0 ? c.newton_steps : c.max_newton;
+                double initial_rms = -1.;
+                bool converged = false, residual_ready = false;
+                for (int it = 0; it <= limit; ++it) {
+                    const auto update_start = Clock::now();
+                    if (!residual_ready) residual_into(state.p, residual.p);
+                    const double norm = dual_norm(residual.p), rms = norm / std::sqrt(geo.lx * geo.ly);
+                    if (initial_rms < 0) initial_rms = rms;
+                    if (c.newton_steps == 0 && rms <= c.nonlinear_tolerance * std::max(1., initial_rms)) {
+                        converged = true;
+                        break;
+                    }
+                    if (it == limit) break;
+                    const double allowance = 128 * std::numeric_limits<double>::epsilon() *
+                                             ((state_norm(state.p) + old_norm) / tau + norm);
+                    const double target = std::max(10 * c.linear_tolerance * norm, allowance);
+                    dg_refresh(state.p);
+                    axpby(-1.0, residual.p, 0.0, nullptr, rhs.p, D);
+                    bj_refresh();
+                    const Linear linear = gmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance);
+                    axpby(-1.0, kres.p, 0.0, nullptr, equation_error.p, D);
+                    const double error_norm = dual_norm(equation_error.p);
+                    require(error_norm <= target, "DG Newton linear residual too large");
+                    double damping = 1.;
+                    bool accepted = false;
+                    for (int search = 0; search < 30; ++search) {
+                        axpby(1.0, state.p, damping, direction.p, trial.p, D);
+                        if (admissible(trial.p)) {
+                            residual_into(trial.p, trial_residual.p);
+                            if (dual_norm(trial_residual.p) <= (1 - 1e-4 * damping) * norm + allowance) {
+                                accepted = true;
+                                break;
+                            }
+                        }
+                        damping *= .5;
+                    }
+                    require(accepted, "Newton line search failed");
+                    std::swap(state, trial);
+                    std::swap(residual, trial_residual);
+                    residual_ready = true;
+                    sync();
+                    const double duration = elapsed(update_start);
+                    stats.max_linear_residual = std::max(stats.max_linear_residual, linear.relative);
+                    const double dgrel = norm > 0 ? error_norm / norm : 0.;
+                    stats.max_dg_linear_residual = std::max(stats.max_dg_linear_residual, dgrel);
+                    stats.krylov_iterations += linear.iterations;
+                    if (stats.newton_iterations >= c.warmup_updates) {
+                        stats.measured_newton_seconds += duration;
+                        ++stats.measured_updates;
+                    }
+                    ++stats.newton_iterations;
+                    dgh_update_stats u{};
+                    u.step = stats.steps;
+                    u.stage = stage;
+                    u.iteration = it;
+                    u.krylov_iterations = linear.iterations;
+                    u.time = stats.final_time + step;
+                    u.stage_time = stats.final_time + (stages == 1 ? 1. : stage_c[stage]) * step;
+                    u.dt = step;
+                    u.cfl = step * rate;
+                    u.residual_physical_rms = rms;
+                    u.damping = damping;
+                    u.linear_relative_residual = linear.relative;
+                    u.dg_linear_relative_residual = dgrel;
+                    u.iteration_s = duration;
+                    u.inner_iterations = linear.iterations;
+                    u.outer_iterations = 0;
+                    updates.push_back(u);
+                }
+                require(converged || c.newton_steps > 0, "Newton iteration limit reached");
+                if (stages > 1) HIPCHECK(launch_sub_div(state.p, stage_base.p, tau, slope(stage), D, stream));
+            }
+            if (stages == 1) copy(solution, state.p, D);
+            else {
+                copy(solution, step_state.p, D);
+                for (int s = 0; s < stages; ++s) axpby(1.0, solution, step * b[s], slope(s), solution, D);
+                require(admissible(solution), "DIRK3 step produced an inadmissible state");
+            }
+            stats.final_time = c.steps == 0 && step >= c.final_time - stats.final_time ? c.final_time
+                                                                                       : stats.final_time + step;
+            stats.minimum_step = stats.steps == 0 ? step : std::min(stats.minimum_step, step);
+            stats.maximum_step = std::max(stats.maximum_step, step);
+            ++stats.steps;
+            const auto now = integral(solution);
+            double drift = 0;
+            for (int v = 0; v < 4; ++v) drift = std::max(drift, std::abs(now[v] - initial_integral[v]));
+            stats.conservation_drift = std::max(stats.conservation_drift, drift);
+        }
+        sync();
+        stats.seconds = elapsed(begin);
+    }
+};
+
+// ------------------------------------------------------------------ C ABI
+namespace {
+template <class F> int guarded(dgh_ctx *ctx, F &&f) {
+    try {
+        f();
+        return DGH_OK;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->error = e.what();
+        else g_create_error = e.what();
+        return DGH_ERROR;
+    }
+}
+void check_ctx(dgh_ctx *ctx) { require(ctx != nullptr, "Null context"); }
+}  // namespace
+
+extern "C" {
+
+size_t dgh_config_size(void) { return sizeof(dgh_config); }
+
+int dgh_config_default(dgh_config *out) {
+    if (!out) return DGH_ERROR;
+    *out = default_config();
+    return DGH_OK;
+}
+
+int dgh_config_parse(int argc, const char *const *argv, dgh_config *out, char *err, size_t errlen) {
+    try {
+        *out = parse_config(argc, argv);
+        return DGH_OK;
+    } catch (const std::exception &e) {
+        if (err && errlen) {
+            std::strncpy(err, e.what(), errlen - 1);
+            err[errlen - 1] = 0;
+        }
+        g_create_error = e.what();
+        return DGH_ERROR;
+    }
+}
+
+int dgh_create(const dgh_config *cfg, dgh_ctx **out) {
+    return guarded(nullptr, [&] {
+        require(cfg != nullptr && out != nullptr, "Null argument");
+        validate_config(*cfg);
+        require(cfg->quadrature == 0 || cfg->quadrature == cfg->degree + 1,
+                "Over-integration (--quad > k+1) is not available on the device path");
+        *out = new dgh_ctx(*cfg);
+    });
+}
+
+void dgh_destroy(dgh_ctx *ctx) { delete ctx; }
+
+const char *dgh_last_error(const dgh_ctx *ctx) { return ctx ? ctx->error.c_str() : g_create_error.c_str(); }
+
+int dgh_dims(const dgh_ctx *ctx, int64_t out[10]) {
+    if (!ctx) return DGH_ERROR;
+    out[0] = ctx->D;
+    out[1] = ctx->T;
+    out[2] = ctx->cells;
+    out[3] = ctx->n;
+    out[4] = ctx->faces;
+    out[5] = ctx->m;
+    out[6] = ctx->fw;
+    out[7] = ctx->cfg.nx;
+    out[8] = ctx->cfg.ny;
+    out[9] = static_cast<int64_t>(ctx->bytes);
+    return DGH_OK;
+}
+
+int dgh_geometry(const dgh_ctx *ctx, double out[5]) {
+    if (!ctx) return DGH_ERROR;
+    out[0] = ctx->geo.hx;
+    out[1] = ctx->geo.hy;
+    out[2] = ctx->geo.lx;
+    out[3] = ctx->geo.ly;
+    out[4] = ctx->tau;
+    return DGH_OK;
+}
+
+int dgh_basis(const dgh_ctx *ctx, double *x, double *w) {
+    if (!ctx) return DGH_ERROR;
+    for (int i = 0; i < ctx->n; ++i) {
+        if (x) x[i] = ctx->hbasis.x[i];
+        if (w) w[i] = ctx->hbasis.w[i];
+    }
+    return DGH_OK;
+}
+
+int dgh_initial(dgh_ctx *ctx, double *u) {
+    return guarded(ctx, [&] { check_ctx(ctx); ctx->initial(u); });
+}
+int dgh_seeded(dgh_ctx *ctx, uint64_t seed, double *u) {
+    return guarded(ctx, [&] { check_ctx(ctx); ctx->seeded(u, seed); });
+}
+int dgh_freeze(dgh_ctx *ctx, const double *u, double *rate) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        const double r = ctx->freeze(u);
+        if (rate) *rate = r;
+    });
+}
+int dgh_set_step(dgh_ctx *ctx, double tau, const double *old) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->tau = tau;
+        if (old) ctx->copy(ctx->old.p, old, ctx->D);
+        ctx->refreshed = false;
+        ctx->bj_ready = false;
+        ctx->sync();
+    });
+}
+int dgh_dg_residual(dgh_ctx *ctx, const double *u, double *r) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->residual_into(u, r);
+        ctx->sync();
+    });
+}
+int dgh_admissible(dgh_ctx *ctx, const double *u, int *ok) {
+    return guarded(ctx, [&] { check_ctx(ctx); *ok = ctx->admissible(u) ? 1 : 0; });
+}
+int dgh_residual_norm(dgh_ctx *ctx, const double *r, double *rms) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        *rms = std::sqrt(ctx->reduce_sq(r, ctx->D, 0) / (ctx->geo.lx * ctx->geo.ly));
+    });
+}
+int dgh_dual_norm(dgh_ctx *ctx, const double *x, double *out) {
+    return guarded(ctx, [&] { check_ctx(ctx); *out = ctx->dual_norm(x); });
+}
+int dgh_state_norm(dgh_ctx *ctx, const double *x, double *out) {
+    return guarded(ctx, [&] { check_ctx(ctx); *out = ctx->state_norm(x); });
+}
+int dgh_integral(dgh_ctx *ctx, const double *u, double out[4]) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        const auto v = ctx->integral(u);
+        for (int k = 0; k < 4; ++k) out[k] = v[k];
+    });
+}
+int dgh_error_l2(dgh_ctx *ctx, const double *u, double time, double out[4]) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        const auto v = ctx->error_l2(u, time);
+        for (int k = 0; k < 4; ++k) out[k] = v[k];
+    });
+}
+int dgh_speeds(dgh_ctx *ctx, double *out) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->copy(out, ctx->speeds.p, 2LL * ctx->faces * ctx->n);
+        ctx->sync();
+    });
+}
+int dgh_dg_refresh(dgh_ctx *ctx, const double *u) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->dg_refresh(u);
+        ctx->sync();
+    });
+}
+int dgh_dg_apply(dgh_ctx *ctx, const double *x, double *y) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->dg_apply(x, y);
+        ctx->sync();
+    });
+}
+int dgh_dg_apply_transpose(dgh_ctx *ctx, const double *w, double *z) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->dg_apply_transpose(w, z);
+        ctx->sync();
+    });
+}
+int dgh_dg_block(dgh_ctx *ctx, int row, double *out_host) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        require(ctx->refreshed, "Refresh the volume Jacobian before assembling a block");
+        require(row >= 0 && row < ctx->cells, "Invalid block coordinates");
+        const size_t mm = static_cast<size_t>(ctx->m) * ctx->m;
+        DevBuf<double> one;
+        one.alloc(mm, ctx->bytes);
+        HIPCHECK(launch_assemble_blocks(ctx->n, ctx->vcache.p, ctx->fcache.p, one.p, ctx->inv_step, ctx->basis.p,
+                                        ctx->geo, row, 1, ctx->stream));
+        HIPCHECK(hipMemcpyAsync(out_host, one.p, mm * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        one.release(ctx->bytes);
+    });
+}
+int dgh_bj_refresh(dgh_ctx *ctx) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->bj_refresh();
+        ctx->sync();
+    });
+}
+int dgh_bj_apply(dgh_ctx *ctx, const double *r, double *z) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->bj_apply(r, z);
+        ctx->sync();
+    });
+}
+int dgh_dg_gmres(dgh_ctx *ctx, const double *rhs, double *x, int max_iterations, double tolerance, int *iterations,
+                 double *relative_residual, double *residual) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        const auto r = ctx->gmres(rhs, x, max_iterations, tolerance);
+        if (residual) ctx->copy(residual, ctx->kres.p, ctx->D);
+        ctx->sync();
+        if (iterations) *iterations = r.iterations;
+        if (relative_residual) *relative_residual = r.relative;
+    });
+}
+int dgh_advance(dgh_ctx *ctx, double *u, dgh_run_stats *run, dgh_update_stats *updates, int capacity, int *count) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        dgh_run_stats stats{};
+        std::vector<dgh_update_stats> list;
+        ctx->advance(u, stats, list);
+        if (run) *run = stats;
+        if (updates)
+            for (int k = 0; k < capacity && k < static_cast<int>(list.size()); ++k) updates[k] = list[k];
+        if (count) *count = static_cast<int>(list.size());
+    });
+}
+int dgh_kernel_time(dgh_ctx *ctx, const char *name, double *total_ms, long long *launches) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        auto it = ctx->timers.find(name);
+        *total_ms = it == ctx->timers.end() ? 0.0 : it->second.ms;
+        *launches = it == ctx->timers.end() ? 0 : it->second.launches;
+    });
+}
+int dgh_reset_kernel_times(dgh_ctx *ctx) {
+    return guarded(ctx, [&] { check_ctx(ctx); ctx->timers.clear(); });
+}
+int dgh_set_timing(dgh_ctx *ctx, int enabled) {
+    return guarded(ctx, [&] { check_ctx(ctx); ctx->timing = enabled != 0; });
+}
+}

@@ -1,0 +1,105 @@
+// Internal declarations shared by the HIP kernel translation units and the
+// host driver of libdgh_device.so. Not part of the public C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace dgh {
+
+constexpr int NVAR = 4;
+constexpr int MAXN = 9;    // degree <= 8 (reference config.cpp:7)
+constexpr int MAXEQ = 21;  // max(20, 2n+3) error/projection points
+
+// Reference-element tables of one context (nodal_basis.hpp:34-65), uploaded
+// once; kernels read them through uniform (scalar-cache) loads.
+struct Basis {
+    double wd[MAXN * MAXN];      // weighted derivative d(q, i) at [q + n*i]
+    double w[MAXN], x[MAXN], left[MAXN], right[MAXN];
+    double mass[MAXN * MAXN];    // .25*hx*hy*w_i*w_j at node i + n*j
+    double inv_mass[MAXN * MAXN];
+    double proj[MAXN * MAXEQ];   // L2 projection n x EQ, column-major [i + n*q]
+    double ev[MAXEQ * MAXN];     // basis at error points EQ x n, [q + EQ*i]
+    double ex[MAXEQ], ew[MAXEQ];
+};
+
+// Mesh / physics scalars passed by value to every kernel.
+struct Geo {
+    int nx, ny, n, cells, eq;
+    int testcase;  // 0 diagonal, 1 wave, 2 axis, 3 seeded
+    double hx, hy, lx, ly, gamma, safety, cosine, sine;
+};
+
+// Face-node linearisation cache (face_jacobian.cpp:16-37), structure of
+// arrays over face nodes f*n + q.
+enum FaceField { F_UL, F_VL, F_HL, F_UR, F_VR, F_HR, F_BL, F_BR, F_PEN, F_TAUL, F_TAUR, F_NFIELDS };
+
+// ------------------------------------------------------------ launchers
+// All launchers enqueue on `stream` and return hipError_t of the launch.
+// Vectors use the reference layout cell*4n^2 + v*n^2 + (i + n*j).
+hipError_t launch_restrict(int n, const double *x, double *traces, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_freeze_speeds(int n, const double *traces, double *speeds, int *flag, const Geo &g,
+                                hipStream_t s);
+hipError_t launch_freeze_rate(int n, const double *u, const double *speeds, unsigned long long *rate_bits,
+                              int *flag, const Geo &g, hipStream_t s);
+hipError_t launch_residual_flux(int n, const double *traces, const double *speeds, double *flux,
+                                const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_element_residual(int n, const double *u, const double *old, const double *flux, double *r,
+                                   double tau, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_admissible(int n, const double *u, int *flag, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_volume_cache(int n, const double *u, double *vcache, const Geo &g, hipStream_t s);
+hipError_t launch_face_cache(int n, const double *traces, const double *speeds, double *fcache,
+                             const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_jv_flux(int n, const double *traces, const double *fcache, double *flux, const Geo &g,
+                          hipStream_t s);
+hipError_t launch_element_jv(int n, const double *x, const double *vcache, const double *flux, double *y,
+                             double inv_step, const Basis *b, const Geo &g, hipStream_t s);
+// Adjoint: z = J^T w, volume part and the face-flux adjoint.
+hipError_t launch_adjoint_face(int n, const double *wtraces, const double *fcache, double *gtraces,
+                               const Geo &g, hipStream_t s);
+hipError_t launch_element_jtv(int n, const double *w, const double *vcache, const double *gtraces, double *z,
+                              double inv_step, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_initial(int n, double *u, int *flag, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_seeded(int n, double *u, unsigned long long seed, int *flag, const Basis *b, const Geo &g,
+                         hipStream_t s);
+hipError_t launch_error_partials(int n, const double *u, double time, double *partials, int nblocks,
+                                 const Basis *b, const Geo &g, hipStream_t s);
+
+// Reductions (two-pass, fixed order, deterministic).
+constexpr int REDUCE_BLOCKS = 1024;
+// mode 0: sum x^2 * inv_mass(node); 1: sum x^2 * mass(node); 2: plain sum x^2
+hipError_t launch_weighted_sq(int n, const double *x, long long count, int mode, double *partials,
+                              const Basis *b, hipStream_t s);
+hipError_t launch_integral(int n, const double *u, long long cells, double *partials, const Basis *b,
+                           hipStream_t s);
+hipError_t launch_finish_sum(const double *partials, int nparts, int width, double *out, hipStream_t s);
+
+// Block Jacobi (DG element blocks m = 4n^2).
+// Blocks of cells [first, first+count) into blocks[(cell-first) * m*m].
+hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fcache, double *blocks,
+                                  double inv_step, const Basis *b, const Geo &g, int first, int count,
+                                  hipStream_t s);
+hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s);
+hipError_t launch_lu_f32(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s);
+hipError_t launch_bj_apply(int m, const double *factors, const int *rowsrc, const double *r, double *z,
+                           int nblocks, hipStream_t s);
+hipError_t launch_convert_f32(const double *src, float *dst, long long count, hipStream_t s);
+
+// GMRES building blocks over vectors of length `count` with the DG metric
+// inv_mass(node) (metric 0) or a plain per-entry weight vector (metric 1).
+hipError_t launch_multi_dot(int n, const double *const *basis, int nvec, const double *w, const double *weights,
+                            long long count, double *partials, const Basis *b, hipStream_t s);
+hipError_t launch_update_dot(int n, const double *const *basis, int nvec, const double *h, double *w,
+                             const double *weights, long long count, double *partials, int mode,
+                             const Basis *b, hipStream_t s);
+hipError_t launch_combine(const double *const *basis, int nvec, const double *y, double *out, long long count,
+                          hipStream_t s);
+hipError_t launch_axpby(double a, const double *x, double b, const double *y, double *out, long long count,
+                        hipStream_t s);
+// out = x / d
+hipError_t launch_scale_div(const double *x, double d, double *out, long long count, hipStream_t s);
+// out = (x - y) / d
+hipError_t launch_sub_div(const double *x, const double *y, double d, double *out, long long count, hipStream_t s);
+
+}  // namespace dgh

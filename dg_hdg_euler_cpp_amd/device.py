@@ -1,0 +1,256 @@
+"""ctypes binding of the C ABI in ``include/dgh/dgh_device.h``.
+
+Vectors are torch CUDA (HIP) tensors of dtype float64 whose device pointers are
+handed to the library; torch is only memory plumbing here. The binding fails
+loudly when ``libdgh_device.so`` is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libdgh_device.so")
+_lib = None
+
+_D = C.POINTER(C.c_double)
+
+
+class DeviceError(RuntimeError):
+    """An error reported by libdgh_device.so (reference message text)."""
+
+
+class UpdateStats(C.Structure):
+    _fields_ = [("step", C.c_int), ("stage", C.c_int), ("iteration", C.c_int), ("krylov_iterations", C.c_int),
+                ("time", C.c_double), ("stage_time", C.c_double), ("dt", C.c_double), ("cfl", C.c_double),
+                ("residual_physical_rms", C.c_double), ("damping", C.c_double),
+                ("linear_relative_residual", C.c_double), ("dg_linear_relative_residual", C.c_double),
+                ("iteration_s", C.c_double), ("inner_iterations", C.c_int), ("outer_iterations", C.c_int)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("final_time", C.c_double), ("conservation_drift", C.c_double), ("seconds", C.c_double),
+                ("max_linear_residual", C.c_double), ("max_dg_linear_residual", C.c_double),
+                ("minimum_step", C.c_double), ("maximum_step", C.c_double),
+                ("measured_newton_seconds", C.c_double), ("steps", C.c_int), ("newton_iterations", C.c_int),
+                ("krylov_iterations", C.c_int), ("measured_updates", C.c_int)]
+
+
+def load_library():
+    """Load libdgh_device.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DeviceError(f"{LIB_PATH} is missing: build it with `python -m dg_hdg_euler_cpp_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    L.dgh_last_error.restype = C.c_char_p
+    L.dgh_last_error.argtypes = [C.c_void_p]
+    L.dgh_config_size.restype = C.c_size_t
+    L.dgh_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+    L.dgh_destroy.argtypes = [C.c_void_p]
+    L.dgh_config_parse.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_void_p, C.c_char_p, C.c_size_t]
+    _lib = L
+    return L
+
+
+def _ptr(t):
+    """Device pointer of a float64 torch tensor (contiguous), as c_void_p."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise DeviceError("device vectors must be contiguous")
+    return C.c_void_p(t.data_ptr())
+
+
+def parse_config(args):
+    """Reference CLI flags -> opaque dgh_config buffer."""
+    L = load_library()
+    argv = ["dg_hdg_2d"] + [str(a) for a in args]
+    arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+    buf = C.create_string_buffer(L.dgh_config_size())
+    err = C.create_string_buffer(512)
+    if L.dgh_config_parse(len(argv), arr, buf, err, 512) != 0:
+        raise DeviceError(err.value.decode())
+    return buf
+
+
+class DeviceContext:
+    """One device-resident Problem + operators (the reference's Problem,
+    DGOperator, BlockJacobi, GMRES and TimeIntegrator behind the C ABI)."""
+
+    def __init__(self, args):
+        import torch  # memory plumbing only
+
+        self._torch = torch
+        L = load_library()
+        cfg = parse_config(args)
+        h = C.c_void_p()
+        if L.dgh_create(cfg, C.byref(h)) != 0:
+            raise DeviceError(L.dgh_last_error(None).decode())
+        self.h = h
+        dims = (C.c_int64 * 10)()
+        L.dgh_dims(self.h, dims)
+        self.size, self.trace_size, self.cells, self.n, self.faces, self.m, self.fw, self.nx, self.ny, _ = list(dims)
+        geo = (C.c_double * 5)()
+        L.dgh_geometry(self.h, geo)
+        self.hx, self.hy, self.lx, self.ly = geo[0], geo[1], geo[2], geo[3]
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().dgh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- helpers
+    def _call(self, name, *args):
+        L = load_library()
+        self._torch.cuda.synchronize()
+        rc = getattr(L, name)(self.h, *args)
+        if rc != 0:
+            raise DeviceError(L.dgh_last_error(self.h).decode())
+
+    def zeros(self, n=None):
+        return self._torch.zeros(self.size if n is None else n, dtype=self._torch.float64, device="cuda")
+
+    def device_bytes(self):
+        dims = (C.c_int64 * 10)()
+        load_library().dgh_dims(self.h, dims)
+        return int(dims[9])
+
+    def basis(self):
+        x = (C.c_double * self.n)()
+        w = (C.c_double * self.n)()
+        load_library().dgh_basis(self.h, x, w)
+        return list(x), list(w)
+
+    # ---- state / Problem
+    def initial(self):
+        u = self.zeros()
+        self._call("dgh_initial", _ptr(u))
+        return u
+
+    def seeded(self, seed):
+        u = self.zeros()
+        self._call("dgh_seeded", C.c_uint64(seed), _ptr(u))
+        return u
+
+    def freeze(self, u):
+        rate = C.c_double()
+        self._call("dgh_freeze", _ptr(u), C.byref(rate))
+        return rate.value
+
+    def speeds(self):
+        out = self.zeros(2 * self.faces * self.n)
+        self._call("dgh_speeds", _ptr(out))
+        return out
+
+    def set_step(self, tau, old=None):
+        self._call("dgh_set_step", C.c_double(tau), _ptr(old))
+
+    def dg_residual(self, u, out=None):
+        r = self.zeros() if out is None else out
+        self._call("dgh_dg_residual", _ptr(u), _ptr(r))
+        return r
+
+    def admissible(self, u):
+        ok = C.c_int()
+        self._call("dgh_admissible", _ptr(u), C.byref(ok))
+        return bool(ok.value)
+
+    def residual_norm(self, r):
+        out = C.c_double()
+        self._call("dgh_residual_norm", _ptr(r), C.byref(out))
+        return out.value
+
+    def dual_norm(self, x):
+        out = C.c_double()
+        self._call("dgh_dual_norm", _ptr(x), C.byref(out))
+        return out.value
+
+    def state_norm(self, x):
+        out = C.c_double()
+        self._call("dgh_state_norm", _ptr(x), C.byref(out))
+        return out.value
+
+    def integral(self, u):
+        out = (C.c_double * 4)()
+        self._call("dgh_integral", _ptr(u), out)
+        return list(out)
+
+    def error_l2(self, u, t):
+        out = (C.c_double * 4)()
+        self._call("dgh_error_l2", _ptr(u), C.c_double(t), out)
+        return list(out)
+
+    # ---- DG operator / block Jacobi / GMRES
+    def dg_refresh(self, u):
+        self._call("dgh_dg_refresh", _ptr(u))
+
+    def dg_apply(self, x, out=None):
+        y = self.zeros() if out is None else out
+        self._call("dgh_dg_apply", _ptr(x), _ptr(y))
+        return y
+
+    def dg_apply_transpose(self, w, out=None):
+        z = self.zeros() if out is None else out
+        self._call("dgh_dg_apply_transpose", _ptr(w), _ptr(z))
+        return z
+
+    def dg_block(self, row):
+        import numpy as np
+
+        out = np.zeros(self.m * self.m)
+        self._call("dgh_dg_block", C.c_int(row), out.ctypes.data_as(_D))
+        return out.reshape(self.m, self.m, order="F")
+
+    def bj_refresh(self):
+        self._call("dgh_bj_refresh")
+
+    def bj_apply(self, r, out=None):
+        z = self.zeros() if out is None else out
+        self._call("dgh_bj_apply", _ptr(r), _ptr(z))
+        return z
+
+    def dg_gmres(self, rhs, max_iterations, tolerance):
+        x = self.zeros()
+        res = self.zeros()
+        its = C.c_int()
+        rel = C.c_double()
+        self._call("dgh_dg_gmres", _ptr(rhs), _ptr(x), C.c_int(max_iterations), C.c_double(tolerance),
+                   C.byref(its), C.byref(rel), _ptr(res))
+        return x, its.value, rel.value, res
+
+    def advance(self, u, capacity=100000):
+        run = RunStats()
+        upd = (UpdateStats * capacity)()
+        count = C.c_int()
+        self._call("dgh_advance", _ptr(u), C.byref(run), upd, C.c_int(capacity), C.byref(count))
+        stats = {f: getattr(run, f) for f, _ in RunStats._fields_}
+        stats["updates"] = [{f: getattr(upd[i], f) for f, _ in UpdateStats._fields_}
+                            for i in range(min(count.value, capacity))]
+        return stats
+
+    # ---- timing hooks
+    def set_timing(self, enabled):
+        self._call("dgh_set_timing", C.c_int(1 if enabled else 0))
+
+    def reset_kernel_times(self):
+        self._call("dgh_reset_kernel_times")
+
+    def kernel_time(self, name):
+        ms = C.c_double()
+        launches = C.c_longlong()
+        self._call("dgh_kernel_time", name.encode(), C.byref(ms), C.byref(launches))
+        return ms.value, launches.value

@@ -1,0 +1,138 @@
+"""HIP path vs the CPU oracle on identical seeded / projected states.
+
+Tolerances: the north-star bar is 1e-12 relative for the residual, Jv and the
+preconditioner apply. Reductions and FMA contraction differ between the GPU
+and the host oracle, so nothing here is bitwise; each tolerance is written at
+its assert.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ("diagonal-k2-8x8", ["--nx", "8", "--ny", "8", "--k", "2", "--testcase", "diagonal"], None),
+    ("seeded-k2-8x8", ["--nx", "8", "--ny", "8", "--k", "2", "--testcase", "seeded"], 7),
+    ("wave-k4-6x5", ["--nx", "6", "--ny", "5", "--k", "4", "--testcase", "wave"], None),
+    ("seeded-k8-16x16", ["--nx", "16", "--ny", "16", "--k", "8", "--testcase", "seeded"], 2025),
+    ("diagonal-k8-12x12", ["--nx", "12", "--ny", "12", "--k", "8", "--testcase", "diagonal"], None),
+]
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def setup_pair(args, seed, torch):
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    print(f"\n[case] {args} seed={seed}", flush=True)
+    dev = DeviceContext(args + ["--cfl", "5", "--integrator", "be"])
+    orc = O.Oracle(args + ["--cfl", "5", "--integrator", "be"])
+    if seed is None:
+        u_o = orc.initial()
+        u_d = dev.initial()
+    else:
+        u_o = orc.seeded(seed)
+        u_d = dev.seeded(seed)
+    return dev, orc, u_d, u_o
+
+
+@pytest.mark.parametrize("name,args,seed", CASES, ids=[c[0] for c in CASES])
+def test_state_freeze_residual(name, args, seed, torch_cuda):
+    dev, orc, u_d, u_o = setup_pair(args, seed, torch_cuda)
+    # initial/seeded state: projection + positivity scaling, 1e-13 relative
+    assert rel(u_d.cpu().numpy(), u_o) < 1e-13
+    u = torch_cuda.tensor(u_o, device="cuda")  # identical input for the rest
+    rate_o = orc.freeze(u_o)
+    rate_d = dev.freeze(u)
+    assert abs(rate_d - rate_o) <= 1e-14 * rate_o
+    sp_d = dev.speeds().cpu().numpy().reshape(-1, 2)
+    assert rel(sp_d, orc.speeds()) < 1e-14
+    tau = 0.3 / rate_o
+    old_o = u_o * (1 + 1e-3 * np.sin(np.arange(u_o.size)))
+    orc.set_step(tau, old_o)
+    dev.set_step(tau, torch_cuda.tensor(old_o, device="cuda"))
+    r_o = orc.dg_residual(u_o)
+    r_d = dev.dg_residual(u).cpu().numpy()
+    assert rel(r_d, r_o) < 1e-12, "DG residual"
+    assert abs(dev.residual_norm(torch_cuda.tensor(r_o, device="cuda")) - orc.residual_norm(r_o)) <= 1e-13 * orc.residual_norm(r_o)
+    assert dev.admissible(u) and orc.admissible(u_o)
+    bad = u_o.copy()
+    bad[5] = -1.0  # negative density at one node
+    assert not orc.admissible(bad)
+    assert not dev.admissible(torch_cuda.tensor(bad, device="cuda"))
+    integ_o = orc.integral(u_o)
+    integ_d = dev.integral(u)
+    assert np.allclose(integ_d, integ_o, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("name,args,seed", CASES, ids=[c[0] for c in CASES])
+def test_jv_block_adjoint(name, args, seed, torch_cuda):
+    dev, orc, _, u_o = setup_pair(args, seed, torch_cuda)
+    u = torch_cuda.tensor(u_o, device="cuda")
+    rate = orc.freeze(u_o)
+    dev.freeze(u)
+    tau = 5.0 / rate
+    orc.set_step(tau, u_o)
+    dev.set_step(tau, u)
+    orc.dg_refresh(u_o)
+    dev.dg_refresh(u)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(u_o.size)
+    y_o = orc.dg_apply(x)
+    y_d = dev.dg_apply(torch_cuda.tensor(x, device="cuda")).cpu().numpy()
+    assert rel(y_d, y_o) < 1e-12, "Jv"
+    # element block (row == col) vs the oracle's DGOperator::block
+    for row in [0, orc.cells // 2, orc.cells - 1]:
+        b_o = orc.dg_block(row, row)
+        b_d = dev.dg_block(row)
+        assert rel(b_d, b_o) < 1e-13, f"block {row}"
+    # J^T w: against the oracle's block-row transpose and the duality identity
+    w = rng.standard_normal(u_o.size)
+    z_o = orc.dg_apply_transpose(w)
+    z_d = dev.dg_apply_transpose(torch_cuda.tensor(w, device="cuda")).cpu().numpy()
+    assert rel(z_d, z_o) < 1e-12, "J^T w"
+    lhs = float(np.dot(w, y_o))
+    rhs = float(np.dot(z_d, x))
+    assert abs(lhs - rhs) <= 1e-11 * (abs(lhs) + np.linalg.norm(w) * np.linalg.norm(y_o))
+
+
+@pytest.mark.parametrize("name,args,seed", CASES, ids=[c[0] for c in CASES])
+def test_block_jacobi_and_gmres(name, args, seed, torch_cuda):
+    dev, orc, _, u_o = setup_pair(args, seed, torch_cuda)
+    u = torch_cuda.tensor(u_o, device="cuda")
+    rate = orc.freeze(u_o)
+    dev.freeze(u)
+    tau = 5.0 / rate
+    orc.set_step(tau, u_o)
+    dev.set_step(tau, u)
+    orc.dg_refresh(u_o)
+    dev.dg_refresh(u)
+    orc.bj_refresh()
+    dev.bj_refresh()
+    rng = np.random.default_rng(5)
+    r = rng.standard_normal(u_o.size)
+    z_o = orc.bj_apply(r)
+    z_d = dev.bj_apply(torch_cuda.tensor(r, device="cuda")).cpu().numpy()
+    # partial-pivot LU of the same blocks: solve-level agreement (the pivot
+    # sequence and the factor rounding are not bit-identical to a blocked LU)
+    assert rel(z_d, z_o) < 1e-11, "block-Jacobi apply"
+    # the apply inverts the assembled block: B z = r per element
+    m = orc.m
+    for row in [0, orc.cells - 1]:
+        B = orc.dg_block(row, row)
+        resid = B @ z_d[row * m:(row + 1) * m] - r[row * m:(row + 1) * m]
+        assert np.linalg.norm(resid) <= 1e-11 * np.linalg.norm(r[row * m:(row + 1) * m])
+    # GMRES(20) on J dx = -r(u) to 1e-12
+    rhs = -orc.dg_residual(u_o)
+    x_o, its_o, rel_o, _ = orc.dg_gmres(rhs, 30, 1500, 1e-12)
+    x_d, its_d, rel_d, res_d = dev.dg_gmres(torch_cuda.tensor(rhs, device="cuda"), 1500, 1e-12)
+    x_d = x_d.cpu().numpy()
+    assert rel_d <= 1e-12 and rel_o <= 1e-12
+    assert abs(its_d - its_o) <= 2, (its_d, its_o)
+    assert rel(x_d, x_o) < 1e-9, "GMRES solution"
