@@ -346,6 +346,39 @@ int orc_hdg_bj_apply(void *h, const double *r, double *z) {
         copy_out(out, z);
     });
 }
+// Physics kernels for the check of physics.cpp: flux, analytic and cached
+// Jacobian (row-major 4x4) of one state.
+int orc_physics(const double *state, int axis, double *flux, double *analytic, double *cached) {
+    return guarded([&] {
+        Gas gas;
+        State u = {state[0], state[1], state[2], state[3]};
+        State f = gas.flux(u, axis);
+        Jac a = gas.jacobian(u, axis);
+        auto p = gas.primitive(u);
+        Jac c = gas.cached_jacobian(p.velocity[0], p.velocity[1], p.enthalpy, axis);
+        for (int k = 0; k < 4; ++k) flux[k] = f[k];
+        for (int k = 0; k < 16; ++k) { analytic[k] = a[k]; cached[k] = c[k]; }
+    });
+}
+// TensorProduct::subtract_weak on one element with Q quadrature points.
+int orc_subtract_weak(int degree, int quadrature, double hx, double hy, const double *fx, const double *fy,
+                      double *residual) {
+    return guarded([&] {
+        Basis b(degree, quadrature);
+        TensorProduct t(b, hx, hy);
+        t.subtract_weak(fx, fy, residual);
+    });
+}
+// Basis tables of a (degree, quadrature) pair: values [Q x n], weighted derivatives [Q x n], qw [Q].
+int orc_basis_tables(int degree, int quadrature, double *values, double *wd, double *qw) {
+    return guarded([&] {
+        Basis b(degree, quadrature);
+        copy_out(b.values.a, values);
+        copy_out(b.weighted_derivatives.a, wd);
+        copy_out(b.qw, qw);
+    });
+}
+
 // Dense DG Jacobian (volume x volume, column-major) from the block provider.
 int orc_dense_jacobian(void *h, double *out) {
     return guarded([&] {

@@ -274,6 +274,38 @@ def lu_solve(a, b):
     return x
 
 
+def physics(state, axis):
+    f = np.zeros(4)
+    a = np.zeros(16)
+    c = np.zeros(16)
+    rc = lib().orc_physics(_p(np.ascontiguousarray(state, dtype=np.float64)), C.c_int(axis), _p(f), _p(a), _p(c))
+    if rc:
+        raise OracleError(lib().orc_error().decode())
+    return f, a.reshape(4, 4), c.reshape(4, 4)
+
+
+def subtract_weak(degree, quadrature, hx, hy, fx, fy, residual):
+    out = np.array(residual, dtype=np.float64, copy=True)
+    rc = lib().orc_subtract_weak(C.c_int(degree), C.c_int(quadrature), C.c_double(hx), C.c_double(hy),
+                                 _p(np.ascontiguousarray(fx, dtype=np.float64)),
+                                 _p(np.ascontiguousarray(fy, dtype=np.float64)), _p(out))
+    if rc:
+        raise OracleError(lib().orc_error().decode())
+    return out
+
+
+def basis_tables(degree, quadrature):
+    n = degree + 1
+    Q = quadrature or n
+    values = np.zeros(Q * n)
+    wd = np.zeros(Q * n)
+    qw = np.zeros(Q)
+    rc = lib().orc_basis_tables(C.c_int(degree), C.c_int(quadrature), _p(values), _p(wd), _p(qw))
+    if rc:
+        raise OracleError(lib().orc_error().decode())
+    return values.reshape(Q, n, order="F"), wd.reshape(Q, n, order="F"), qw
+
+
 def uniform(seed, count):
     out = np.zeros(count)
     lib().orc_uniform(C.c_ulonglong(seed), C.c_longlong(count), _p(out))
