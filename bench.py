@@ -1,0 +1,193 @@
+"""Newton-update throughput of the 2D DG k=8 256x256 implicit Euler solve on one MI355X.
+
+A step is one damped Newton update of the BE stage (residual, Jacobian refresh,
+block-Jacobi assembly + batched 324x324 LU, right-preconditioned GMRES(20) to
+ltol=1e-12, line search with positivity check) -- the reference's speed protocol
+(speed_test/2d/benchmark.py:26-36: BE, CFL 5, 1 step, fixed Newton updates, the
+first W are warm-up). Inputs are resident in HBM before the timed region.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--init seeded|diagonal]
+
+Prints one JSON line on stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def solver_args(nx, k, init, precision, newton_steps, warmup, device=True):
+    args = ["--nx", str(nx), "--ny", str(nx), "--k", str(k), "--method", "dg", "--integrator", "be",
+            "--steps", "1", "--newton-steps", str(newton_steps), "--warmup-updates", str(warmup), "--cfl", "5",
+            "--ntol", "1e-12", "--ltol", "1e-12", "--restart", "20", "--max-linear", "1500", "--safety", "1.1",
+            "--timing-only", "--testcase", init]
+    if device:
+        args += ["--seed", "2025", "--precision", precision]
+    return args
+
+
+def run_device(args, nx, k, init, precision):
+    import torch
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    with DeviceContext(solver_args(nx, k, init, precision, args.warmup + args.steps, args.warmup)) as dev:
+        u = dev.initial()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = dev.advance(u)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        measured = stats["measured_newton_seconds"]
+        ups = stats["measured_updates"] / measured if measured > 0 else float("nan")
+        upd = stats["updates"]
+        log(f"[bench] {precision} {init}: updates={len(upd)} measured={stats['measured_updates']} "
+            f"mean_s={measured / max(1, stats['measured_updates']):.4f} wall={wall:.2f}s")
+        for x in upd:
+            log(f"   update {x['iteration']}: rms={x['residual_physical_rms']:.6e} krylov={x['krylov_iterations']} "
+                f"lin={x['linear_relative_residual']:.3e} damping={x['damping']} t={x['iteration_s']:.4f}s")
+        # Dominant kernel: block-Jacobi apply, timed with HIP events on the
+        # library's stream over a few launches on the final linearisation.
+        r = torch.randn(dev.size, dtype=torch.float64, device="cuda")
+        z = dev.zeros()
+        dev.bj_apply(r, z)
+        dev.set_timing(True)
+        dev.reset_kernel_times()
+        for _ in range(10):
+            dev.bj_apply(r, z)
+        bj_ms, bj_n = dev.kernel_time("bj_apply")
+        dev.reset_kernel_times()
+        y = dev.zeros()
+        for _ in range(10):
+            dev.dg_apply(z, y)
+        jv_ms, jv_n = dev.kernel_time("jv")
+        dev.set_timing(False)
+        geom = dict(size=dev.size, cells=dev.cells, m=dev.m, n=dev.n, faces=dev.faces)
+    return dict(ups=ups, stats=stats, wall=wall, bj_avg_s=bj_ms / 1e3 / max(1, bj_n),
+                jv_avg_s=jv_ms / 1e3 / max(1, jv_n), geom=geom)
+
+
+def bj_bytes(g, elem_bytes=8):
+    """Algorithmic bytes of one block-Jacobi apply: LU factors (m^2 per block),
+    int32 row permutation, read r and write z."""
+    return g["cells"] * g["m"] * g["m"] * elem_bytes + g["cells"] * g["m"] * 4 + 2 * g["size"] * 8
+
+
+def jv_bytes(g):
+    """Algorithmic bytes of one Jv: read x, the volume cache (u, v, H per node),
+    the face cache (9 doubles per face node) and write y."""
+    n = g["n"]
+    return 2 * g["size"] * 8 + 3 * g["cells"] * n * n * 8 + 9 * g["faces"] * n * 8
+
+
+def cpu_baseline(nx_full, k, init, krylov_per_update):
+    """The oracle (1 core, FP64) on a 32x32 sub-problem of the same k=8 workload,
+    scaled per element to 256^2: t_update = (C/1024) * (t_refresh + n_K * t_iter
+    + 2 * t_residual), with n_K the device run's mean Krylov count per update."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_py as O
+
+    nx = 32
+    o = O.Oracle(solver_args(nx, k, init, "fp64", 1, 0, device=False))
+    u = o.seeded(2025) if init == "seeded" else o.initial()
+    rate = o.freeze(u)
+    o.set_step(5.0 / rate, u)
+    t0 = time.perf_counter()
+    r = o.dg_residual(u)
+    t_res = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.dg_refresh(u)
+    o.bj_refresh()
+    t_ref = time.perf_counter() - t0
+    iters = 20
+    t0 = time.perf_counter()
+    try:
+        _, its, _, _ = o.dg_gmres(-r, iters, iters, 1e-300)
+    except O.OracleError:
+        its = iters
+    t_g = time.perf_counter() - t0
+    t_iter = t_g / (its + 1)  # + the end-of-cycle M^-1 V y and true-residual Jv
+    scale = (nx_full * nx_full) / (nx * nx)
+    t_update = scale * (t_ref + krylov_per_update * t_iter + 2 * t_res)
+    sample_s = t_res + t_ref + t_g
+    log(f"[cpu] 32x32 k={k}: residual {t_res:.3f}s refresh+LU {t_ref:.3f}s gmres/iter {t_iter:.4f}s "
+        f"-> {t_update:.1f} s/update at {nx_full}^2 (sample {sample_s:.1f}s)")
+    return dict(value=1.0 / t_update, unit="Newton updates/s", cores=1, kind="port",
+                sample=(f"oracle FP64 on 1 host core: residual, Jacobian refresh + 1024 block LUs and {its} "
+                        f"GMRES(20) iterations on a 32x32 k={k} mesh ({sample_s:.1f}s), scaled by "
+                        f"{scale:.0f}x elements and the device run's {krylov_per_update:.0f} Krylov "
+                        f"iterations per update"),
+                seconds_per_update=t_update)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--k", type=int, default=8)
+    ap.add_argument("--init", default="seeded", choices=["seeded", "diagonal"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "mixed-bf16"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X; multi-GPU runs are not part of this build "
+              "(the path would shard by 2D element tiles, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    dev = run_device(args, args.nx, args.k, args.init, args.precision)
+    g = dev["geom"]
+    upd = [x for x in dev["stats"]["updates"]][args.warmup:]
+    krylov = sum(x["krylov_iterations"] for x in upd) / max(1, len(upd))
+    bytes_bj = bj_bytes(g)
+    achieved = bytes_bj / dev["bj_avg_s"] / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "bj_apply_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.nx, args.k, args.init, krylov)
+    line = {
+        "metric": "2D DG k=8 256² Newton updates/s: FP64 vs mixed-prec at equal residual",
+        "value": dev["ups"],
+        "unit": "Newton updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / dev["ups"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32",
+        "data": f"synthetic ({args.init} state, seed 2025)",
+        "config": {"workload": f"2D DG BE Newton update, k={args.k}, {args.nx}x{args.nx} periodic quads, "
+                               f"CFL 5, GMRES(20) + block-Jacobi LU, ntol=ltol=1e-12",
+                   "nx": args.nx, "ny": args.nx, "k": args.k, "init": args.init, "precision": args.precision,
+                   "krylov_per_update": krylov},
+        "roofline": {"kernel": "bj_apply", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": bytes_bj, "avg_launch_s": dev["bj_avg_s"]},
+        "jv": {"dof_per_s": g["size"] / dev["jv_avg_s"], "achieved_GBs": jv_bytes(g) / dev["jv_avg_s"] / 1e9,
+               "avg_launch_s": dev["jv_avg_s"]},
+        "cpu_baseline": None if cpu is None else {k: v for k, v in cpu.items() if k != "seconds_per_update"},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
