@@ -119,20 +119,43 @@ def test_block_jacobi_and_gmres(name, args, seed, torch_cuda):
     r = rng.standard_normal(u_o.size)
     z_o = orc.bj_apply(r)
     z_d = dev.bj_apply(torch_cuda.tensor(r, device="cuda")).cpu().numpy()
-    # partial-pivot LU of the same blocks: solve-level agreement (the pivot
-    # sequence and the factor rounding are not bit-identical to a blocked LU)
-    assert rel(z_d, z_o) < 1e-11, "block-Jacobi apply"
-    # the apply inverts the assembled block: B z = r per element
     m = orc.m
-    for row in [0, orc.cells - 1]:
+    eps = np.finfo(np.float64).eps
+    # Backward error of every block solve against the oracle's assembled block
+    # (DGOperator::block): both LUs are backward stable, so B z = r holds to
+    # round-off relative to ||B|| ||z|| for every element.
+    worst_backward = 0.0
+    for row in range(orc.cells):
         B = orc.dg_block(row, row)
-        resid = B @ z_d[row * m:(row + 1) * m] - r[row * m:(row + 1) * m]
-        assert np.linalg.norm(resid) <= 1e-11 * np.linalg.norm(r[row * m:(row + 1) * m])
-    # GMRES(20) on J dx = -r(u) to 1e-12
+        zb = z_d[row * m:(row + 1) * m]
+        res = B @ zb - r[row * m:(row + 1) * m]
+        worst_backward = max(worst_backward, np.linalg.norm(res) / (np.linalg.norm(B) * np.linalg.norm(zb)))
+    assert worst_backward < 1e-14, worst_backward
+    # Forward agreement with the oracle's partial-pivot LU apply: two backward
+    # stable solves differ by at most ~kappa(B) * eps per block (the blocked
+    # device LU and the unblocked oracle LU round differently).
+    for row in range(0, orc.cells, max(1, orc.cells // 16)):
+        B = orc.dg_block(row, row)
+        kappa = np.linalg.cond(B)
+        zb_d, zb_o = z_d[row * m:(row + 1) * m], z_o[row * m:(row + 1) * m]
+        assert rel(zb_d, zb_o) <= max(1e-13, 50 * eps * kappa), (row, rel(zb_d, zb_o), kappa)
+    # GMRES(m) on J dx = -r(u) to ltol = 1e-12 (gmres.cpp:17-86): the device
+    # solution satisfies the oracle's own linear system to the same tolerance,
+    # and the iteration counts agree.
     rhs = -orc.dg_residual(u_o)
     x_o, its_o, rel_o, _ = orc.dg_gmres(rhs, 30, 1500, 1e-12)
     x_d, its_d, rel_d, res_d = dev.dg_gmres(torch_cuda.tensor(rhs, device="cuda"), 1500, 1e-12)
     x_d = x_d.cpu().numpy()
     assert rel_d <= 1e-12 and rel_o <= 1e-12
-    assert abs(its_d - its_o) <= 2, (its_d, its_o)
-    assert rel(x_d, x_o) < 1e-9, "GMRES solution"
+    assert abs(its_d - its_o) <= max(2, its_o // 20), (its_d, its_o)
+    w = 1.0 / np.asarray(mass_vector(orc))
+    true_res = orc.dg_apply(x_d) - rhs
+    assert np.sqrt(np.sum(true_res ** 2 * w) / np.sum(rhs ** 2 * w)) <= 2e-12
+
+
+def mass_vector(orc):
+    """Diagonal DG mass M = hx hy w_i w_j / 4 per node (discretization.cpp:13-18)."""
+    w = orc.basis()["w"]
+    n = orc.n
+    node = np.array([0.25 * orc.hx * orc.hy * w[k % n] * w[k // n] for k in range(n * n)])
+    return np.tile(node, 4 * orc.cells)
