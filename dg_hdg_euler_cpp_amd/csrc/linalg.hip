@@ -179,36 +179,58 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ v
 }
 
 // -------------------------------------------------------------- batched LU
-// Partial-pivot LU of one column-major M x M block per workgroup (Eigen
-// PartialPivLU semantics, block_jacobi.hpp:12-23: max |a| with the first index
-// on ties, LAPACK row interchanges across the whole row, unit lower L).
-// Right-looking with NB-column panels factored in LDS; the trailing update
-// streams through L2. rowsrc[i] = original row moved to row i.
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float readlane_d(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Partial-pivot LU of one column-major M x M block per workgroup with Eigen
+// PartialPivLU / LAPACK semantics (block_jacobi.hpp:12-23): pivot = max |a| in
+// the current column (first row in the current order on ties), rows
+// interchanged across the whole matrix, unit lower L. rowsrc[i] = original row
+// that ends at row i, so (P b)[i] = b[rowsrc[i]].
+//
+// Right-looking with NB-column panels. Almost every row of these blocks is
+// pivoted, so row interchanges are never applied as strided row swaps:
+//  * the panel (rows k0..M-1) is factorised in LDS;
+//  * the panel's interchanges, the U12 triangular solve and the trailing update
+//    are one pass over each trailing column: a wavefront gathers the column's
+//    rows through the panel's row permutation, solves the NB-row U12 segment
+//    in registers (readlane broadcasts) and writes the updated column back;
+//  * interchanges of later panels are applied to the finished L columns in one
+//    final gather pass.
 template <int M, int NB, typename T>
 __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__restrict__ rowsrc, int nblocks,
                                               int *flag) {
-    __shared__ T panel[NB * M];       // [c][r], rows k0..M-1 used
-    __shared__ T u12[NB * M];         // [c][j], columns k0+NB..M-1 used
-    __shared__ int perm[M];
-    __shared__ int piv[NB];
+    constexpr int NA = (M + 63) / 64;  // row chunks per lane
+    __shared__ T panel[NB * M];         // [c][r], rows k0..M-1 of the current panel
+    __shared__ int perm[M];             // original row held by each logical row
+    __shared__ int piv[M];              // row interchanged with row k at step k
+    __shared__ int sigma[M];            // gather map of the current panel's interchanges
+    __shared__ int tmp[M];
     __shared__ double red_val[BLOCK / 64];
     __shared__ int red_idx[BLOCK / 64];
     __shared__ int bad;
     const int blk = blockIdx.x;
     T *A = blocks + static_cast<long long>(blk) * M * M;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const double tiny = sizeof(T) == 8 ? DBL_MIN : FLT_MIN;
     for (int r = threadIdx.x; r < M; r += BLOCK) perm[r] = r;
     if (threadIdx.x == 0) bad = 0;
     __syncthreads();
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     for (int k0 = 0; k0 < M; k0 += NB) {
-        const int nb = min(NB, M - k0);
-        // 1. load the panel
-        for (int idx = threadIdx.x; idx < nb * (M - k0); idx += BLOCK) {
-            const int c = idx / (M - k0), r = k0 + idx % (M - k0);
+        const int nb = min(NB, M - k0), rows = M - k0;
+        // (a) panel columns k0..k0+nb-1, rows k0..M-1 (already in the current row order)
+        for (int idx = threadIdx.x; idx < nb * rows; idx += BLOCK) {
+            const int c = idx / rows, r = k0 + idx % rows;
             panel[c * M + r] = A[static_cast<long long>(k0 + c) * M + r];
         }
         __syncthreads();
-        // 2. unblocked factorisation of the panel
+        // (b) unblocked factorisation of the panel
         for (int c = 0; c < nb; ++c) {
             const int col = k0 + c;
             double best = -1.0;
@@ -231,13 +253,11 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
                 for (int w = 1; w < BLOCK / 64; ++w)
                     if (red_val[w] > b || (red_val[w] == b && red_idx[w] < bi)) { b = red_val[w]; bi = red_idx[w]; }
                 if (bi >= M) bi = col;  // all-NaN column: keep the diagonal, flagged below
-                piv[c] = bi;
-                if (bi != col) {
-                    const int t = perm[col]; perm[col] = perm[bi]; perm[bi] = t;
-                }
+                piv[col] = bi;
+                if (bi != col) { const int t = perm[col]; perm[col] = perm[bi]; perm[bi] = t; }
             }
             __syncthreads();
-            const int p = piv[c];
+            const int p = piv[col];
             if (p != col)
                 for (int cc = threadIdx.x; cc < nb; cc += BLOCK) {
                     const T t = panel[cc * M + col];
@@ -255,59 +275,95 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
             }
             __syncthreads();
         }
-        // 3. write the panel back (final L columns and U rows of the panel)
-        for (int idx = threadIdx.x; idx < nb * (M - k0); idx += BLOCK) {
-            const int c = idx / (M - k0), r = k0 + idx % (M - k0);
+        // (c) panel write-back; (d) gather map of the panel's interchanges
+        for (int idx = threadIdx.x; idx < nb * rows; idx += BLOCK) {
+            const int c = idx / rows, r = k0 + idx % rows;
             const T val = panel[c * M + r];
-            if (!isfinite(static_cast<double>(val)) || (r == k0 + c && !(fabs(static_cast<double>(val)) > (sizeof(T) == 8 ? DBL_MIN : FLT_MIN))))
+            if (!isfinite(static_cast<double>(val)) || (r == k0 + c && !(fabs(static_cast<double>(val)) > tiny)))
                 bad = 1;
             A[static_cast<long long>(k0 + c) * M + r] = val;
         }
-        // 4. row interchanges on the other columns; forward substitution with
-        //    the unit lower L11 for the U12 block row.
-        for (int j = threadIdx.x; j < M; j += BLOCK) {
-            if (j >= k0 && j < k0 + nb) continue;
-            T *colp = A + static_cast<long long>(j) * M;
+        if (threadIdx.x == 0) {
+            for (int r = k0; r < M; ++r) sigma[r] = r;
             for (int c = 0; c < nb; ++c) {
-                const int p = piv[c];
-                if (p != k0 + c) {
-                    const T t = colp[k0 + c]; colp[k0 + c] = colp[p]; colp[p] = t;
-                }
-            }
-            if (j >= k0 + nb) {
-                T x[NB];
-#pragma unroll
-                for (int c = 0; c < NB; ++c) x[c] = c < nb ? colp[k0 + c] : T(0);
-#pragma unroll
-                for (int c = 0; c < NB; ++c) {
-                    const T xc = x[c];
-#pragma unroll
-                    for (int r = c + 1; r < NB; ++r)
-                        if (r < nb) x[r] -= panel[c * M + k0 + r] * xc;
-                }
-#pragma unroll
-                for (int c = 0; c < NB; ++c)
-                    if (c < nb) {
-                        if (!isfinite(static_cast<double>(x[c]))) bad = 1;
-                        colp[k0 + c] = x[c];
-                        u12[c * M + j] = x[c];
-                    }
+                const int a = k0 + c, b = piv[a];
+                const int t = sigma[a]; sigma[a] = sigma[b]; sigma[b] = t;
             }
         }
         __syncthreads();
-        // 5. trailing update A22 -= L21 U12, one wave per column stripe
-        const int r0 = k0 + nb;
-        if (r0 < M) {
-            for (int j = r0 + wave; j < M; j += BLOCK / 64) {
-                T *colp = A + static_cast<long long>(j) * M;
-                for (int r = r0 + lane; r < M; r += 64) {
-                    T acc = colp[r];
+        // (e) per trailing column j: gather rows through sigma, U12 segment by
+        //     forward substitution with the unit L11, A22 -= L21 U12, write back.
+        for (int j = k0 + nb + wave; j < M; j += BLOCK / 64) {
+            T *colp = A + static_cast<long long>(j) * M;
+            T x[NA];
 #pragma unroll
-                    for (int c = 0; c < NB; ++c)
-                        if (c < nb) acc -= panel[c * M + r] * u12[c * M + j];
-                    colp[r] = acc;
+            for (int a = 0; a < NA; ++a) {
+                const int row = k0 + a * 64 + lane;
+                x[a] = row < M ? colp[sigma[row]] : T(0);
+            }
+            T u[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                u[c] = T(0);
+                if (c < nb) {
+                    u[c] = readlane_d(x[0], c);
+                    if (lane > c && lane < nb) x[0] -= panel[c * M + k0 + lane] * u[c];
                 }
             }
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const int row = k0 + a * 64 + lane;
+                if (row >= k0 + nb && row < M) {
+                    T acc = x[a];
+#pragma unroll
+                    for (int c = 0; c < NB; ++c)
+                        if (c < nb) acc -= panel[c * M + row] * u[c];
+                    x[a] = acc;
+                }
+            }
+            if (lane < nb && !isfinite(static_cast<double>(x[0]))) bad = 1;
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const int row = k0 + a * 64 + lane;
+                if (row < M) colp[row] = x[a];
+            }
+        }
+        __syncthreads();
+    }
+    // (f) apply the interchanges of later panels to the finished L columns:
+    //     tau = gather map of all interchanges after the panel, built backwards.
+    for (int r = threadIdx.x; r < M; r += BLOCK) sigma[r] = r;
+    __syncthreads();
+    const int npanels = (M + NB - 1) / NB;
+    for (int q = npanels - 1; q >= 0; --q) {
+        const int k0 = q * NB, nb = min(NB, M - k0), lo = k0 + nb;
+        if (q < npanels - 1)
+            for (int c = k0 + wave; c < k0 + nb; c += BLOCK / 64) {
+                T *colp = A + static_cast<long long>(c) * M;
+                T x[NA];
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                    const int row = lo + a * 64 + lane;
+                    x[a] = row < M ? colp[sigma[row]] : T(0);
+                }
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                    const int row = lo + a * 64 + lane;
+                    if (row < M) colp[row] = x[a];
+                }
+            }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int r = k0; r < M; ++r) tmp[r] = r;
+            for (int c = 0; c < nb; ++c) {
+                const int a = k0 + c, b = piv[a];
+                const int t = tmp[a]; tmp[a] = tmp[b]; tmp[b] = t;
+            }
+        }
+        __syncthreads();
+        for (int r = k0 + threadIdx.x; r < M; r += BLOCK) {
+            const int src = sigma[r];
+            sigma[r] = tmp[src];
         }
         __syncthreads();
     }
@@ -315,64 +371,90 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
     if (threadIdx.x == 0 && bad) atomicOr(flag, 8);
 }
 
-__device__ __forceinline__ double readlane_d(double v, int lane) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
 
 // block_jacobi.hpp:24-39: z = U^-1 L^-1 P r per block; one wavefront per block,
-// lane l owns rows l, l+64, ...; the pivot value of each column is broadcast
-// with readlane, the factor columns stream once from HBM (column-major, so each
-// column is one coalesced load).
+// lane l owns rows l, l+64, ...; the value of each pivot row is broadcast with
+// readlane and the factor columns stream once from HBM (column-major: one
+// coalesced load per column and row chunk). The loads of column j+1 are issued
+// before the FMAs of column j so every wave keeps two columns in flight.
 template <int M, typename T>
 __global__ __launch_bounds__(BLOCK) void k_bj_apply(const T *__restrict__ factors, const int *__restrict__ rowsrc,
                                                     const double *__restrict__ r, double *__restrict__ z,
                                                     int nblocks) {
-    constexpr int A = (M + 63) / 64;
+    constexpr int NA = (M + 63) / 64;
     const int lane = threadIdx.x % 64;
     const long long blk = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + threadIdx.x / 64;
     if (blk >= nblocks) return;
     const T *F = factors + blk * M * M;
-    double x[A];
+    double x[NA];
 #pragma unroll
-    for (int a = 0; a < A; ++a) {
+    for (int a = 0; a < NA; ++a) {
         const int row = a * 64 + lane;
         x[a] = row < M ? r[blk * M + rowsrc[blk * M + row]] : 0.0;
     }
-    // forward substitution with unit lower L
+    // forward substitution with the unit lower L: x_i -= L_ij x_j for i > j
+    T cur[NA], nxt[NA];
 #pragma unroll
-    for (int ja = 0; ja < A; ++ja) {
+    for (int a = 0; a < NA; ++a) {
+        const int row = a * 64 + lane;
+        nxt[a] = (row > 0 && row < M) ? F[row] : T(0);
+    }
+#pragma unroll
+    for (int ja = 0; ja < NA; ++ja) {
         const int jend = min(64, M - ja * 64);
         for (int jl = 0; jl < jend; ++jl) {
             const int j = ja * 64 + jl;
-            const double xj = readlane_d(x[ja], jl);
-            const T *colp = F + static_cast<long long>(j) * M;
 #pragma unroll
-            for (int a = ja; a < A; ++a) {
-                const int row = a * 64 + lane;
-                if (row > j && row < M) x[a] -= static_cast<double>(colp[row]) * xj;
+            for (int a = ja; a < NA; ++a) cur[a] = nxt[a];
+            if (j + 1 < M) {
+                const T *colp = F + static_cast<long long>(j + 1) * M;
+#pragma unroll
+                for (int a = ja; a < NA; ++a) {
+                    const int row = a * 64 + lane;
+                    nxt[a] = (row > j + 1 && row < M) ? colp[row] : T(0);
+                }
             }
+            const double xj = readlane_d(x[ja], jl);
+#pragma unroll
+            for (int a = ja; a < NA; ++a) x[a] -= static_cast<double>(cur[a]) * xj;
         }
     }
-    // backward substitution with U
+    // backward substitution with U: x_j /= U_jj, x_i -= U_ij x_j for i < j
+    {
+        const T *colp = F + static_cast<long long>(M - 1) * M;
 #pragma unroll
-    for (int ja = A - 1; ja >= 0; --ja) {
+        for (int a = 0; a < NA; ++a) {
+            const int row = a * 64 + lane;
+            nxt[a] = row < M ? colp[row] : T(0);
+        }
+    }
+#pragma unroll
+    for (int ja = NA - 1; ja >= 0; --ja) {
         const int jend = min(64, M - ja * 64);
         for (int jl = jend - 1; jl >= 0; --jl) {
             const int j = ja * 64 + jl;
-            const T *colp = F + static_cast<long long>(j) * M;
-            const double xj = readlane_d(x[ja], jl) / static_cast<double>(colp[j]);
+#pragma unroll
+            for (int a = 0; a <= ja; ++a) cur[a] = nxt[a];
+            if (j > 0) {
+                const T *colp = F + static_cast<long long>(j - 1) * M;
+#pragma unroll
+                for (int a = 0; a <= ja; ++a) {
+                    const int row = a * 64 + lane;
+                    nxt[a] = row <= j - 1 ? colp[row] : T(0);
+                }
+            }
+            const double diag = readlane_d(static_cast<double>(cur[ja]), jl);
+            const double xj = readlane_d(x[ja], jl) / diag;
             if (lane == jl) x[ja] = xj;
 #pragma unroll
             for (int a = 0; a <= ja; ++a) {
                 const int row = a * 64 + lane;
-                if (row < j) x[a] -= static_cast<double>(colp[row]) * xj;
+                if (row < j) x[a] -= static_cast<double>(cur[a]) * xj;
             }
         }
     }
 #pragma unroll
-    for (int a = 0; a < A; ++a) {
+    for (int a = 0; a < NA; ++a) {
         const int row = a * 64 + lane;
         if (row < M) z[blk * M + row] = x[a];
     }
