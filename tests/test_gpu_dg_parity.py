@@ -108,9 +108,9 @@ def test_block_jacobi_and_gmres(name, args, seed, torch_cuda):
     u = torch_cuda.tensor(u_o, device="cuda")
     rate = orc.freeze(u_o)
     dev.freeze(u)
-    # i.i.d. seeded k=8 states at CFL 5 exceed 1500 GMRES(30) iterations with
-    # block Jacobi (oracle and device alike); the seeded k=8 case runs at CFL 1.
-    tau = (1.0 if (seed is not None and "8" in args[args.index("--k") + 1]) else 5.0) / rate
+    # i.i.d. seeded k=8 states stall GMRES(30) + block Jacobi beyond 3000
+    # iterations at CFL >= 0.3 (oracle and device alike); that case runs at CFL 0.1.
+    tau = (0.1 if (seed is not None and args[args.index("--k") + 1] == "8") else 5.0) / rate
     orc.set_step(tau, u_o)
     dev.set_step(tau, u)
     orc.dg_refresh(u_o)
