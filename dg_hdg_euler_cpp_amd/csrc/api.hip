@@ -169,6 +169,12 @@ struct dgh_ctx {
     DevBuf<int> rowsrc;
     // GMRES
     DevBuf<double> kbasis, kw, kz, kwork, kres, ky;
+    // mixed precision: FP32 / BF16 factors, FP32 inner Krylov space, FGMRES Z
+    DevBuf<float> factors_f32, ibasis, iw, it, is, ires, ib, zbasis;
+    DevBuf<unsigned short> factors_bf16;
+    DevBuf<int> rowsrc32;
+    DevBuf<double> check_dir;
+    bool bj32_ready = false;
     // Newton
     DevBuf<double> state, trial, residual, trial_residual, direction, rhs, equation_error, step_state, stage_base,
         slopes;
@@ -223,8 +229,12 @@ struct dgh_ctx {
         DevBuf<double> *bufs[] = {&old, &speeds, &tr_u, &tr_x, &fq, &gtr, &vcache, &fcache, &partials, &scalars,
                                   &factors, &kbasis, &kw, &kz, &kwork, &kres, &ky, &state, &trial, &residual,
                                   &trial_residual, &direction, &rhs, &equation_error, &step_state, &stage_base,
-                                  &slopes};
+                                  &slopes, &check_dir};
         for (auto *b : bufs) b->release(bytes);
+        DevBuf<float> *fbufs[] = {&factors_f32, &ibasis, &iw, &it, &is, &ires, &ib, &zbasis};
+        for (auto *b : fbufs) b->release(bytes);
+        factors_bf16.release(bytes);
+        rowsrc32.release(bytes);
         factors32.release(bytes);
         rowsrc.release(bytes);
         flags.release(bytes);
@@ -352,6 +362,7 @@ struct dgh_ctx {
         HIPCHECK(launch_face_cache(n, tr_u.p, speeds.p, fcache.p, basis.p, geo, stream));
         refreshed = true;
         bj_ready = false;
+        bj32_ready = false;
     }
     void dg_apply(const double *x, double *y) {
         require(refreshed, "Refresh the volume Jacobian before applying it");
@@ -472,6 +483,235 @@ struct dgh_ctx {
     }
     void scale_div(const double *src, double d, double *dst) { HIPCHECK(launch_scale_div(src, d, dst, D, stream)); }
 
+    // ---- mixed precision: FP32 operator, FP32/BF16 block Jacobi, FP32 inner
+    //      GMRES inside FP64 flexible GMRES (no reference counterpart)
+    void bj_refresh_mixed() {
+        require(refreshed, "Refresh the volume Jacobian before assembling a block");
+        factors_f32.alloc(static_cast<size_t>(cells) * m * m, bytes);
+        rowsrc32.alloc(static_cast<size_t>(cells) * m, bytes);
+        set_flag(2, 0);
+        timed("assemble", [&] {
+            HIPCHECK(launch_assemble_blocks(n, vcache.p, fcache.p, factors_f32.p, inv_step, basis.p, geo, 0, cells,
+                                            stream));
+        });
+        timed("lu", [&] { HIPCHECK(launch_lu(m, factors_f32.p, rowsrc32.p, cells, flags.p + 2, stream)); });
+        require(read_flag(2) == 0, "Singular block Jacobi factor");
+        if (cfg.precision == DGH_PRECISION_MIXED_BF16) {
+            factors_bf16.alloc(static_cast<size_t>(cells) * m * m, bytes);
+            HIPCHECK(launch_to_bf16(factors_f32.p, factors_bf16.p, static_cast<long long>(cells) * m * m, stream));
+        }
+        bj32_ready = true;
+    }
+    void bj_apply32(const float *r, float *z) {
+        require(bj32_ready, "Refresh the block Jacobi preconditioner first");
+        timed("bj_apply", [&] {
+            if (cfg.precision == DGH_PRECISION_MIXED_BF16)
+                HIPCHECK(launch_bj_apply_bf16(m, factors_bf16.p, rowsrc32.p, r, z, cells, stream));
+            else
+                HIPCHECK(launch_bj_apply(m, factors_f32.p, rowsrc32.p, r, z, cells, stream));
+        });
+    }
+    void dg_apply32(const float *x, float *y) {
+        require(refreshed, "Refresh the volume Jacobian before applying it");
+        timed("jv", [&] {
+            HIPCHECK(launch_restrict(n, x, tr_x.p, basis.p, geo, stream));
+            HIPCHECK(launch_jv_flux(n, tr_x.p, fcache.p, fq.p, geo, stream));
+            HIPCHECK(launch_element_jv(n, x, vcache.p, fq.p, y, inv_step, basis.p, geo, stream));
+        });
+    }
+    void dg_apply_mixed(const float *x, double *y) {
+        require(refreshed, "Refresh the volume Jacobian before applying it");
+        timed("jv", [&] {
+            HIPCHECK(launch_restrict(n, x, tr_x.p, basis.p, geo, stream));
+            HIPCHECK(launch_jv_flux(n, tr_x.p, fcache.p, fq.p, geo, stream));
+            HIPCHECK(launch_element_jv(n, x, vcache.p, fq.p, y, inv_step, basis.p, geo, stream));
+        });
+    }
+    double dot32(const float *a) {
+        const float *v[1] = {a};
+        HIPCHECK(launch_multi_dot(n, v, 1, a, nullptr, D, partials.p, basis.p, stream));
+        HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 1, scalars.p + 200, stream));
+        double out = 0;
+        HIPCHECK(hipMemcpyAsync(&out, scalars.p + 200, sizeof(double), hipMemcpyDeviceToHost, stream));
+        sync();
+        return std::sqrt(std::max(0.0, out));
+    }
+    // Right-preconditioned FP32 GMRES(inner_restart) on J z = b with the FP32
+    // block Jacobi, the M^-1 metric and FP64 dot accumulation; stops at the
+    // Givens residual estimate tol * ||b|| or after max_its iterations.
+    int gmres_inner(const float *b, float *z, double tol, int max_its) {
+        const int restart = std::min<int>(cfg.inner_restart, static_cast<int>(std::min<long long>(D, 31)));
+        ibasis.alloc(static_cast<size_t>(restart + 1) * D, bytes);
+        iw.alloc(D, bytes);
+        it.alloc(D, bytes);
+        is.alloc(D, bytes);
+        ires.alloc(D, bytes);
+        std::vector<const float *> V(restart + 1);
+        for (int k = 0; k <= restart; ++k) V[k] = ibasis.p + static_cast<size_t>(k) * D;
+        auto Vk = [&](int k) { return ibasis.p + static_cast<size_t>(k) * D; };
+        HIPCHECK(hipMemsetAsync(z, 0, D * sizeof(float), stream));
+        HIPCHECK(hipMemcpyAsync(ires.p, b, D * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        const double rhs_norm = dot32(b);
+        if (rhs_norm == 0) return 0;
+        std::vector<double> H((restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), y(restart);
+        auto Hm = [&](int i, int j) -> double & { return H[i + (restart + 1) * j]; };
+        int iterations = 0;
+        double host_scalars[65];
+        while (iterations < max_its) {
+            const double beta = dot32(ires.p);
+            if (beta <= tol * rhs_norm) break;
+            HIPCHECK(launch_scale_div(ires.p, beta, Vk(0), D, stream));
+            std::fill(H.begin(), H.end(), 0.0);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            int count = 0;
+            bool done = false;
+            for (int j = 0; j < restart && iterations < max_its; ++j) {
+                bj_apply32(Vk(j), it.p);
+                dg_apply32(it.p, iw.p);
+                timed("gmres_ortho", [&] {
+                    HIPCHECK(launch_multi_dot(n, V.data(), j + 1, iw.p, nullptr, D, partials.p, basis.p, stream));
+                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p, stream));
+                    HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p, iw.p, nullptr, D, partials.p, 0,
+                                               basis.p, stream));
+                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p + 32, stream));
+                    HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p + 32, iw.p, nullptr, D, partials.p, 1,
+                                               basis.p, stream));
+                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 1, scalars.p + 64, stream));
+                    HIPCHECK(hipMemcpyAsync(host_scalars, scalars.p, 65 * sizeof(double), hipMemcpyDeviceToHost,
+                                            stream));
+                    sync();
+                });
+                for (int k = 0; k <= j; ++k) Hm(k, j) = (0.0 + host_scalars[k]) + host_scalars[32 + k];
+                const double next = std::sqrt(std::max(0.0, host_scalars[64]));
+                Hm(j + 1, j) = next;
+                if (next > 0) HIPCHECK(launch_scale_div(iw.p, next, Vk(j + 1), D, stream));
+                for (int i = 0; i < j; ++i) {
+                    const double a = Hm(i, j), bb = Hm(i + 1, j);
+                    Hm(i, j) = cs[i] * a + sn[i] * bb;
+                    Hm(i + 1, j) = -sn[i] * a + cs[i] * bb;
+                }
+                const double d = std::hypot(Hm(j, j), Hm(j + 1, j));
+                require(std::isfinite(d) && d > 0, "GMRES Arnoldi breakdown");
+                cs[j] = Hm(j, j) / d;
+                sn[j] = Hm(j + 1, j) / d;
+                Hm(j, j) = d;
+                Hm(j + 1, j) = 0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] *= cs[j];
+                ++iterations;
+                count = j + 1;
+                if (std::abs(g[j + 1]) <= tol * rhs_norm || next <= std::numeric_limits<double>::min()) {
+                    done = true;
+                    break;
+                }
+            }
+            for (int i = count - 1; i >= 0; --i) {
+                double value = g[i];
+                for (int k = i + 1; k < count; ++k) value -= Hm(i, k) * y[k];
+                y[i] = value / Hm(i, i);
+            }
+            HIPCHECK(hipMemcpyAsync(ky.p, y.data(), count * sizeof(double), hipMemcpyHostToDevice, stream));
+            HIPCHECK(launch_combine(V.data(), count, ky.p, it.p, D, stream));
+            bj_apply32(it.p, is.p);
+            HIPCHECK(launch_axpby(1.0, z, 1.0, is.p, z, D, stream));
+            if (done || iterations >= max_its) break;
+            dg_apply32(z, iw.p);
+            HIPCHECK(launch_axpby(1.0, b, -1.0, iw.p, ires.p, D, stream));
+        }
+        return iterations;
+    }
+    // Flexible GMRES(restart) in FP64 (weighted CGS2, Givens, true residual
+    // after each cycle as gmres.cpp:17-86) whose variable right
+    // preconditioner is one FP32 inner solve per outer iteration.
+    Linear fgmres(const double *rhs, double *x, int limit, double tolerance, int &outer) {
+        const int restart = static_cast<int>(std::min<long long>(D, cfg.restart));
+        require(restart + 1 <= 32, "GMRES restart above 31 is not supported on the device");
+        kbasis.alloc(static_cast<size_t>(restart + 1) * D, bytes);
+        zbasis.alloc(static_cast<size_t>(restart) * D, bytes);
+        kw.alloc(D, bytes);
+        kwork.alloc(D, bytes);
+        kres.alloc(D, bytes);
+        ky.alloc(64, bytes);
+        ib.alloc(D, bytes);
+        std::vector<const double *> V(restart + 1);
+        std::vector<const float *> Z(restart);
+        for (int k = 0; k <= restart; ++k) V[k] = kbasis.p + static_cast<size_t>(k) * D;
+        for (int k = 0; k < restart; ++k) Z[k] = zbasis.p + static_cast<size_t>(k) * D;
+        auto Vk = [&](int k) { return kbasis.p + static_cast<size_t>(k) * D; };
+        auto Zk = [&](int k) { return zbasis.p + static_cast<size_t>(k) * D; };
+        HIPCHECK(hipMemsetAsync(x, 0, D * sizeof(double), stream));
+        copy(kres.p, rhs, D);
+        outer = 0;
+        const double rhs_norm = gmres_norm(rhs);
+        if (rhs_norm == 0) return {0, 0};
+        std::vector<double> H((restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), y(restart);
+        auto Hm = [&](int i, int j) -> double & { return H[i + (restart + 1) * j]; };
+        int inner_total = 0;
+        double host_scalars[65];
+        while (inner_total < limit) {
+            const double beta = gmres_norm(kres.p);
+            if (beta <= tolerance * rhs_norm) return {inner_total, beta / rhs_norm};
+            scale_div(kres.p, beta, Vk(0));
+            std::fill(H.begin(), H.end(), 0.0);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            int count = 0;
+            for (int j = 0; j < restart && inner_total < limit; ++j) {
+                HIPCHECK(launch_scale_div(Vk(j), 1.0, ib.p, D, stream));
+                inner_total += gmres_inner(ib.p, Zk(j), cfg.inner_tolerance,
+                                           std::max(1, std::min(cfg.inner_max, limit - inner_total)));
+                dg_apply_mixed(Zk(j), kw.p);
+                timed("gmres_ortho", [&] {
+                    HIPCHECK(launch_multi_dot(n, V.data(), j + 1, kw.p, nullptr, D, partials.p, basis.p, stream));
+                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p, stream));
+                    HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p, kw.p, nullptr, D, partials.p, 0,
+                                               basis.p, stream));
+                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p + 32, stream));
+                    HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p + 32, kw.p, nullptr, D, partials.p, 1,
+                                               basis.p, stream));
+                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 1, scalars.p + 64, stream));
+                    HIPCHECK(hipMemcpyAsync(host_scalars, scalars.p, 65 * sizeof(double), hipMemcpyDeviceToHost,
+                                            stream));
+                    sync();
+                });
+                for (int k = 0; k <= j; ++k) Hm(k, j) = (0.0 + host_scalars[k]) + host_scalars[32 + k];
+                const double next = std::sqrt(std::max(0.0, host_scalars[64]));
+                Hm(j + 1, j) = next;
+                if (next > 0) scale_div(kw.p, next, Vk(j + 1));
+                for (int i = 0; i < j; ++i) {
+                    const double a = Hm(i, j), b = Hm(i + 1, j);
+                    Hm(i, j) = cs[i] * a + sn[i] * b;
+                    Hm(i + 1, j) = -sn[i] * a + cs[i] * b;
+                }
+                const double d = std::hypot(Hm(j, j), Hm(j + 1, j));
+                require(std::isfinite(d) && d > 0, "GMRES Arnoldi breakdown");
+                cs[j] = Hm(j, j) / d;
+                sn[j] = Hm(j + 1, j) / d;
+                Hm(j, j) = d;
+                Hm(j + 1, j) = 0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] *= cs[j];
+                ++outer;
+                count = j + 1;
+                if (std::abs(g[j + 1]) <= tolerance * rhs_norm || next <= std::numeric_limits<double>::min()) break;
+            }
+            for (int i = count - 1; i >= 0; --i) {
+                double value = g[i];
+                for (int k = i + 1; k < count; ++k) value -= Hm(i, k) * y[k];
+                y[i] = value / Hm(i, i);
+            }
+            HIPCHECK(hipMemcpyAsync(ky.p, y.data(), count * sizeof(double), hipMemcpyHostToDevice, stream));
+            HIPCHECK(launch_combine_add(Z.data(), count, ky.p, x, D, stream));
+            dg_apply(x, kwork.p);
+            axpby(1.0, rhs, -1.0, kwork.p, kres.p, D);
+            const double relative = gmres_norm(kres.p) / rhs_norm;
+            require(std::isfinite(relative), "Nonfinite GMRES residual");
+            if (relative <= tolerance) return {inner_total, relative};
+        }
+        throw std::runtime_error("GMRES iteration limit reached");
+    }
+
     // ---- TimeIntegrator::advance (time_integrator.cpp:48-213), DG method
     void advance(double *solution, dgh_run_stats &stats, std::vector<dgh_update_stats> &updates) {
         const auto &c = cfg;
@@ -543,9 +783,25 @@ struct dgh_ctx {
                     const double target = std::max(10 * c.linear_tolerance * norm, allowance);
                     dg_refresh(state.p);
                     axpby(-1.0, residual.p, 0.0, nullptr, rhs.p, D);
-                    bj_refresh();
-                    const Linear linear = gmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance);
+                    Linear linear{0, 0};
+                    int outer = 0;
+                    if (c.precision == DGH_PRECISION_FP64) {
+                        bj_refresh();
+                        linear = gmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance);
+                    } else {
+                        bj_refresh_mixed();
+                        linear = fgmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance, outer);
+                    }
                     axpby(-1.0, kres.p, 0.0, nullptr, equation_error.p, D);
+                    double precision_error = -1;
+                    if (c.precision != DGH_PRECISION_FP64 && c.check_update) {
+                        // the same Newton system through the FP64 path
+                        check_dir.alloc(D, bytes);
+                        bj_refresh();
+                        gmres(rhs.p, check_dir.p, c.max_linear, c.linear_tolerance);
+                        axpby(1.0, direction.p, -1.0, check_dir.p, kwork.p, D);
+                        precision_error = state_norm(kwork.p) / state_norm(check_dir.p);
+                    }
                     const double error_norm = dual_norm(equation_error.p);
                     require(error_norm <= target, "DG Newton linear residual too large");
                     double damping = 1.;
@@ -591,7 +847,8 @@ struct dgh_ctx {
                     u.dg_linear_relative_residual = dgrel;
                     u.iteration_s = duration;
                     u.inner_iterations = linear.iterations;
-                    u.outer_iterations = 0;
+                    u.outer_iterations = outer;
+                    u.precision_error = precision_error;
                     updates.push_back(u);
                 }
                 require(converged || c.newton_steps > 0, "Newton iteration limit reached");
@@ -819,6 +1076,39 @@ int dgh_bj_apply(dgh_ctx *ctx, const double *r, double *z) {
         check_ctx(ctx);
         ctx->bj_apply(r, z);
         ctx->sync();
+    });
+}
+int dgh_bj_refresh_mixed(dgh_ctx *ctx) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->bj_refresh_mixed();
+        ctx->sync();
+    });
+}
+int dgh_bj_apply_f32(dgh_ctx *ctx, const float *r, float *z) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->bj_apply32(r, z);
+        ctx->sync();
+    });
+}
+int dgh_dg_apply_f32(dgh_ctx *ctx, const float *x, float *y) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->dg_apply32(x, y);
+        ctx->sync();
+    });
+}
+int dgh_dg_fgmres(dgh_ctx *ctx, const double *rhs, double *x, int max_iterations, double tolerance, int *iterations,
+                  int *outer, double *relative_residual) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        int o = 0;
+        const auto r = ctx->fgmres(rhs, x, max_iterations, tolerance, o);
+        ctx->sync();
+        if (iterations) *iterations = r.iterations;
+        if (outer) *outer = o;
+        if (relative_residual) *relative_residual = r.relative;
     });
 }
 int dgh_dg_gmres(dgh_ctx *ctx, const double *rhs, double *x, int max_iterations, double tolerance, int *iterations,

@@ -39,6 +39,8 @@ dgh_config default_config() {
     c.testcase = DGH_DIAGONAL;
     c.precision = DGH_PRECISION_FP64;
     c.inner_restart = 30;
+    c.inner_max = 300;
+    c.inner_tolerance = 1e-4;
     c.seed = 2025;
     std::strncpy(c.prefix, "output/run_2d", sizeof(c.prefix) - 1);
     return c;
@@ -59,7 +61,9 @@ void validate_config(const dgh_config &c) {
             "Invalid solver tolerance or wave-speed safety");
     require(c.method >= DGH_METHOD_DG && c.method <= DGH_METHOD_BOTH, "Method must be dg, hdg, or both");
     require(c.precision >= DGH_PRECISION_FP64 && c.precision <= DGH_PRECISION_MIXED_BF16, "Invalid precision");
-    require(c.inner_restart > 0, "Invalid iteration limit");
+    require(c.inner_restart > 0 && c.inner_restart <= 31 && c.inner_max > 0, "Invalid iteration limit");
+    require(std::isfinite(c.inner_tolerance) && c.inner_tolerance > 0 && c.inner_tolerance < 1,
+            "Invalid solver tolerance or wave-speed safety");
 }
 
 dgh_config parse_config(int argc, const char *const *argv) {
@@ -144,6 +148,8 @@ dgh_config parse_config(int argc, const char *const *argv) {
             else if (name == "mixed-bf16") c.precision = DGH_PRECISION_MIXED_BF16;
             else throw std::runtime_error("Invalid precision");
         } else if (option == "--inner-restart") c.inner_restart = integer();
+        else if (option == "--inner-max") c.inner_max = integer();
+        else if (option == "--inner-tol") c.inner_tolerance = real();
         else if (option == "--seed") {
             auto text = value();
             size_t end;
@@ -158,7 +164,8 @@ dgh_config parse_config(int argc, const char *const *argv) {
                          "--ntol R --ltol R --safety S --prefix PATH\n"
                          "--integrator ssp-dirk3|be --newton-steps N --warmup-updates N\n"
                          "--timing-only --check-update --no-save-solution\n"
-                         "--device gpu --precision fp64|mixed|mixed-bf16 --inner-restart N --seed S\n";
+                         "--device gpu --precision fp64|mixed|mixed-bf16 --inner-restart N --inner-max N\n"
+                         "--inner-tol R --seed S\n";
             throw std::runtime_error("help");
         } else
             throw std::runtime_error("Unknown option: " + option);

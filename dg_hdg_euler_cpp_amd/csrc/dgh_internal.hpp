@@ -39,6 +39,7 @@ enum FaceField { F_UL, F_VL, F_HL, F_UR, F_VR, F_HR, F_BL, F_BR, F_PEN, F_TAUL, 
 // All launchers enqueue on `stream` and return hipError_t of the launch.
 // Vectors use the reference layout cell*4n^2 + v*n^2 + (i + n*j).
 hipError_t launch_restrict(int n, const double *x, double *traces, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_restrict(int n, const float *x, double *traces, const Basis *b, const Geo &g, hipStream_t s);
 hipError_t launch_freeze_speeds(int n, const double *traces, double *speeds, int *flag, const Geo &g,
                                 hipStream_t s);
 hipError_t launch_freeze_rate(int n, const double *u, const double *speeds, unsigned long long *rate_bits,
@@ -54,6 +55,10 @@ hipError_t launch_face_cache(int n, const double *traces, const double *speeds, 
 hipError_t launch_jv_flux(int n, const double *traces, const double *fcache, double *flux, const Geo &g,
                           hipStream_t s);
 hipError_t launch_element_jv(int n, const double *x, const double *vcache, const double *flux, double *y,
+                             double inv_step, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_element_jv(int n, const float *x, const double *vcache, const double *flux, float *y,
+                             double inv_step, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_element_jv(int n, const float *x, const double *vcache, const double *flux, double *y,
                              double inv_step, const Basis *b, const Geo &g, hipStream_t s);
 // Adjoint: z = J^T w, volume part and the face-flux adjoint.
 hipError_t launch_adjoint_face(int n, const double *wtraces, const double *fcache, double *gtraces,
@@ -75,30 +80,55 @@ hipError_t launch_integral(int n, const double *u, long long cells, double *part
                            hipStream_t s);
 hipError_t launch_finish_sum(const double *partials, int nparts, int width, double *out, hipStream_t s);
 
-// Block Jacobi (DG element blocks m = 4n^2).
+// Block Jacobi (DG element blocks m = 4n^2, HDG face blocks 4n).
 // Blocks of cells [first, first+count) into blocks[(cell-first) * m*m].
 hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fcache, double *blocks,
                                   double inv_step, const Basis *b, const Geo &g, int first, int count,
                                   hipStream_t s);
+hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fcache, float *blocks,
+                                  double inv_step, const Basis *b, const Geo &g, int first, int count,
+                                  hipStream_t s);
 hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s);
-hipError_t launch_lu_f32(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s);
+hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s);
 hipError_t launch_bj_apply(int m, const double *factors, const int *rowsrc, const double *r, double *z,
                            int nblocks, hipStream_t s);
-hipError_t launch_convert_f32(const double *src, float *dst, long long count, hipStream_t s);
+hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const float *r, float *z, int nblocks,
+                           hipStream_t s);
+// factors: bfloat16 storage (FP32 arithmetic)
+hipError_t launch_bj_apply_bf16(int m, const void *factors, const int *rowsrc, const float *r, float *z,
+                                int nblocks, hipStream_t s);
+hipError_t launch_convert(const double *src, float *dst, long long count, hipStream_t s);
+hipError_t launch_convert(const float *src, double *dst, long long count, hipStream_t s);
+hipError_t launch_to_bf16(const float *src, void *dst, long long count, hipStream_t s);
 
 // GMRES building blocks over vectors of length `count` with the DG metric
-// inv_mass(node) (metric 0) or a plain per-entry weight vector (metric 1).
+// inv_mass(node) (weights == nullptr) or a per-entry weight vector; dot
+// products accumulate in FP64 for both vector types.
 hipError_t launch_multi_dot(int n, const double *const *basis, int nvec, const double *w, const double *weights,
+                            long long count, double *partials, const Basis *b, hipStream_t s);
+hipError_t launch_multi_dot(int n, const float *const *basis, int nvec, const float *w, const double *weights,
                             long long count, double *partials, const Basis *b, hipStream_t s);
 hipError_t launch_update_dot(int n, const double *const *basis, int nvec, const double *h, double *w,
                              const double *weights, long long count, double *partials, int mode,
                              const Basis *b, hipStream_t s);
+hipError_t launch_update_dot(int n, const float *const *basis, int nvec, const double *h, float *w,
+                             const double *weights, long long count, double *partials, int mode,
+                             const Basis *b, hipStream_t s);
 hipError_t launch_combine(const double *const *basis, int nvec, const double *y, double *out, long long count,
                           hipStream_t s);
+hipError_t launch_combine(const float *const *basis, int nvec, const double *y, float *out, long long count,
+                          hipStream_t s);
+// out += sum_k V_k y_k with FP32 basis and FP64 accumulation target
+hipError_t launch_combine_add(const float *const *basis, int nvec, const double *y, double *out, long long count,
+                              hipStream_t s);
 hipError_t launch_axpby(double a, const double *x, double b, const double *y, double *out, long long count,
+                        hipStream_t s);
+hipError_t launch_axpby(double a, const float *x, double b, const float *y, float *out, long long count,
                         hipStream_t s);
 // out = x / d
 hipError_t launch_scale_div(const double *x, double d, double *out, long long count, hipStream_t s);
+hipError_t launch_scale_div(const float *x, double d, float *out, long long count, hipStream_t s);
+hipError_t launch_scale_div(const double *x, double d, float *out, long long count, hipStream_t s);
 // out = (x - y) / d
 hipError_t launch_sub_div(const double *x, const double *y, double d, double *out, long long count, hipStream_t s);
 

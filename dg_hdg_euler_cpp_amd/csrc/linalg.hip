@@ -2,6 +2,8 @@
 // reductions and vector kernels of GMRES.
 #include "dgh_internal.hpp"
 
+#include <hip/hip_bf16.h>
+
 #include <cfloat>
 
 namespace dgh {
@@ -97,9 +99,9 @@ __global__ __launch_bounds__(BLOCK) void k_finish_sum(const double *__restrict__
 // + [j==jy] (-l_jx H0[v][w](q=j)) l_i + (r_jx H1[v][w]) r_i
 // + [i==jx] (-l_jy H2[v][w](q=i)) l_j + (r_jy H3[v][w]) r_j
 // with H_s the HLL derivative of the face on side s (face_jacobian.cpp:69-73).
-template <int N>
+template <int N, typename T>
 __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ vc, const double *__restrict__ fc,
-                                                    double *__restrict__ blocks, double inv_step,
+                                                    T *__restrict__ blocks, double inv_step,
                                                     const Basis *__restrict__ B, Geo g, int first) {
     constexpr int P = N * N, M = 4 * P;
     __shared__ double ax[P * 16], ay[P * 16];  // [node][v*4+w]
@@ -152,7 +154,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ v
         for (int k = 0; k < 16; ++k) hs[(side * N + q) * 16 + k] = a[k];
     }
     __syncthreads();
-    double *out = blocks + static_cast<long long>(blockIdx.x) * M * M;
+    T *out = blocks + static_cast<long long>(blockIdx.x) * M * M;
     const double sx = .5 * g.hy, sy = -.5 * g.hx;
     for (int col = 0; col < M; ++col) {
         const int w = col / P, cnode = col % P, jx = cnode % N, jy = cnode / N;
@@ -173,7 +175,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ v
                 val += (-1 * B->left[jy] * hs[(2 * N + i) * 16 + vw]) * B->left[j];
                 val += (1 * B->right[jy] * hs[(3 * N + i) * 16 + vw]) * B->right[j];
             }
-            out[static_cast<long long>(col) * M + row] = val;
+            out[static_cast<long long>(col) * M + row] = static_cast<T>(val);
         }
     }
 }
@@ -377,20 +379,26 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
 // readlane and the factor columns stream once from HBM (column-major: one
 // coalesced load per column and row chunk). The loads of column j+1 are issued
 // before the FMAs of column j so every wave keeps two columns in flight.
-template <int M, typename T>
-__global__ __launch_bounds__(BLOCK) void k_bj_apply(const T *__restrict__ factors, const int *__restrict__ rowsrc,
-                                                    const double *__restrict__ r, double *__restrict__ z,
-                                                    int nblocks) {
+__device__ __forceinline__ float to_compute(__hip_bfloat16 v) { return __bfloat162float(v); }
+__device__ __forceinline__ float to_compute(float v) { return v; }
+__device__ __forceinline__ double to_compute(double v) { return v; }
+
+// TF: stored factor type (double, float or bfloat16); TV: vector and
+// arithmetic type (double for the FP64 path, float for the inner FP32 solve).
+template <int M, typename TF, typename TV>
+__global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ factors, const int *__restrict__ rowsrc,
+                                                    const TV *__restrict__ r, TV *__restrict__ z, int nblocks) {
+    using T = TF;
     constexpr int NA = (M + 63) / 64;
     const int lane = threadIdx.x % 64;
     const long long blk = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + threadIdx.x / 64;
     if (blk >= nblocks) return;
     const T *F = factors + blk * M * M;
-    double x[NA];
+    TV x[NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
         const int row = a * 64 + lane;
-        x[a] = row < M ? r[blk * M + rowsrc[blk * M + row]] : 0.0;
+        x[a] = row < M ? r[blk * M + rowsrc[blk * M + row]] : TV(0);
     }
     // forward substitution with the unit lower L: x_i -= L_ij x_j for i > j
     T cur[NA], nxt[NA];
@@ -414,9 +422,9 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const T *__restrict__ factor
                     nxt[a] = (row > j + 1 && row < M) ? colp[row] : T(0);
                 }
             }
-            const double xj = readlane_d(x[ja], jl);
+            const TV xj = readlane_d(x[ja], jl);
 #pragma unroll
-            for (int a = ja; a < NA; ++a) x[a] -= static_cast<double>(cur[a]) * xj;
+            for (int a = ja; a < NA; ++a) x[a] -= static_cast<TV>(to_compute(cur[a])) * xj;
         }
     }
     // backward substitution with U: x_j /= U_jj, x_i -= U_ij x_j for i < j
@@ -443,13 +451,13 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const T *__restrict__ factor
                     nxt[a] = row <= j - 1 ? colp[row] : T(0);
                 }
             }
-            const double diag = readlane_d(static_cast<double>(cur[ja]), jl);
-            const double xj = readlane_d(x[ja], jl) / diag;
+            const TV diag = readlane_d(static_cast<TV>(to_compute(cur[ja])), jl);
+            const TV xj = readlane_d(x[ja], jl) / diag;
             if (lane == jl) x[ja] = xj;
 #pragma unroll
             for (int a = 0; a <= ja; ++a) {
                 const int row = a * 64 + lane;
-                if (row < j) x[a] -= static_cast<double>(cur[a]) * xj;
+                if (row < j) x[a] -= static_cast<TV>(to_compute(cur[a])) * xj;
             }
         }
     }
@@ -460,21 +468,28 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const T *__restrict__ factor
     }
 }
 
-__global__ void k_convert_f32(const double *__restrict__ src, float *__restrict__ dst, long long count) {
+template <typename TS, typename TD>
+__global__ void k_convert(const TS *__restrict__ src, TD *__restrict__ dst, long long count) {
     for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
          i += static_cast<long long>(gridDim.x) * BLOCK)
-        dst[i] = static_cast<float>(src[i]);
+        dst[i] = static_cast<TD>(src[i]);
+}
+__global__ void k_float_to_bf16(const float *__restrict__ src, __hip_bfloat16 *__restrict__ dst, long long count) {
+    for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
+         i += static_cast<long long>(gridDim.x) * BLOCK)
+        dst[i] = __float2bfloat16(src[i]);
 }
 
 // ----------------------------------------------------------------- GMRES
 constexpr int MAXV = 32;  // restart + 1 <= 32 basis vectors per launch
 
-struct Ptrs { const double *p[MAXV]; };
+template <typename T> struct PtrsT { const T *p[MAXV]; };
+using Ptrs = PtrsT<double>;
 
 // partials[b][k] = sum_i V_k[i] * (W_i * w_i); W = inv_mass(node) (metric
 // weights == nullptr) or weights[i] (gmres.cpp:39-43 projection).
-template <int N>
-__global__ __launch_bounds__(BLOCK) void k_multi_dot(Ptrs V, int nvec, const double *__restrict__ w,
+template <int N, typename TV>
+__global__ __launch_bounds__(BLOCK) void k_multi_dot(PtrsT<TV> V, int nvec, const TV *__restrict__ w,
                                                      const double *__restrict__ weights, long long count,
                                                      double *partials, const Basis *__restrict__ B) {
     constexpr int P = N * N;
@@ -484,10 +499,10 @@ __global__ __launch_bounds__(BLOCK) void k_multi_dot(Ptrs V, int nvec, const dou
     for (int k = 0; k < MAXV; ++k) acc[k] = 0;
     for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
          i += static_cast<long long>(gridDim.x) * BLOCK) {
-        const double ww = (weights ? weights[i] : B->inv_mass[i % P]) * w[i];
+        const double ww = (weights ? weights[i] : B->inv_mass[i % P]) * static_cast<double>(w[i]);
 #pragma unroll
         for (int k = 0; k < MAXV; ++k)
-            if (k < nvec) acc[k] += V.p[k][i] * ww;
+            if (k < nvec) acc[k] += static_cast<double>(V.p[k][i]) * ww;
     }
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
 #pragma unroll
@@ -507,9 +522,9 @@ __global__ __launch_bounds__(BLOCK) void k_multi_dot(Ptrs V, int nvec, const dou
 
 // w -= sum_k V_k h_k (gmres.cpp:44-45), then mode 0: partial projections of
 // the updated w (next CGS pass), mode 1: partial weighted sum of w^2, mode 2: none.
-template <int N>
-__global__ __launch_bounds__(BLOCK) void k_update_dot(Ptrs V, int nvec, const double *__restrict__ h,
-                                                      double *__restrict__ w, const double *__restrict__ weights,
+template <int N, typename TV>
+__global__ __launch_bounds__(BLOCK) void k_update_dot(PtrsT<TV> V, int nvec, const double *__restrict__ h,
+                                                      TV *__restrict__ w, const double *__restrict__ weights,
                                                       long long count, double *partials, int mode,
                                                       const Basis *__restrict__ B) {
     constexpr int P = N * N;
@@ -525,15 +540,16 @@ __global__ __launch_bounds__(BLOCK) void k_update_dot(Ptrs V, int nvec, const do
         double s = 0;
 #pragma unroll
         for (int k = 0; k < MAXV; ++k)
-            if (k < nvec) s += V.p[k][i] * hk[k];
-        const double wi = w[i] - s;
-        w[i] = wi;
+            if (k < nvec) s += static_cast<double>(V.p[k][i]) * hk[k];
+        const TV wt = static_cast<TV>(static_cast<double>(w[i]) - s);
+        w[i] = wt;
+        const double wi = static_cast<double>(wt);
         if (mode != 2) {
             const double ww = (weights ? weights[i] : B->inv_mass[i % P]) * wi;
             if (mode == 0) {
 #pragma unroll
                 for (int k = 0; k < MAXV; ++k)
-                    if (k < nvec) acc[k] += V.p[k][i] * ww;
+                    if (k < nvec) acc[k] += static_cast<double>(V.p[k][i]) * ww;
             } else {
                 acc[0] += wi * ww;
             }
@@ -558,8 +574,9 @@ __global__ __launch_bounds__(BLOCK) void k_update_dot(Ptrs V, int nvec, const do
 }
 
 // out = sum_k V_k y_k (gmres.cpp:75)
-__global__ __launch_bounds__(BLOCK) void k_combine(Ptrs V, int nvec, const double *__restrict__ y,
-                                                   double *__restrict__ out, long long count) {
+template <typename TV, typename TO>
+__global__ __launch_bounds__(BLOCK) void k_combine(PtrsT<TV> V, int nvec, const double *__restrict__ y,
+                                                   TO *__restrict__ out, long long count, int accumulate) {
     double yk[MAXV];
 #pragma unroll
     for (int k = 0; k < MAXV; ++k) yk[k] = k < nvec ? y[k] : 0.0;
@@ -568,16 +585,17 @@ __global__ __launch_bounds__(BLOCK) void k_combine(Ptrs V, int nvec, const doubl
         double s = 0;
 #pragma unroll
         for (int k = 0; k < MAXV; ++k)
-            if (k < nvec) s += V.p[k][i] * yk[k];
-        out[i] = s;
+            if (k < nvec) s += static_cast<double>(V.p[k][i]) * yk[k];
+        out[i] = accumulate ? static_cast<TO>(static_cast<double>(out[i]) + s) : static_cast<TO>(s);
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scale_div(const double *__restrict__ x, double d,
-                                                     double *__restrict__ out, long long count) {
+template <typename TX, typename TO>
+__global__ __launch_bounds__(BLOCK) void k_scale_div(const TX *__restrict__ x, double d,
+                                                     TO *__restrict__ out, long long count) {
     for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
          i += static_cast<long long>(gridDim.x) * BLOCK)
-        out[i] = x[i] / d;
+        out[i] = static_cast<TO>(static_cast<double>(x[i]) / d);
 }
 __global__ __launch_bounds__(BLOCK) void k_sub_div(const double *__restrict__ x, const double *__restrict__ y,
                                                    double d, double *__restrict__ out, long long count) {
@@ -586,12 +604,13 @@ __global__ __launch_bounds__(BLOCK) void k_sub_div(const double *__restrict__ x,
         out[i] = (x[i] - y[i]) / d;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_axpby(double a, const double *__restrict__ x, double b,
-                                                 const double *__restrict__ y, double *__restrict__ out,
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_axpby(double a, const T *__restrict__ x, double b,
+                                                 const T *__restrict__ y, T *__restrict__ out,
                                                  long long count) {
     for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
          i += static_cast<long long>(gridDim.x) * BLOCK)
-        out[i] = (b == 0.0 || y == nullptr) ? a * x[i] : a * x[i] + b * y[i];
+        out[i] = (b == 0.0 || y == nullptr) ? static_cast<T>(a * x[i]) : static_cast<T>(a * x[i] + b * y[i]);
 }
 
 }  // namespace
@@ -648,60 +667,130 @@ hipError_t launch_finish_sum(const double *partials, int nparts, int width, doub
 hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fcache, double *blocks,
                                   double inv_step, const Basis *b, const Geo &g, int first, int count,
                                   hipStream_t s) {
-    DGH_DISPATCH_N(n, { k_assemble<N><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first); })
+    DGH_DISPATCH_N(n, { k_assemble<N, double><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first); })
+}
+hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fcache, float *blocks,
+                                  double inv_step, const Basis *b, const Geo &g, int first, int count,
+                                  hipStream_t s) {
+    DGH_DISPATCH_N(n, { k_assemble<N, float><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first); })
 }
 hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s) {
     DGH_DISPATCH_M(m, { k_lu<M, 16, double><<<nblocks, BLOCK, 0, s>>>(blocks, rowsrc, nblocks, flag); })
 }
-hipError_t launch_lu_f32(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s) {
+hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s) {
     DGH_DISPATCH_M(m, { k_lu<M, 16, float><<<nblocks, BLOCK, 0, s>>>(blocks, rowsrc, nblocks, flag); })
 }
 hipError_t launch_bj_apply(int m, const double *factors, const int *rowsrc, const double *r, double *z,
                            int nblocks, hipStream_t s) {
     DGH_DISPATCH_M(m, {
-        k_bj_apply<M, double><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks);
+        k_bj_apply<M, double, double><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks);
     })
 }
-hipError_t launch_convert_f32(const double *src, float *dst, long long count, hipStream_t s) {
-    k_convert_f32<<<4096, BLOCK, 0, s>>>(src, dst, count);
+hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const float *r, float *z, int nblocks,
+                           hipStream_t s) {
+    DGH_DISPATCH_M(m, {
+        k_bj_apply<M, float, float><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks);
+    })
+}
+hipError_t launch_bj_apply_bf16(int m, const void *factors, const int *rowsrc, const float *r, float *z,
+                                int nblocks, hipStream_t s) {
+    const auto *f = static_cast<const __hip_bfloat16 *>(factors);
+    DGH_DISPATCH_M(m, {
+        k_bj_apply<M, __hip_bfloat16, float><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, rowsrc, r, z, nblocks);
+    })
+}
+hipError_t launch_convert(const double *src, float *dst, long long count, hipStream_t s) {
+    k_convert<double, float><<<4096, BLOCK, 0, s>>>(src, dst, count);
+    return hipGetLastError();
+}
+hipError_t launch_convert(const float *src, double *dst, long long count, hipStream_t s) {
+    k_convert<float, double><<<4096, BLOCK, 0, s>>>(src, dst, count);
+    return hipGetLastError();
+}
+hipError_t launch_to_bf16(const float *src, void *dst, long long count, hipStream_t s) {
+    k_float_to_bf16<<<4096, BLOCK, 0, s>>>(src, static_cast<__hip_bfloat16 *>(dst), count);
     return hipGetLastError();
 }
 
-static Ptrs make_ptrs(const double *const *basis, int nvec) {
-    Ptrs p{};
+template <typename T> static PtrsT<T> make_ptrs(const T *const *basis, int nvec) {
+    PtrsT<T> p{};
     for (int k = 0; k < nvec && k < MAXV; ++k) p.p[k] = basis[k];
     return p;
 }
 
+template <typename TV>
+static hipError_t multi_dot_any(int n, const TV *const *basis, int nvec, const TV *w, const double *weights,
+                                long long count, double *partials, const Basis *b, hipStream_t s) {
+    if (nvec > MAXV) return hipErrorInvalidValue;
+    const auto p = make_ptrs(basis, nvec);
+    DGH_DISPATCH_N(n, { k_multi_dot<N, TV><<<REDUCE_BLOCKS, BLOCK, 0, s>>>(p, nvec, w, weights, count, partials, b); })
+}
 hipError_t launch_multi_dot(int n, const double *const *basis, int nvec, const double *w, const double *weights,
                             long long count, double *partials, const Basis *b, hipStream_t s) {
+    return multi_dot_any(n, basis, nvec, w, weights, count, partials, b, s);
+}
+hipError_t launch_multi_dot(int n, const float *const *basis, int nvec, const float *w, const double *weights,
+                            long long count, double *partials, const Basis *b, hipStream_t s) {
+    return multi_dot_any(n, basis, nvec, w, weights, count, partials, b, s);
+}
+template <typename TV>
+static hipError_t update_dot_any(int n, const TV *const *basis, int nvec, const double *h, TV *w,
+                                 const double *weights, long long count, double *partials, int mode,
+                                 const Basis *b, hipStream_t s) {
     if (nvec > MAXV) return hipErrorInvalidValue;
-    const Ptrs p = make_ptrs(basis, nvec);
-    DGH_DISPATCH_N(n, { k_multi_dot<N><<<REDUCE_BLOCKS, BLOCK, 0, s>>>(p, nvec, w, weights, count, partials, b); })
+    const auto p = make_ptrs(basis, nvec);
+    DGH_DISPATCH_N(n, {
+        k_update_dot<N, TV><<<REDUCE_BLOCKS, BLOCK, 0, s>>>(p, nvec, h, w, weights, count, partials, mode, b);
+    })
 }
 hipError_t launch_update_dot(int n, const double *const *basis, int nvec, const double *h, double *w,
                              const double *weights, long long count, double *partials, int mode, const Basis *b,
                              hipStream_t s) {
-    if (nvec > MAXV) return hipErrorInvalidValue;
-    const Ptrs p = make_ptrs(basis, nvec);
-    DGH_DISPATCH_N(n, {
-        k_update_dot<N><<<REDUCE_BLOCKS, BLOCK, 0, s>>>(p, nvec, h, w, weights, count, partials, mode, b);
-    })
+    return update_dot_any(n, basis, nvec, h, w, weights, count, partials, mode, b, s);
+}
+hipError_t launch_update_dot(int n, const float *const *basis, int nvec, const double *h, float *w,
+                             const double *weights, long long count, double *partials, int mode, const Basis *b,
+                             hipStream_t s) {
+    return update_dot_any(n, basis, nvec, h, w, weights, count, partials, mode, b, s);
 }
 hipError_t launch_combine(const double *const *basis, int nvec, const double *y, double *out, long long count,
                           hipStream_t s) {
     if (nvec > MAXV) return hipErrorInvalidValue;
-    k_combine<<<4096, BLOCK, 0, s>>>(make_ptrs(basis, nvec), nvec, y, out, count);
+    k_combine<double, double><<<4096, BLOCK, 0, s>>>(make_ptrs(basis, nvec), nvec, y, out, count, 0);
+    return hipGetLastError();
+}
+hipError_t launch_combine(const float *const *basis, int nvec, const double *y, float *out, long long count,
+                          hipStream_t s) {
+    if (nvec > MAXV) return hipErrorInvalidValue;
+    k_combine<float, float><<<4096, BLOCK, 0, s>>>(make_ptrs(basis, nvec), nvec, y, out, count, 0);
+    return hipGetLastError();
+}
+hipError_t launch_combine_add(const float *const *basis, int nvec, const double *y, double *out, long long count,
+                              hipStream_t s) {
+    if (nvec > MAXV) return hipErrorInvalidValue;
+    k_combine<float, double><<<4096, BLOCK, 0, s>>>(make_ptrs(basis, nvec), nvec, y, out, count, 1);
     return hipGetLastError();
 }
 hipError_t launch_axpby(double a, const double *x, double b, const double *y, double *out, long long count,
                         hipStream_t s) {
-    k_axpby<<<4096, BLOCK, 0, s>>>(a, x, b, y, out, count);
+    k_axpby<double><<<4096, BLOCK, 0, s>>>(a, x, b, y, out, count);
     return hipGetLastError();
 }
-
+hipError_t launch_axpby(double a, const float *x, double b, const float *y, float *out, long long count,
+                        hipStream_t s) {
+    k_axpby<float><<<4096, BLOCK, 0, s>>>(a, x, b, y, out, count);
+    return hipGetLastError();
+}
 hipError_t launch_scale_div(const double *x, double d, double *out, long long count, hipStream_t s) {
-    k_scale_div<<<4096, BLOCK, 0, s>>>(x, d, out, count);
+    k_scale_div<double, double><<<4096, BLOCK, 0, s>>>(x, d, out, count);
+    return hipGetLastError();
+}
+hipError_t launch_scale_div(const float *x, double d, float *out, long long count, hipStream_t s) {
+    k_scale_div<float, float><<<4096, BLOCK, 0, s>>>(x, d, out, count);
+    return hipGetLastError();
+}
+hipError_t launch_scale_div(const double *x, double d, float *out, long long count, hipStream_t s) {
+    k_scale_div<double, float><<<4096, BLOCK, 0, s>>>(x, d, out, count);
     return hipGetLastError();
 }
 hipError_t launch_sub_div(const double *x, const double *y, double d, double *out, long long count, hipStream_t s) {

@@ -88,14 +88,14 @@ template <int N> struct Shape {
 // ---------------------------------------------------------------- traces
 // Restriction of every element to its four sides (discretization.cpp:50-60):
 // x sides sum over i of sheet(i, q) * shape_i, y sides over j of sheet(q, j).
-template <int N>
-__global__ __launch_bounds__(512) void k_restrict(const double *__restrict__ x, double *__restrict__ tr, Geo g,
+template <int N, typename TX>
+__global__ __launch_bounds__(512) void k_restrict(const TX *__restrict__ x, double *__restrict__ tr, Geo g,
                                                   const Basis *__restrict__ B) {
     using S = Shape<N>;
     __shared__ double cellbuf[S::RCPB * S::M];
     const long long first = static_cast<long long>(blockIdx.x) * S::RCPB;
     const int ncell = min(S::RCPB, static_cast<int>(g.cells - first));
-    for (int k = threadIdx.x; k < ncell * S::M; k += blockDim.x) cellbuf[k] = x[first * S::M + k];
+    for (int k = threadIdx.x; k < ncell * S::M; k += blockDim.x) cellbuf[k] = static_cast<double>(x[first * S::M + k]);
     __syncthreads();
     const int local = threadIdx.x / (16 * N), t = threadIdx.x % (16 * N);
     if (local >= ncell) return;
@@ -383,9 +383,9 @@ __global__ void k_jv_flux(const double *__restrict__ tr, const double *__restric
 }
 
 // volume_jacobian.cpp:59-71 + dg_operator.cpp:10-21: y = M x / tau - weak(dF x) + face lifts.
-template <int N>
-__global__ __launch_bounds__(BLOCK) void k_element_jv(const double *__restrict__ x, const double *__restrict__ vc,
-                                                      const double *__restrict__ fq, double *__restrict__ y,
+template <int N, typename TX, typename TY>
+__global__ __launch_bounds__(BLOCK) void k_element_jv(const TX *__restrict__ x, const double *__restrict__ vc,
+                                                      const double *__restrict__ fq, TY *__restrict__ y,
                                                       double inv_step, const Basis *__restrict__ B, Geo g) {
     using S = Shape<N>;
     __shared__ double sfx[S::CPB * S::M], sfy[S::CPB * S::M];
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(BLOCK) void k_element_jv(const double *__restrict__
     if (active) {
         double d[4];
         for (int v = 0; v < 4; ++v) {
-            d[v] = x[cell * S::M + v * S::P + node];
+            d[v] = static_cast<double>(x[cell * S::M + v * S::P + node]);
             res[v] = B->mass[node] * d[v] * inv_step;
         }
         if (N > 1) {
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(BLOCK) void k_element_jv(const double *__restrict__
     }
     for (int v = 0; v < 4; ++v) {
         res[v] = lift_faces<N>(fq, B, static_cast<int>(cell), i, j, v, g, res[v]);
-        y[cell * S::M + v * S::P + node] = res[v];
+        y[cell * S::M + v * S::P + node] = static_cast<TY>(res[v]);
     }
 }
 
@@ -785,11 +785,18 @@ unsigned grid_for(long long threads, int block) { return static_cast<unsigned>((
     }                                             \
     return hipGetLastError();
 
-hipError_t launch_restrict(int n, const double *x, double *traces, const Basis *b, const Geo &g, hipStream_t s) {
+template <typename TX>
+static hipError_t restrict_any(int n, const TX *x, double *traces, const Basis *b, const Geo &g, hipStream_t s) {
     DGH_DISPATCH_N(n, {
         using S = Shape<N>;
-        k_restrict<N><<<grid_for(g.cells, S::RCPB), S::RCPB * 16 * N, 0, s>>>(x, traces, g, b);
+        k_restrict<N, TX><<<grid_for(g.cells, S::RCPB), S::RCPB * 16 * N, 0, s>>>(x, traces, g, b);
     })
+}
+hipError_t launch_restrict(int n, const double *x, double *traces, const Basis *b, const Geo &g, hipStream_t s) {
+    return restrict_any(n, x, traces, b, g, s);
+}
+hipError_t launch_restrict(int n, const float *x, double *traces, const Basis *b, const Geo &g, hipStream_t s) {
+    return restrict_any(n, x, traces, b, g, s);
 }
 hipError_t launch_freeze_speeds(int n, const double *traces, double *speeds, int *flag, const Geo &g,
                                 hipStream_t s) {
@@ -825,11 +832,24 @@ hipError_t launch_jv_flux(int n, const double *traces, const double *fcache, dou
                           hipStream_t s) {
     DGH_DISPATCH_N(n, { k_jv_flux<N><<<grid_for(2LL * g.cells * N, BLOCK), BLOCK, 0, s>>>(traces, fcache, fq, g); })
 }
+template <typename TX, typename TY>
+static hipError_t element_jv_any(int n, const TX *x, const double *vcache, const double *fq, TY *y, double inv_step,
+                                 const Basis *b, const Geo &g, hipStream_t s) {
+    DGH_DISPATCH_N(n, {
+        k_element_jv<N, TX, TY><<<grid_for(g.cells, Shape<N>::CPB), BLOCK, 0, s>>>(x, vcache, fq, y, inv_step, b, g);
+    })
+}
 hipError_t launch_element_jv(int n, const double *x, const double *vcache, const double *fq, double *y,
                              double inv_step, const Basis *b, const Geo &g, hipStream_t s) {
-    DGH_DISPATCH_N(n, {
-        k_element_jv<N><<<grid_for(g.cells, Shape<N>::CPB), BLOCK, 0, s>>>(x, vcache, fq, y, inv_step, b, g);
-    })
+    return element_jv_any(n, x, vcache, fq, y, inv_step, b, g, s);
+}
+hipError_t launch_element_jv(int n, const float *x, const double *vcache, const double *fq, float *y,
+                             double inv_step, const Basis *b, const Geo &g, hipStream_t s) {
+    return element_jv_any(n, x, vcache, fq, y, inv_step, b, g, s);
+}
+hipError_t launch_element_jv(int n, const float *x, const double *vcache, const double *fq, double *y,
+                             double inv_step, const Basis *b, const Geo &g, hipStream_t s) {
+    return element_jv_any(n, x, vcache, fq, y, inv_step, b, g, s);
 }
 hipError_t launch_adjoint_face(int n, const double *wtraces, const double *fcache, double *gtraces, const Geo &g,
                                hipStream_t s) {

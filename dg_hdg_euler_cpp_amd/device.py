@@ -25,7 +25,8 @@ class UpdateStats(C.Structure):
                 ("time", C.c_double), ("stage_time", C.c_double), ("dt", C.c_double), ("cfl", C.c_double),
                 ("residual_physical_rms", C.c_double), ("damping", C.c_double),
                 ("linear_relative_residual", C.c_double), ("dg_linear_relative_residual", C.c_double),
-                ("iteration_s", C.c_double), ("inner_iterations", C.c_int), ("outer_iterations", C.c_int)]
+                ("iteration_s", C.c_double), ("inner_iterations", C.c_int), ("outer_iterations", C.c_int),
+                ("precision_error", C.c_double)]
 
 
 class RunStats(C.Structure):
@@ -222,6 +223,28 @@ class DeviceContext:
         z = self.zeros() if out is None else out
         self._call("dgh_bj_apply", _ptr(r), _ptr(z))
         return z
+
+    def bj_refresh_mixed(self):
+        self._call("dgh_bj_refresh_mixed")
+
+    def bj_apply_f32(self, r, out=None):
+        z = self._torch.zeros(self.size, dtype=self._torch.float32, device="cuda") if out is None else out
+        self._call("dgh_bj_apply_f32", _ptr(r), _ptr(z))
+        return z
+
+    def dg_apply_f32(self, x, out=None):
+        y = self._torch.zeros(self.size, dtype=self._torch.float32, device="cuda") if out is None else out
+        self._call("dgh_dg_apply_f32", _ptr(x), _ptr(y))
+        return y
+
+    def dg_fgmres(self, rhs, max_iterations, tolerance):
+        x = self.zeros()
+        its = C.c_int()
+        outer = C.c_int()
+        rel = C.c_double()
+        self._call("dgh_dg_fgmres", _ptr(rhs), _ptr(x), C.c_int(max_iterations), C.c_double(tolerance),
+                   C.byref(its), C.byref(outer), C.byref(rel))
+        return x, its.value, outer.value, rel.value
 
     def dg_gmres(self, rhs, max_iterations, tolerance):
         x = self.zeros()

@@ -63,7 +63,9 @@ typedef struct dgh_config {
     int method;     /* enum dgh_method */
     int testcase;   /* enum dgh_testcase */
     int precision;  /* enum dgh_precision */
-    int inner_restart;
+    int inner_restart;       /* mixed: restart of the FP32 inner GMRES */
+    int inner_max;           /* mixed: iteration cap of one inner solve */
+    double inner_tolerance;  /* mixed: relative tolerance of one inner solve */
     unsigned long long seed;
     char prefix[512];
 } dgh_config;
@@ -75,6 +77,9 @@ typedef struct dgh_update_stats {
     double time, stage_time, dt, cfl, residual_physical_rms, damping;
     double linear_relative_residual, dg_linear_relative_residual, iteration_s;
     int inner_iterations, outer_iterations;
+    /* mixed precision with check_update: ||du - du_fp64||_M / ||du_fp64||_M of
+     * the same Newton system solved by the FP64 path; -1 when not measured */
+    double precision_error;
 } dgh_update_stats;
 
 /* Mirrors euler::RunStatistics (time_integrator.hpp:14-20) without the vector. */
@@ -89,7 +94,8 @@ typedef struct dgh_run_stats {
 size_t dgh_config_size(void);
 int dgh_config_default(dgh_config *out);
 /* Parses reference CLI flags (argv[0] is skipped) plus --device gpu,
- * --precision fp64|mixed|mixed-bf16, --inner-restart N and --seed S. On error
+ * --precision fp64|mixed|mixed-bf16, --inner-restart N, --inner-max N,
+ * --inner-tol R and --seed S. On error
  * writes the reference message into err. */
 int dgh_config_parse(int argc, const char *const *argv, dgh_config *out, char *err, size_t errlen);
 
@@ -152,6 +158,19 @@ int dgh_dg_block(dgh_ctx *ctx, int row, double *out_host);
 int dgh_bj_refresh(dgh_ctx *ctx);
 /* BlockJacobi::apply (block_jacobi.hpp:24-39): z = P^-1 r. */
 int dgh_bj_apply(dgh_ctx *ctx, const double *r, double *z);
+
+/* ---- mixed precision (no reference counterpart) ---------------------------- */
+/* Block Jacobi factors in FP32 (and BF16 storage for DGH_PRECISION_MIXED_BF16)
+ * from the same element blocks as dgh_bj_refresh. */
+int dgh_bj_refresh_mixed(dgh_ctx *ctx);
+/* FP32 vectors: z = P32^-1 r and y = J x (Jacobian snapshot of dgh_dg_refresh). */
+int dgh_bj_apply_f32(dgh_ctx *ctx, const float *r, float *z);
+int dgh_dg_apply_f32(dgh_ctx *ctx, const float *x, float *y);
+/* FP64 flexible GMRES(restart) whose preconditioner is an FP32 GMRES
+ * (inner_restart, inner_tolerance, inner_max) with the FP32 block Jacobi.
+ * iterations = inner iterations, outer = FGMRES iterations. */
+int dgh_dg_fgmres(dgh_ctx *ctx, const double *rhs, double *x, int max_iterations, double tolerance,
+                  int *iterations, int *outer, double *relative_residual);
 
 /* ---- Krylov ------------------------------------------------------------- */
 /* GMRES::solve (gmres.cpp:17-86) with A = DG Jv and M = block Jacobi, metric

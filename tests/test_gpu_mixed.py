@@ -1,0 +1,120 @@
+"""Mixed-precision device path (no reference counterpart): FP32 / BF16-stored
+block-Jacobi factors, FP32 Jv, FP32 inner GMRES inside FP64 flexible GMRES.
+
+The FP64 oracle pins it: the FP32 pieces against round-off bounds of their
+storage precision, the FGMRES solution against the oracle's own linear system
+at the reference's ltol (1e-12), and whole Newton runs against the reference's
+archived RMS sequences ("equal residual").
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_runs.json")))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def mass_vector(orc):
+    w = orc.basis()["w"]
+    n = orc.n
+    node = np.array([0.25 * orc.hx * orc.hy * w[k % n] * w[k // n] for k in range(n * n)])
+    return np.tile(node, 4 * orc.cells)
+
+
+@pytest.mark.parametrize("precision,bound", [("mixed", 1e-5), ("mixed-bf16", 6e-2)])
+@pytest.mark.parametrize("k,nx", [(2, 8), (8, 12)])
+def test_fp32_pieces_and_fgmres(precision, bound, k, nx, torch_cuda):
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    args = ["--nx", str(nx), "--ny", str(nx), "--k", str(k), "--cfl", "5", "--integrator", "be"]
+    print(f"\n[case] {precision} {args}", flush=True)
+    orc = O.Oracle(args)
+    dev = DeviceContext(args + ["--precision", precision, "--restart", "20"])
+    u_o = orc.initial()
+    u = torch_cuda.tensor(u_o, device="cuda")
+    rate = orc.freeze(u_o)
+    dev.freeze(u)
+    tau = 5.0 / rate
+    orc.set_step(tau, u_o)
+    dev.set_step(tau, u)
+    orc.dg_refresh(u_o)
+    dev.dg_refresh(u)
+    dev.bj_refresh_mixed()
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(u_o.size)
+    # FP32 Jv (FP32 in/out, FP64 arithmetic on the FP64 snapshot)
+    y32 = dev.dg_apply_f32(torch_cuda.tensor(x, dtype=torch_cuda.float32, device="cuda")).cpu().numpy()
+    assert rel(y32, orc.dg_apply(x.astype(np.float32).astype(np.float64))) < 1e-6
+    # FP32 / BF16 factor apply: backward error per block at the storage precision
+    r = rng.standard_normal(u_o.size)
+    z32 = dev.bj_apply_f32(torch_cuda.tensor(r, dtype=torch_cuda.float32, device="cuda")).cpu().numpy()
+    m = orc.m
+    worst = 0.0
+    for row in range(0, orc.cells, max(1, orc.cells // 8)):
+        B = orc.dg_block(row, row)
+        zb = z32[row * m:(row + 1) * m].astype(np.float64)
+        res = B @ zb - r[row * m:(row + 1) * m].astype(np.float32).astype(np.float64)
+        worst = max(worst, np.linalg.norm(res) / (np.linalg.norm(B) * np.linalg.norm(zb)))
+    assert worst < bound, worst
+    # FP64 FGMRES with FP32 inner solves reaches the reference's ltol on the
+    # oracle's own FP64 system
+    rhs = -orc.dg_residual(u_o)
+    xd, inner, outer, reld = dev.dg_fgmres(torch_cuda.tensor(rhs, device="cuda"), 1500, 1e-12)
+    xd = xd.cpu().numpy()
+    assert reld <= 1e-12 and inner > 0 and outer > 0
+    w = 1.0 / mass_vector(orc)
+    res = orc.dg_apply(xd) - rhs
+    assert np.sqrt(np.sum(res ** 2 * w) / np.sum(rhs ** 2 * w)) <= 2e-12
+    dev.close()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "mixed-bf16"])
+def test_mixed_newton_runs_match_reference(precision, torch_cuda):
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    run = GOLD["runs"]["be_4x4_k2"]
+    with DeviceContext(run["args"] + ["--method", "dg", "--precision", precision]) as dev:
+        u = dev.initial()
+        st = dev.advance(u)
+        err = dev.error_l2(u, st["final_time"])
+    ref = [r for r in run["iterations"] if r["method"] == "dg"]
+    first = float(ref[0]["residual_physical_rms"])
+    tol = [1e-13, 1e-10, 1e-6]
+    for got, want in zip(st["updates"], ref):
+        it = int(want["iteration"])
+        w = float(want["residual_physical_rms"])
+        assert abs(got["residual_physical_rms"] - w) <= max(tol[it] * w, 1e-12 * first)
+        assert got["linear_relative_residual"] <= 1e-12
+        assert got["outer_iterations"] > 0
+    summ = [s for s in run["summary"] if s["method"] == "dg"][0]
+    for v, k in enumerate(["rho_L2", "momentum_x_L2", "momentum_y_L2", "energy_L2"]):
+        assert abs(err[v] - float(summ[k])) <= 1e-9 * float(summ[k])
+
+
+def test_mixed_update_error_vs_fp64_256(torch_cuda):
+    """256^2, k=8: two BE Newton updates with FP32 factors + FP32 inner GMRES;
+    each update's direction is compared with the FP64 path's solution of the
+    same system (precision_error) and the RMS sequence with the reference's."""
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    run = GOLD["runs"]["k8_256_be"]
+    args = list(run["args"])
+    args[args.index("--newton-steps") + 1] = "2"
+    args[args.index("--warmup-updates") + 1] = "0"
+    args = [a for a in args if a not in ("--method", "dg")]
+    with DeviceContext(args + ["--method", "dg", "--precision", "mixed", "--check-update"]) as dev:
+        u = dev.initial()
+        st = dev.advance(u)
+    for it, got in enumerate(st["updates"]):
+        w = float(run["iterations"][it]["residual_physical_rms"])
+        assert abs(got["residual_physical_rms"] - w) <= [1e-13, 1e-10][it] * w
+        assert 0 <= got["precision_error"] <= 1e-9, got["precision_error"]
