@@ -59,18 +59,22 @@ def run_device(args, nx, k, init, precision):
                 f"lin={x['linear_relative_residual']:.3e} damping={x['damping']} t={x['iteration_s']:.4f}s")
         # Dominant kernel: block-Jacobi apply, timed with HIP events on the
         # library's stream over a few launches on the final linearisation.
-        r = torch.randn(dev.size, dtype=torch.float64, device="cuda")
-        z = dev.zeros()
-        dev.bj_apply(r, z)
+        mixed = precision != "fp64"
+        vdtype = torch.float32 if mixed else torch.float64
+        r = torch.randn(dev.size, dtype=vdtype, device="cuda")
+        z = torch.zeros_like(r)
+        apply = dev.bj_apply_f32 if mixed else dev.bj_apply
+        apply(r, z)
         dev.set_timing(True)
         dev.reset_kernel_times()
         for _ in range(10):
-            dev.bj_apply(r, z)
+            apply(r, z)
         bj_ms, bj_n = dev.kernel_time("bj_apply")
         dev.reset_kernel_times()
+        x64 = z.to(torch.float64)
         y = dev.zeros()
         for _ in range(10):
-            dev.dg_apply(z, y)
+            dev.dg_apply(x64, y)
         jv_ms, jv_n = dev.kernel_time("jv")
         dev.set_timing(False)
         geom = dict(size=dev.size, cells=dev.cells, m=dev.m, n=dev.n, faces=dev.faces)
@@ -78,10 +82,15 @@ def run_device(args, nx, k, init, precision):
                 jv_avg_s=jv_ms / 1e3 / max(1, jv_n), geom=geom)
 
 
-def bj_bytes(g, elem_bytes=8):
+FACTOR_BYTES = {"fp64": 8, "mixed": 4, "mixed-bf16": 2}
+VECTOR_BYTES = {"fp64": 8, "mixed": 4, "mixed-bf16": 4}
+
+
+def bj_bytes(g, precision="fp64"):
     """Algorithmic bytes of one block-Jacobi apply: LU factors (m^2 per block),
     int32 row permutation, read r and write z."""
-    return g["cells"] * g["m"] * g["m"] * elem_bytes + g["cells"] * g["m"] * 4 + 2 * g["size"] * 8
+    return (g["cells"] * g["m"] * g["m"] * FACTOR_BYTES[precision] + g["cells"] * g["m"] * 4
+            + 2 * g["size"] * VECTOR_BYTES[precision])
 
 
 def jv_bytes(g):
@@ -139,8 +148,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--k", type=int, default=8)
-    ap.add_argument("--init", default="seeded", choices=["seeded", "diagonal"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "mixed-bf16"])
+    ap.add_argument("--init", default="diagonal", choices=["seeded", "diagonal"])
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed", "mixed-bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -148,20 +157,42 @@ def main():
               "(the path would shard by 2D element tiles, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    dev = run_device(args, args.nx, args.k, args.init, args.precision)
+    forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    runs = {p: run_device(args, args.nx, args.k, args.init, p) for p in forms}
+    head = forms[-1]  # the headline form: mixed precision when both are run
+    dev = runs[head]
     g = dev["geom"]
-    upd = [x for x in dev["stats"]["updates"]][args.warmup:]
-    krylov = sum(x["krylov_iterations"] for x in upd) / max(1, len(upd))
-    bytes_bj = bj_bytes(g)
+
+    def summary(p):
+        d = runs[p]
+        upd = d["stats"]["updates"][args.warmup:]
+        krylov = sum(x["krylov_iterations"] for x in upd) / max(1, len(upd))
+        b = bj_bytes(d["geom"], p)
+        ach = b / d["bj_avg_s"] / 1e9
+        return {"updates_per_s": d["ups"], "ms_per_update": 1e3 / d["ups"], "krylov_per_update": krylov,
+                "rms": [x["residual_physical_rms"] for x in d["stats"]["updates"]],
+                "linear_relative_residual": [x["linear_relative_residual"] for x in d["stats"]["updates"]],
+                "bj_apply": {"avg_launch_s": d["bj_avg_s"], "algorithmic_bytes": b, "achieved_GBs": ach,
+                             "frac_hbm": ach / HBM_PEAK_GBS}}
+
+    forms_out = {p: summary(p) for p in forms}
+    krylov = forms_out[head]["krylov_per_update"]
+    bytes_bj = bj_bytes(g, head)
     achieved = bytes_bj / dev["bj_avg_s"] / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "bj_apply_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"bj_apply_traffic_{head}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    equal = None
+    if len(forms) == 2:
+        a, b = forms_out["fp64"]["rms"], forms_out["mixed"]["rms"]
+        equal = {"rms_fp64": a, "rms_mixed": b,
+                 "max_rel_diff_first_two_updates": max(abs(x - y) / abs(x) for x, y in zip(a[:2], b[:2])),
+                 "speedup_mixed_over_fp64": forms_out["mixed"]["updates_per_s"] / forms_out["fp64"]["updates_per_s"]}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.nx, args.k, args.init, krylov)
+        cpu = cpu_baseline(args.nx, args.k, args.init, forms_out.get("fp64", forms_out[head])["krylov_per_update"])
     line = {
         "metric": "2D DG k=8 256² Newton updates/s: FP64 vs mixed-prec at equal residual",
         "value": dev["ups"],
@@ -173,15 +204,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64+f32",
-        "data": f"synthetic ({args.init} state, seed 2025)",
+        "dtype": "f64" if head == "fp64" else ("f64+f32" if head == "mixed" else "f64+f32+bf16"),
+        "data": f"synthetic ({args.init} state{', seed 2025' if args.init == 'seeded' else ''})",
         "config": {"workload": f"2D DG BE Newton update, k={args.k}, {args.nx}x{args.nx} periodic quads, "
                                f"CFL 5, GMRES(20) + block-Jacobi LU, ntol=ltol=1e-12",
-                   "nx": args.nx, "ny": args.nx, "k": args.k, "init": args.init, "precision": args.precision,
+                   "nx": args.nx, "ny": args.nx, "k": args.k, "init": args.init, "precision": head,
                    "krylov_per_update": krylov},
         "roofline": {"kernel": "bj_apply", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": bytes_bj, "avg_launch_s": dev["bj_avg_s"]},
+        "forms": forms_out,
+        "equal_residual": equal,
         "jv": {"dof_per_s": g["size"] / dev["jv_avg_s"], "achieved_GBs": jv_bytes(g) / dev["jv_avg_s"] / 1e9,
                "avg_launch_s": dev["jv_avg_s"]},
         "cpu_baseline": None if cpu is None else {k: v for k, v in cpu.items() if k != "seconds_per_update"},
