@@ -1,0 +1,88 @@
+"""CPU-side checks of the C ABI: the library loads without a GPU, exports every
+symbol include/dgh/dgh_device.h declares, and parses the reference CLI with the
+reference's defaults and error messages (config.cpp:6-122)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "dgh", "dgh_device.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dgh_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from dg_hdg_euler_cpp_amd import load_library
+
+    L = load_library()
+    syms = header_symbols()
+    assert len(syms) >= 30
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def parse(args):
+    from dg_hdg_euler_cpp_amd.device import parse_config
+
+    return parse_config(args)
+
+
+class Cfg(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("degree", C.c_int), ("quadrature", C.c_int),
+                ("restart", C.c_int), ("max_linear", C.c_int), ("max_newton", C.c_int), ("steps", C.c_int),
+                ("newton_steps", C.c_int), ("warmup_updates", C.c_int), ("cfl", C.c_double),
+                ("final_time", C.c_double), ("safety", C.c_double), ("nonlinear_tolerance", C.c_double),
+                ("linear_tolerance", C.c_double), ("direct", C.c_int), ("colamd", C.c_int),
+                ("timing_only", C.c_int), ("check_update", C.c_int), ("save_solution", C.c_int),
+                ("integrator", C.c_int), ("method", C.c_int), ("testcase", C.c_int), ("precision", C.c_int),
+                ("inner_restart", C.c_int), ("inner_max", C.c_int), ("inner_tolerance", C.c_double),
+                ("seed", C.c_ulonglong), ("prefix", C.c_char * 512)]
+
+
+def test_config_struct_layout_and_defaults():
+    from dg_hdg_euler_cpp_amd import load_library
+
+    L = load_library()
+    assert L.dgh_config_size() == C.sizeof(Cfg)
+    c = Cfg.from_buffer_copy(parse([]))
+    # config.hpp:7-18
+    assert (c.nx, c.ny, c.degree, c.quadrature, c.restart, c.max_linear, c.max_newton) == (8, 8, 3, 0, 30, 1500, 30)
+    assert (c.cfl, c.final_time, c.safety, c.nonlinear_tolerance, c.linear_tolerance) == (2.0, 0.2, 1.1, 1e-12, 1e-12)
+    assert c.integrator == 1 and c.method == 2 and c.testcase == 0 and c.prefix == b"output/run_2d"
+    c = Cfg.from_buffer_copy(parse(["--nx", "256", "--k", "8", "--integrator", "be", "--steps", "1",
+                                    "--precision", "mixed", "--device", "gpu", "--restart", "20"]))
+    assert (c.nx, c.degree, c.integrator, c.steps, c.precision, c.restart) == (256, 8, 0, 1, 1, 20)
+
+
+@pytest.mark.parametrize("args,message", [
+    (["--k", "9"], "Invalid mesh or degree"),
+    (["--T", "0.1", "--steps", "2"], "Choose either --T or --steps"),
+    (["--precond", "block-ilu0"], "2D currently supports block-jacobi only"),
+    (["--linear", "lu"], "Invalid linear solver"),
+    (["--integrator", "rk4"], "Invalid integrator"),
+    (["--method", "fem"], "Method must be dg, hdg, or both"),
+    (["--ltol", "2"], "Invalid solver tolerance or wave-speed safety"),
+    (["--bogus"], "Unknown option: --bogus"),
+    (["--nx"], "Missing option value"),
+    (["--nx", "4x"], "Invalid integer"),
+    (["--device", "cpu"], "Only --device gpu is built; the CPU path is the reference executable"),
+])
+def test_config_errors_use_reference_messages(args, message):
+    from dg_hdg_euler_cpp_amd.device import DeviceError
+
+    with pytest.raises(DeviceError) as e:
+        parse(args)
+    assert str(e.value) == message
+
+
+def test_cli_reports_errors_like_the_reference():
+    import subprocess
+
+    exe = os.path.join(ROOT, "dg_hdg_euler_cpp_amd", "bin", "dg_hdg_2d")
+    p = subprocess.run([exe, "--k", "9"], capture_output=True, text=True)
+    assert p.returncode == 1 and p.stderr.strip() == "ERROR: Invalid mesh or degree"
