@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <map>
 #include <memory>
@@ -175,6 +176,11 @@ struct dgh_ctx {
     DevBuf<int> rowsrc32;
     DevBuf<double> check_dir;
     bool bj32_ready = false;
+    // HDG: X [cells][m x t], y [cells][m], S_K [cells][t x t], per-element face
+    // results, trace work vectors, face block-Jacobi factors
+    DevBuf<double> hX, hY, hS, hfres, hloc, hdiag, hrhs, itm, face_factors, trace_dir, hlambda, indep;
+    DevBuf<int> face_rowsrc;
+    bool hdg_ready = false, face_bj_ready = false;
     // Newton
     DevBuf<double> state, trial, residual, trial_residual, direction, rhs, equation_error, step_state, stage_base,
         slopes;
@@ -229,12 +235,14 @@ struct dgh_ctx {
         DevBuf<double> *bufs[] = {&old, &speeds, &tr_u, &tr_x, &fq, &gtr, &vcache, &fcache, &partials, &scalars,
                                   &factors, &kbasis, &kw, &kz, &kwork, &kres, &ky, &state, &trial, &residual,
                                   &trial_residual, &direction, &rhs, &equation_error, &step_state, &stage_base,
-                                  &slopes, &check_dir};
+                                  &slopes, &check_dir, &hX, &hY, &hS, &hfres, &hloc, &hdiag, &hrhs, &itm,
+                                  &face_factors, &trace_dir, &hlambda, &indep};
         for (auto *b : bufs) b->release(bytes);
         DevBuf<float> *fbufs[] = {&factors_f32, &ibasis, &iw, &it, &is, &ires, &ib, &zbasis};
         for (auto *b : fbufs) b->release(bytes);
         factors_bf16.release(bytes);
         rowsrc32.release(bytes);
+        face_rowsrc.release(bytes);
         factors32.release(bytes);
         rowsrc.release(bytes);
         flags.release(bytes);
@@ -264,8 +272,8 @@ struct dgh_ctx {
     void sync() { HIPCHECK(hipStreamSynchronize(stream)); }
 
     // ---- reductions
-    double reduce_sq(const double *x, long long count, int mode) {
-        HIPCHECK(launch_weighted_sq(n, x, count, mode, partials.p, basis.p, stream));
+    double reduce_sq(const double *x, long long count, int mode, const double *weights = nullptr) {
+        HIPCHECK(launch_weighted_sq(n, x, count, mode, weights, partials.p, basis.p, stream));
         HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 1, scalars.p + 200, stream));
         double out = 0;
         HIPCHECK(hipMemcpyAsync(&out, scalars.p + 200, sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -363,6 +371,8 @@ struct dgh_ctx {
         refreshed = true;
         bj_ready = false;
         bj32_ready = false;
+        hdg_ready = false;
+        face_bj_ready = false;
     }
     void dg_apply(const double *x, double *y) {
         require(refreshed, "Refresh the volume Jacobian before applying it");
@@ -397,9 +407,20 @@ struct dgh_ctx {
         timed("bj_apply", [&] { HIPCHECK(launch_bj_apply(m, factors.p, rowsrc.p, r, z, cells, stream)); });
     }
 
-    // ---- GMRES (gmres.cpp:17-86), A = DG Jv, M = block Jacobi, metric M^-1
+    // ---- GMRES (gmres.cpp:17-86): restarted, right-preconditioned, weighted
+    //      CGS2, Givens, true residual after each cycle. Vectors of length N
+    //      with the DG metric M^-1 (weights == nullptr) or a weight vector.
     struct Linear { int iterations; double relative; };
+    using Op = std::function<void(const double *, double *)>;
     Linear gmres(const double *rhs, double *x, int limit, double tolerance) {
+        return gmres_generic(D, nullptr, [&](const double *a, double *b) { dg_apply(a, b); },
+                             [&](const double *a, double *b) { bj_apply(a, b); }, rhs, x, limit, tolerance);
+    }
+    double vec_norm(const double *x, long long N, const double *weights) {
+        return std::sqrt(std::max(0.0, weights ? reduce_sq(x, N, 3, weights) : reduce_sq(x, N, 0)));
+    }
+    Linear gmres_generic(long long D, const double *weights, const Op &A, const Op &M, const double *rhs, double *x,
+                         int limit, double tolerance) {
         const int restart = static_cast<int>(std::min<long long>(D, cfg.restart));
         require(restart + 1 <= 32, "GMRES restart above 31 is not supported on the device");
         kbasis.alloc(static_cast<size_t>(restart + 1) * D, bytes);
@@ -413,30 +434,30 @@ struct dgh_ctx {
         auto Vk = [&](int k) { return kbasis.p + static_cast<size_t>(k) * D; };
         HIPCHECK(hipMemsetAsync(x, 0, D * sizeof(double), stream));
         copy(kres.p, rhs, D);
-        const double rhs_norm = gmres_norm(rhs);
+        const double rhs_norm = vec_norm(rhs, D, weights);
         if (rhs_norm == 0) return {0, 0};
         std::vector<double> H((restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), y(restart);
         auto Hm = [&](int i, int j) -> double & { return H[i + (restart + 1) * j]; };
         int iterations = 0;
         double host_scalars[65];
         while (iterations < limit) {
-            const double beta = gmres_norm(kres.p);
+            const double beta = vec_norm(kres.p, D, weights);
             if (beta <= tolerance * rhs_norm) return {iterations, beta / rhs_norm};
-            scale_div(kres.p, beta, Vk(0));  // basis_.col(0) = residual_ / beta
+            HIPCHECK(launch_scale_div(kres.p, beta, Vk(0), D, stream));  // basis_.col(0) = residual_ / beta
             std::fill(H.begin(), H.end(), 0.0);
             std::fill(g.begin(), g.end(), 0.0);
             g[0] = beta;
             int count = 0;
             for (int j = 0; j < restart && iterations < limit; ++j) {
-                bj_apply(Vk(j), kz.p);
-                dg_apply(kz.p, kw.p);
+                M(Vk(j), kz.p);
+                A(kz.p, kw.p);
                 timed("gmres_ortho", [&] {
-                    HIPCHECK(launch_multi_dot(n, V.data(), j + 1, kw.p, nullptr, D, partials.p, basis.p, stream));
+                    HIPCHECK(launch_multi_dot(n, V.data(), j + 1, kw.p, weights, D, partials.p, basis.p, stream));
                     HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p, stream));
-                    HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p, kw.p, nullptr, D, partials.p, 0,
+                    HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p, kw.p, weights, D, partials.p, 0,
                                                basis.p, stream));
                     HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p + 32, stream));
-                    HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p + 32, kw.p, nullptr, D, partials.p, 1,
+                    HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p + 32, kw.p, weights, D, partials.p, 1,
                                                basis.p, stream));
                     HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 1, scalars.p + 64, stream));
                     HIPCHECK(hipMemcpyAsync(host_scalars, scalars.p, 65 * sizeof(double), hipMemcpyDeviceToHost,
@@ -446,7 +467,7 @@ struct dgh_ctx {
                 for (int k = 0; k <= j; ++k) Hm(k, j) = (0.0 + host_scalars[k]) + host_scalars[32 + k];
                 const double next = std::sqrt(std::max(0.0, host_scalars[64]));
                 Hm(j + 1, j) = next;
-                if (next > 0) scale_div(kw.p, next, Vk(j + 1));
+                if (next > 0) HIPCHECK(launch_scale_div(kw.p, next, Vk(j + 1), D, stream));
                 for (int i = 0; i < j; ++i) {
                     const double a = Hm(i, j), b = Hm(i + 1, j);
                     Hm(i, j) = cs[i] * a + sn[i] * b;
@@ -471,15 +492,95 @@ struct dgh_ctx {
             }
             HIPCHECK(hipMemcpyAsync(ky.p, y.data(), count * sizeof(double), hipMemcpyHostToDevice, stream));
             HIPCHECK(launch_combine(V.data(), count, ky.p, kwork.p, D, stream));
-            bj_apply(kwork.p, kw.p);
+            M(kwork.p, kw.p);
             axpby(1.0, x, 1.0, kw.p, x, D);
-            dg_apply(x, kwork.p);
+            A(x, kwork.p);
             axpby(1.0, rhs, -1.0, kwork.p, kres.p, D);
-            const double relative = gmres_norm(kres.p) / rhs_norm;
+            const double relative = vec_norm(kres.p, D, weights) / rhs_norm;
             require(std::isfinite(relative), "Nonfinite GMRES residual");
             if (relative <= tolerance) return {iterations, relative};
         }
         throw std::runtime_error("GMRES iteration limit reached");
+    }
+
+    // ---- HDG (hybrid.cpp:31-121)
+    void hdg_refresh_projected(const double *u, const double *r) {
+        dg_refresh(u);  // volume + face linearisation caches (hybrid.cpp:57)
+        const long long t = 16LL * n;
+        hX.alloc(static_cast<size_t>(cells) * m * t, bytes);
+        hY.alloc(static_cast<size_t>(D), bytes);
+        hS.alloc(static_cast<size_t>(cells) * t * t, bytes);
+        hfres.alloc(static_cast<size_t>(cells) * t, bytes);
+        hloc.alloc(static_cast<size_t>(cells) * t, bytes);
+        hrhs.alloc(static_cast<size_t>(T), bytes);
+        // element-streamed: element blocks factorised in batches of at most ~40 GB
+        const long long per = static_cast<long long>(m) * m * 8;
+        const int batch = static_cast<int>(std::max<long long>(1, std::min<long long>(cells, 40000000000LL / per)));
+        factors.alloc(static_cast<size_t>(batch) * m * m, bytes);
+        rowsrc.alloc(static_cast<size_t>(batch) * m, bytes);
+        bj_ready = false;
+        set_flag(2, 0);
+        for (int first = 0; first < cells; first += batch) {
+            const int count = std::min(batch, cells - first);
+            timed("hdg_condense", [&] {
+                HIPCHECK(launch_assemble_hdg(n, vcache.p, fcache.p, factors.p, inv_step, basis.p, geo, first, count,
+                                             stream));
+                HIPCHECK(launch_lu(m, factors.p, rowsrc.p, count, flags.p + 2, stream));
+                HIPCHECK(launch_hdg_solve(n, factors.p, rowsrc.p, fcache.p, r, hX.p, hY.p, first, count, basis.p,
+                                          geo, stream));
+            });
+        }
+        require(read_flag(2) == 0, "Singular or nonfinite LU factors");
+        timed("hdg_condense", [&] {
+            HIPCHECK(launch_hdg_cx(n, hX.p, hY.p, fcache.p, hS.p, hfres.p, basis.p, geo, stream));
+            HIPCHECK(launch_hdg_face_sum(n, nullptr, 0.0, nullptr, hfres.p, hrhs.p, geo, stream));
+        });
+        hdg_ready = true;
+        face_bj_ready = false;
+    }
+    void hdg_apply(const double *x, double *y) {
+        require(hdg_ready, "Refresh the HDG condensation before applying it");
+        hdiag.alloc(static_cast<size_t>(T), bytes);
+        timed("schur", [&] {
+            HIPCHECK(launch_hdg_diag_x(n, fcache.p, x, hdiag.p, basis.p, geo, stream));
+            HIPCHECK(launch_hdg_schur(n, hS.p, x, hloc.p, geo, stream));
+            HIPCHECK(launch_hdg_face_sum(n, nullptr, 0.0, hdiag.p, hloc.p, y, geo, stream));
+        });
+    }
+    void hdg_bj_refresh() {
+        require(hdg_ready, "Refresh the HDG condensation before assembling a block");
+        face_factors.alloc(static_cast<size_t>(faces) * fw * fw, bytes);
+        face_rowsrc.alloc(static_cast<size_t>(faces) * fw, bytes);
+        set_flag(2, 0);
+        timed("face_lu", [&] {
+            HIPCHECK(launch_hdg_face_blocks(n, hS.p, fcache.p, face_factors.p, basis.p, geo, stream));
+            HIPCHECK(launch_lu(fw, face_factors.p, face_rowsrc.p, faces, flags.p + 2, stream));
+        });
+        require(read_flag(2) == 0, "Singular block Jacobi factor");
+        face_bj_ready = true;
+    }
+    void hdg_bj_apply(const double *r, double *z) {
+        require(face_bj_ready, "Refresh the block Jacobi preconditioner first");
+        timed("face_bj_apply", [&] { HIPCHECK(launch_bj_apply(fw, face_factors.p, face_rowsrc.p, r, z, faces, stream)); });
+    }
+    void hdg_recover(const double *dl, double *du) {
+        require(hdg_ready, "Refresh the HDG condensation before applying it");
+        HIPCHECK(launch_hdg_recover(n, hX.p, hY.p, dl, du, geo, stream));
+    }
+    void hdg_recovered_residual(const double *e, double *out) {
+        require(hdg_ready, "Refresh the HDG condensation before applying it");
+        HIPCHECK(launch_hdg_recovered_residual(n, e, fcache.p, out, basis.p, geo, stream));
+    }
+    const double *inverse_trace_mass() {
+        if (!itm.p) {
+            itm.alloc(static_cast<size_t>(T), bytes);
+            HIPCHECK(launch_inverse_trace_mass(n, itm.p, basis.p, geo, stream));
+        }
+        return itm.p;
+    }
+    Linear hdg_gmres(const double *rhs, double *x, int limit, double tolerance) {
+        return gmres_generic(T, inverse_trace_mass(), [&](const double *a, double *b) { hdg_apply(a, b); },
+                             [&](const double *a, double *b) { hdg_bj_apply(a, b); }, rhs, x, limit, tolerance);
     }
     void scale_div(const double *src, double d, double *dst) { HIPCHECK(launch_scale_div(src, d, dst, D, stream)); }
 
@@ -715,7 +816,8 @@ struct dgh_ctx {
     // ---- TimeIntegrator::advance (time_integrator.cpp:48-213), DG method
     void advance(double *solution, dgh_run_stats &stats, std::vector<dgh_update_stats> &updates) {
         const auto &c = cfg;
-        require(c.method != DGH_METHOD_HDG, "HDG method is not available on the device path yet");
+        require(c.method != DGH_METHOD_HDG || c.precision == DGH_PRECISION_FP64,
+                "Mixed precision is available for the DG method only");
         require(!c.direct, "--linear direct is a host reference and is not available on the device path");
         require(admissible(solution), "Invalid initial solution");
         const double shock = (c.testcase == DGH_WAVE || c.testcase == DGH_SEEDED)
@@ -781,10 +883,34 @@ struct dgh_ctx {
                     const double allowance = 128 * std::numeric_limits<double>::epsilon() *
                                              ((state_norm(state.p) + old_norm) / tau + norm);
                     const double target = std::max(10 * c.linear_tolerance * norm, allowance);
-                    dg_refresh(state.p);
-                    axpby(-1.0, residual.p, 0.0, nullptr, rhs.p, D);
                     Linear linear{0, 0};
                     int outer = 0;
+                    double precision_error = -1;
+                    if (c.method == DGH_METHOD_HDG) {
+                        // time_integrator.cpp:164-191: projected condensation, face
+                        // block Jacobi, Schur GMRES with up to four tolerance cuts,
+                        // acceptance on the recovered DG residual
+                        hdg_refresh_projected(state.p, residual.p);
+                        hdg_bj_refresh();
+                        trace_dir.alloc(static_cast<size_t>(T), bytes);
+                        hlambda.alloc(static_cast<size_t>(T), bytes);
+                        double tolerance = c.linear_tolerance;
+                        bool valid = false;
+                        for (int attempt = 0; attempt < 4; ++attempt) {
+                            const Linear current = hdg_gmres(hrhs.p, trace_dir.p, c.max_linear, tolerance);
+                            linear.iterations += current.iterations;
+                            linear.relative = current.relative;
+                            hdg_recover(trace_dir.p, direction.p);
+                            axpby(-1.0, kres.p, 0.0, nullptr, hlambda.p, T);
+                            hdg_recovered_residual(hlambda.p, equation_error.p);
+                            valid = dual_norm(equation_error.p) <= target;
+                            if (valid) break;
+                            tolerance *= .01;
+                        }
+                        require(valid, "Recovered DG residual criterion failed after Schur refinement");
+                    } else {
+                    dg_refresh(state.p);
+                    axpby(-1.0, residual.p, 0.0, nullptr, rhs.p, D);
                     if (c.precision == DGH_PRECISION_FP64) {
                         bj_refresh();
                         linear = gmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance);
@@ -793,7 +919,6 @@ struct dgh_ctx {
                         linear = fgmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance, outer);
                     }
                     axpby(-1.0, kres.p, 0.0, nullptr, equation_error.p, D);
-                    double precision_error = -1;
                     if (c.precision != DGH_PRECISION_FP64 && c.check_update) {
                         // the same Newton system through the FP64 path
                         check_dir.alloc(D, bytes);
@@ -802,8 +927,19 @@ struct dgh_ctx {
                         axpby(1.0, direction.p, -1.0, check_dir.p, kwork.p, D);
                         precision_error = state_norm(kwork.p) / state_norm(check_dir.p);
                     }
+                    }
                     const double error_norm = dual_norm(equation_error.p);
                     require(error_norm <= target, "DG Newton linear residual too large");
+                    if (c.method == DGH_METHOD_HDG && c.check_update) {
+                        // time_integrator.cpp:194-203: independent DG check of the recovered update
+                        indep.alloc(D, bytes);
+                        dg_apply(direction.p, indep.p);
+                        axpby(1.0, indep.p, 1.0, residual.p, indep.p, D);
+                        require(dual_norm(indep.p) <= std::max(4 * target, 1e-12),
+                                "Independent recovered DG equation check failed");
+                        axpby(1.0, indep.p, -1.0, equation_error.p, indep.p, D);
+                        require(dual_norm(indep.p) <= std::max(4 * allowance, 1e-12), "Recovered residual identity failed");
+                    }
                     double damping = 1.;
                     bool accepted = false;
                     for (int search = 0; search < 30; ++search) {
@@ -1117,6 +1253,84 @@ int dgh_dg_gmres(dgh_ctx *ctx, const double *rhs, double *x, int max_iterations,
         check_ctx(ctx);
         const auto r = ctx->gmres(rhs, x, max_iterations, tolerance);
         if (residual) ctx->copy(residual, ctx->kres.p, ctx->D);
+        ctx->sync();
+        if (iterations) *iterations = r.iterations;
+        if (relative_residual) *relative_residual = r.relative;
+    });
+}
+int dgh_hdg_refresh_projected(dgh_ctx *ctx, const double *u, const double *r) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->hdg_refresh_projected(u, r);
+        ctx->sync();
+    });
+}
+int dgh_hdg_rhs(dgh_ctx *ctx, double *out) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        require(ctx->hdg_ready, "Refresh the HDG condensation before applying it");
+        ctx->copy(out, ctx->hrhs.p, ctx->T);
+        ctx->sync();
+    });
+}
+int dgh_hdg_apply(dgh_ctx *ctx, const double *x, double *y) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->hdg_apply(x, y);
+        ctx->sync();
+    });
+}
+int dgh_hdg_block(dgh_ctx *ctx, int face, double *out_host) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        require(ctx->hdg_ready, "Refresh the HDG condensation before assembling a block");
+        require(face >= 0 && face < ctx->faces, "Invalid block coordinates");
+        {
+            ctx->face_factors.alloc(static_cast<size_t>(ctx->faces) * ctx->fw * ctx->fw, ctx->bytes);
+            HIPCHECK(launch_hdg_face_blocks(ctx->n, ctx->hS.p, ctx->fcache.p, ctx->face_factors.p, ctx->basis.p,
+                                            ctx->geo, ctx->stream));
+            ctx->face_bj_ready = false;
+        }
+        const size_t fw2 = static_cast<size_t>(ctx->fw) * ctx->fw;
+        HIPCHECK(hipMemcpyAsync(out_host, ctx->face_factors.p + fw2 * face, fw2 * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+    });
+}
+int dgh_hdg_bj_refresh(dgh_ctx *ctx) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->hdg_bj_refresh();
+        ctx->sync();
+    });
+}
+int dgh_hdg_bj_apply(dgh_ctx *ctx, const double *r, double *z) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->hdg_bj_apply(r, z);
+        ctx->sync();
+    });
+}
+int dgh_hdg_recover(dgh_ctx *ctx, const double *dlambda, double *du) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->hdg_recover(dlambda, du);
+        ctx->sync();
+    });
+}
+int dgh_hdg_recovered_residual(dgh_ctx *ctx, const double *schur_error, double *out) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->hdg_recovered_residual(schur_error, out);
+        ctx->sync();
+    });
+}
+int dgh_hdg_gmres(dgh_ctx *ctx, const double *rhs, double *x, int max_iterations, double tolerance, int *iterations,
+                  double *relative_residual, double *residual) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        const auto r = ctx->hdg_gmres(rhs, x, max_iterations, tolerance);
+        if (residual) ctx->copy(residual, ctx->kres.p, ctx->T);
         ctx->sync();
         if (iterations) *iterations = r.iterations;
         if (relative_residual) *relative_residual = r.relative;

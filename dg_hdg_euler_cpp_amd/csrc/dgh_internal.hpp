@@ -73,9 +73,10 @@ hipError_t launch_error_partials(int n, const double *u, double time, double *pa
 
 // Reductions (two-pass, fixed order, deterministic).
 constexpr int REDUCE_BLOCKS = 1024;
-// mode 0: sum x^2 * inv_mass(node); 1: sum x^2 * mass(node); 2: plain sum x^2
-hipError_t launch_weighted_sq(int n, const double *x, long long count, int mode, double *partials,
-                              const Basis *b, hipStream_t s);
+// mode 0: sum x^2 * inv_mass(node); 1: sum x^2 * mass(node); 2: plain sum x^2;
+// 3: sum x^2 * weights[i]
+hipError_t launch_weighted_sq(int n, const double *x, long long count, int mode, const double *weights,
+                              double *partials, const Basis *b, hipStream_t s);
 hipError_t launch_integral(int n, const double *u, long long cells, double *partials, const Basis *b,
                            hipStream_t s);
 hipError_t launch_finish_sum(const double *partials, int nparts, int width, double *out, hipStream_t s);
@@ -131,5 +132,26 @@ hipError_t launch_scale_div(const float *x, double d, float *out, long long coun
 hipError_t launch_scale_div(const double *x, double d, float *out, long long count, hipStream_t s);
 // out = (x - y) / d
 hipError_t launch_sub_div(const double *x, const double *y, double d, double *out, long long count, hipStream_t s);
+
+// Statically condensed HDG (hybrid.cpp:31-121).
+hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache, double *blocks, double inv_step,
+                               const Basis *b, const Geo &g, int first, int count, hipStream_t s);
+hipError_t launch_hdg_solve(int n, const double *factors, const int *rowsrc, const double *fcache,
+                            const double *resid, double *X, double *Y, int first, int count, const Basis *b,
+                            const Geo &g, hipStream_t s);
+hipError_t launch_hdg_cx(int n, const double *X, const double *Y, const double *fcache, double *S, double *fres,
+                         const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_hdg_face_sum(int n, const double *base, double base_scale, const double *diag_x, const double *loc,
+                               double *out, const Geo &g, hipStream_t s);
+hipError_t launch_hdg_diag_x(int n, const double *fcache, const double *x, double *out, const Basis *b, const Geo &g,
+                             hipStream_t s);
+hipError_t launch_hdg_schur(int n, const double *S, const double *x, double *loc, const Geo &g, hipStream_t s);
+hipError_t launch_hdg_face_blocks(int n, const double *S, const double *fcache, double *blocks, const Basis *b,
+                                  const Geo &g, hipStream_t s);
+hipError_t launch_hdg_recover(int n, const double *X, const double *Y, const double *dl, double *du, const Geo &g,
+                              hipStream_t s);
+hipError_t launch_hdg_recovered_residual(int n, const double *e, const double *fcache, double *out, const Basis *b,
+                                         const Geo &g, hipStream_t s);
+hipError_t launch_inverse_trace_mass(int n, double *out, const Basis *b, const Geo &g, hipStream_t s);
 
 }  // namespace dgh

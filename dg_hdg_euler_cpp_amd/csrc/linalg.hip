@@ -45,7 +45,8 @@ __device__ __forceinline__ double block_sum(double v, double *smem) {
 // norm, :45-47), mode 2 plain x^2.
 template <int N>
 __global__ __launch_bounds__(BLOCK) void k_weighted_sq(const double *__restrict__ x, long long count, int mode,
-                                                       double *partials, const Basis *__restrict__ B) {
+                                                       const double *__restrict__ weights, double *partials,
+                                                       const Basis *__restrict__ B) {
     constexpr int P = N * N;
     __shared__ double smem[BLOCK / 64];
     double acc = 0;
@@ -53,7 +54,10 @@ __global__ __launch_bounds__(BLOCK) void k_weighted_sq(const double *__restrict_
          i += static_cast<long long>(gridDim.x) * BLOCK) {
         const double v = x[i];
         const int node = static_cast<int>(i % P);
-        acc += mode == 0 ? v * v * B->inv_mass[node] : mode == 1 ? v * v * B->mass[node] : v * v;
+        acc += mode == 0 ? v * v * B->inv_mass[node]
+             : mode == 1 ? v * v * B->mass[node]
+             : mode == 3 ? v * v * weights[i]
+                         : v * v;
     }
     const double t = block_sum(acc, smem);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
@@ -99,7 +103,10 @@ __global__ __launch_bounds__(BLOCK) void k_finish_sum(const double *__restrict__
 // + [j==jy] (-l_jx H0[v][w](q=j)) l_i + (r_jx H1[v][w]) r_i
 // + [i==jx] (-l_jy H2[v][w](q=i)) l_j + (r_jy H3[v][w]) r_j
 // with H_s the HLL derivative of the face on side s (face_jacobian.cpp:69-73).
-template <int N, typename T>
+// With HDG = true the face terms are those of HybridOperator::assemble_cell
+// (hybrid.cpp:31-55): H_s = w_q ((+-1) A_phys + tau I) on every side with the
+// same shapes, added with sign +1 (face_jacobian.cpp:77-82).
+template <int N, typename T, bool HDG>
 __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ vc, const double *__restrict__ fc,
                                                     T *__restrict__ blocks, double inv_step,
                                                     const Basis *__restrict__ B, Geo g, int first) {
@@ -149,8 +156,19 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ v
             a[8] = .5 * gm * u * u + .5 * (gamma - 3) * v * v; a[9] = -gm * u; a[10] = (3 - gamma) * v; a[11] = gm;
             a[12] = v * (.5 * gm * s2 - H); a[13] = -gm * u * v; a[14] = H - gm * v * v; a[15] = gamma * v;
         }
-        for (int k = 0; k < 16; ++k) a[k] *= blend;
-        for (int k = 0; k < 4; ++k) a[k * 5] += left ? pen : -pen;
+        if (HDG) {
+            const double tau = fc[(left ? F_TAUL : F_TAUR) * nodes + fn];
+            const double weight = .5 * (side < 2 ? g.hy : g.hx) * B->w[q];
+            for (int k = 0; k < 16; ++k) a[k] *= left ? 1. : -1.;
+            for (int k = 0; k < 4; ++k) a[k * 5] += tau;
+            for (int k = 0; k < 16; ++k) a[k] *= weight;
+        } else {
+            for (int k = 0; k < 16; ++k) a[k] *= blend;
+            for (int k = 0; k < 4; ++k) a[k * 5] += left ? pen : -pen;
+            // dg_operator.cpp:31: sign -1 on the -x/-y sides (exact, folded into H)
+            if (!left)
+                for (int k = 0; k < 16; ++k) a[k] = -a[k];
+        }
         for (int k = 0; k < 16; ++k) hs[(side * N + q) * 16 + k] = a[k];
     }
     __syncthreads();
@@ -168,12 +186,12 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ v
             }
             if (row == col) val += B->mass[rnode] * inv_step;
             if (j == jy) {
-                val += (-1 * B->left[jx] * hs[(0 * N + j) * 16 + vw]) * B->left[i];
-                val += (1 * B->right[jx] * hs[(1 * N + j) * 16 + vw]) * B->right[i];
+                val += (B->left[jx] * hs[(0 * N + j) * 16 + vw]) * B->left[i];
+                val += (B->right[jx] * hs[(1 * N + j) * 16 + vw]) * B->right[i];
             }
             if (i == jx) {
-                val += (-1 * B->left[jy] * hs[(2 * N + i) * 16 + vw]) * B->left[j];
-                val += (1 * B->right[jy] * hs[(3 * N + i) * 16 + vw]) * B->right[j];
+                val += (B->left[jy] * hs[(2 * N + i) * 16 + vw]) * B->left[j];
+                val += (B->right[jy] * hs[(3 * N + i) * 16 + vw]) * B->right[j];
             }
             out[static_cast<long long>(col) * M + row] = static_cast<T>(val);
         }
@@ -652,9 +670,9 @@ __global__ __launch_bounds__(BLOCK) void k_axpby(double a, const T *__restrict__
     }                                                       \
     return hipGetLastError();
 
-hipError_t launch_weighted_sq(int n, const double *x, long long count, int mode, double *partials, const Basis *b,
-                              hipStream_t s) {
-    DGH_DISPATCH_N(n, { k_weighted_sq<N><<<REDUCE_BLOCKS, BLOCK, 0, s>>>(x, count, mode, partials, b); })
+hipError_t launch_weighted_sq(int n, const double *x, long long count, int mode, const double *weights,
+                              double *partials, const Basis *b, hipStream_t s) {
+    DGH_DISPATCH_N(n, { k_weighted_sq<N><<<REDUCE_BLOCKS, BLOCK, 0, s>>>(x, count, mode, weights, partials, b); })
 }
 hipError_t launch_integral(int n, const double *u, long long cells, double *partials, const Basis *b,
                            hipStream_t s) {
@@ -667,12 +685,22 @@ hipError_t launch_finish_sum(const double *partials, int nparts, int width, doub
 hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fcache, double *blocks,
                                   double inv_step, const Basis *b, const Geo &g, int first, int count,
                                   hipStream_t s) {
-    DGH_DISPATCH_N(n, { k_assemble<N, double><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first); })
+    DGH_DISPATCH_N(n, {
+        k_assemble<N, double, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
+    })
 }
 hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fcache, float *blocks,
                                   double inv_step, const Basis *b, const Geo &g, int first, int count,
                                   hipStream_t s) {
-    DGH_DISPATCH_N(n, { k_assemble<N, float><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first); })
+    DGH_DISPATCH_N(n, {
+        k_assemble<N, float, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
+    })
+}
+hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache, double *blocks, double inv_step,
+                               const Basis *b, const Geo &g, int first, int count, hipStream_t s) {
+    DGH_DISPATCH_N(n, {
+        k_assemble<N, double, true><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
+    })
 }
 hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s) {
     DGH_DISPATCH_M(m, { k_lu<M, 16, double><<<nblocks, BLOCK, 0, s>>>(blocks, rowsrc, nblocks, flag); })

@@ -255,6 +255,57 @@ class DeviceContext:
                    C.byref(its), C.byref(rel), _ptr(res))
         return x, its.value, rel.value, res
 
+    # ---- condensed HDG
+    def tzeros(self):
+        return self.zeros(self.trace_size)
+
+    def hdg_refresh_projected(self, u, r):
+        self._call("dgh_hdg_refresh_projected", _ptr(u), _ptr(r))
+
+    def hdg_rhs(self):
+        out = self.tzeros()
+        self._call("dgh_hdg_rhs", _ptr(out))
+        return out
+
+    def hdg_apply(self, x, out=None):
+        y = self.tzeros() if out is None else out
+        self._call("dgh_hdg_apply", _ptr(x), _ptr(y))
+        return y
+
+    def hdg_block(self, face):
+        import numpy as np
+
+        out = np.zeros(self.fw * self.fw)
+        self._call("dgh_hdg_block", C.c_int(face), out.ctypes.data_as(_D))
+        return out.reshape(self.fw, self.fw, order="F")
+
+    def hdg_bj_refresh(self):
+        self._call("dgh_hdg_bj_refresh")
+
+    def hdg_bj_apply(self, r, out=None):
+        z = self.tzeros() if out is None else out
+        self._call("dgh_hdg_bj_apply", _ptr(r), _ptr(z))
+        return z
+
+    def hdg_recover(self, dl, out=None):
+        du = self.zeros() if out is None else out
+        self._call("dgh_hdg_recover", _ptr(dl), _ptr(du))
+        return du
+
+    def hdg_recovered_residual(self, e, out=None):
+        r = self.zeros() if out is None else out
+        self._call("dgh_hdg_recovered_residual", _ptr(e), _ptr(r))
+        return r
+
+    def hdg_gmres(self, rhs, max_iterations, tolerance):
+        x = self.tzeros()
+        res = self.tzeros()
+        its = C.c_int()
+        rel = C.c_double()
+        self._call("dgh_hdg_gmres", _ptr(rhs), _ptr(x), C.c_int(max_iterations), C.c_double(tolerance),
+                   C.byref(its), C.byref(rel), _ptr(res))
+        return x, its.value, rel.value, res
+
     def advance(self, u, capacity=100000):
         run = RunStats()
         upd = (UpdateStats * capacity)()

@@ -179,6 +179,29 @@ int dgh_dg_fgmres(dgh_ctx *ctx, const double *rhs, double *x, int max_iterations
 int dgh_dg_gmres(dgh_ctx *ctx, const double *rhs, double *x, int max_iterations, double tolerance,
                  int *iterations, double *relative_residual, double *residual);
 
+/* ---- condensed HDG (HybridOperator) ------------------------------------- */
+/* HybridOperator::refresh_projected (hybrid.cpp:28-30, 56-72): linearise at u,
+ * condense every element with the DG residual r as volume residual and a
+ * zero skeleton residual (element blocks factorised in batches). */
+int dgh_hdg_refresh_projected(dgh_ctx *ctx, const double *u, const double *r);
+/* HybridOperator::rhs (hybrid.hpp:27): -g + sum_K C_K y_K, trace vector. */
+int dgh_hdg_rhs(dgh_ctx *ctx, double *out);
+/* HybridOperator::apply (hybrid.cpp:73-82): y = D x + sum_K scatter(S_K gather(x)). */
+int dgh_hdg_apply(dgh_ctx *ctx, const double *x, double *y);
+/* HybridOperator::block(face, face) (hybrid.cpp:83-96), column-major 4n x 4n, HOST buffer. */
+int dgh_hdg_block(dgh_ctx *ctx, int face, double *out_host);
+/* BlockJacobi over the face blocks: refresh (LU of every face block) and apply. */
+int dgh_hdg_bj_refresh(dgh_ctx *ctx);
+int dgh_hdg_bj_apply(dgh_ctx *ctx, const double *r, double *z);
+/* HybridOperator::recover (hybrid.cpp:97-105): du = -X dlambda - y. */
+int dgh_hdg_recover(dgh_ctx *ctx, const double *dlambda, double *du);
+/* HybridOperator::recovered_residual_from_schur (hybrid.cpp:106-121). */
+int dgh_hdg_recovered_residual(dgh_ctx *ctx, const double *schur_error, double *out);
+/* GMRES on the Schur system with the face block Jacobi and the inverse face
+ * mass metric (time_integrator.cpp:27, 175-176); residual may be NULL. */
+int dgh_hdg_gmres(dgh_ctx *ctx, const double *rhs, double *x, int max_iterations, double tolerance,
+                  int *iterations, double *relative_residual, double *residual);
+
 /* ---- whole Newton / time loop (TimeIntegrator) -------------------------- */
 /* TimeIntegrator::advance (time_integrator.cpp:48-213) on the device for the
  * configured method. u is advanced in place. updates may be NULL. */

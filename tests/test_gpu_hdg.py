@@ -1,0 +1,135 @@
+"""Condensed HDG on the device vs the oracle's HybridOperator (hybrid.cpp:31-121)
+and the reference's archived HDG runs.
+
+The condensation solves with the element LU, so the forward agreement of X,
+S_K and the rhs is bounded by kappa(A_K) * eps like the block-Jacobi apply;
+the reference's own identities (check_2d.cpp:72-84) are checked exactly.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_runs.json")))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def face_metric(orc):
+    """1 / face_weight per trace entry (inverse trace mass, time_integrator.cpp:27)."""
+    w = orc.basis()["w"]
+    n, C = orc.n, orc.cells
+    out = np.zeros(orc.trace_size)
+    for f in range(orc.faces):
+        h = orc.hy if f < C else orc.hx
+        for v in range(4):
+            for q in range(n):
+                out[f * 4 * n + v * n + q] = 1.0 / (0.5 * h * w[q])
+    return out
+
+
+CASES = [("k2-4x4", ["--nx", "4", "--ny", "4", "--k", "2"], 1e-11),
+         ("k4-5x3", ["--nx", "5", "--ny", "3", "--k", "4", "--testcase", "wave"], 1e-10),
+         ("k8-6x6", ["--nx", "6", "--ny", "6", "--k", "8"], 1e-9)]
+
+
+@pytest.mark.parametrize("name,args,tol", CASES, ids=[c[0] for c in CASES])
+def test_hdg_operator_parity(name, args, tol, torch_cuda):
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    print(f"\n[case] hdg {args}", flush=True)
+    orc = O.Oracle(args + ["--cfl", "5", "--integrator", "be"])
+    dev = DeviceContext(args + ["--cfl", "5", "--integrator", "be"])
+    u_o = orc.initial()
+    u = torch_cuda.tensor(u_o, device="cuda")
+    rate = orc.freeze(u_o)
+    dev.freeze(u)
+    tau = 5.0 / rate
+    orc.set_step(tau, u_o)
+    dev.set_step(tau, u)
+    r_o = orc.dg_residual(u_o)
+    r = torch_cuda.tensor(r_o, device="cuda")
+    orc.hdg_refresh_projected(u_o, r_o)
+    dev.hdg_refresh_projected(u, r)
+    # condensed right-hand side -g + sum C y
+    assert rel(dev.hdg_rhs().cpu().numpy(), orc.hdg_rhs()) < tol, "Schur rhs"
+    rng = np.random.default_rng(4)
+    dl = rng.standard_normal(orc.trace_size)
+    dlt = torch_cuda.tensor(dl, device="cuda")
+    sdl_o = orc.hdg_apply(dl)
+    sdl = dev.hdg_apply(dlt).cpu().numpy()
+    assert rel(sdl, sdl_o) < tol, "Schur apply"
+    for f in [0, orc.faces // 2, orc.faces - 1]:
+        assert rel(dev.hdg_block(f), orc.hdg_block(f, f)) < tol, f"face block {f}"
+    du_o = orc.hdg_recover(dl)
+    du = dev.hdg_recover(dlt).cpu().numpy()
+    assert rel(du, du_o) < tol, "recovery"
+    # check_2d.cpp:82-85 recovered DG residual identity on the device
+    e = dev.hdg_apply(dlt) - dev.hdg_rhs()
+    check = dev.hdg_recovered_residual(e).cpu().numpy()
+    dev.dg_refresh(u)
+    ref = dev.dg_apply(torch_cuda.tensor(du, device="cuda")).cpu().numpy() + r_o
+    assert np.linalg.norm(check - ref) / max(1.0, np.linalg.norm(ref)) < 3e-11
+    assert rel(check, orc.hdg_recovered_residual(e.cpu().numpy())) < 1e-12
+    # face block-Jacobi + Schur GMRES: the device solution satisfies the
+    # oracle's Schur system to the reference's ltol
+    dev.hdg_refresh_projected(u, r)
+    dev.hdg_bj_refresh()
+    rhs_o = orc.hdg_rhs()
+    x, its, reld, _ = dev.hdg_gmres(torch_cuda.tensor(rhs_o, device="cuda"), 1500, 1e-12)
+    x = x.cpu().numpy()
+    assert reld <= 1e-12 and its > 0
+    w = face_metric(orc)
+    res = orc.hdg_apply(x) - rhs_o
+    assert np.sqrt(np.sum(res ** 2 * w) / np.sum(rhs_o ** 2 * w)) <= 5e-12
+    dev.close()
+
+
+@pytest.mark.parametrize("key", ["be_4x4_k2", "dirk3_4x4_k2", "entropy_4x4_k2"])
+def test_hdg_runs_match_reference(key, torch_cuda):
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    run = GOLD["runs"][key]
+    with DeviceContext(run["args"] + ["--method", "hdg"]) as dev:  # --check-update is in the args
+        u = dev.initial()
+        st = dev.advance(u)
+        err = dev.error_l2(u, st["final_time"])
+    ref = [r for r in run["iterations"] if r["method"] == "hdg"]
+    assert len(st["updates"]) == len(ref)
+    tol = [1e-13, 1e-10, 1e-6]
+    for got, want in zip(st["updates"], ref):
+        it = int(want["iteration"])
+        w = float(want["residual_physical_rms"])
+        first = float([r for r in ref if r["stage"] == want["stage"] and r["iteration"] == "0"][0]["residual_physical_rms"])
+        assert abs(got["residual_physical_rms"] - w) <= max(tol[it] * w, 1e-12 * first), (it, got, w)
+        assert abs(got["krylov_iterations"] - int(want["krylov_iterations"])) <= 1
+    summ = [s for s in run["summary"] if s["method"] == "hdg"][0]
+    for v, k in enumerate(["rho_L2", "momentum_x_L2", "momentum_y_L2", "energy_L2"]):
+        assert abs(err[v] - float(summ[k])) <= 1e-9 * float(summ[k])
+    assert st["conservation_drift"] <= 2e-11
+
+
+def test_hdg_k8_256_first_update(torch_cuda):
+    """256^2, k=8 HDG: RMS0 and the first update's Schur Krylov count
+    (k8_hdg_iterations.csv:2-3: 167 iterations, RMS 4.0e-4 after it)."""
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    run = GOLD["runs"]["k8_256_be_hdg"]
+    args = ["--nx", "256", "--ny", "256", "--k", "8", "--method", "hdg", "--integrator", "be", "--steps", "1",
+            "--newton-steps", "2", "--cfl", "5", "--ntol", "1e-12", "--ltol", "1e-12", "--restart", "20",
+            "--max-linear", "1500", "--safety", "1.1", "--timing-only"]
+    with DeviceContext(args) as dev:
+        u = dev.initial()
+        st = dev.advance(u)
+    ref = run["iterations"]
+    for it, got in enumerate(st["updates"]):
+        w = float(ref[it]["residual_physical_rms"])
+        assert abs(got["residual_physical_rms"] - w) <= [1e-13, 1e-10][it] * w, (it, got["residual_physical_rms"], w)
+    assert abs(st["updates"][0]["krylov_iterations"] - int(ref[0]["krylov_iterations"])) <= 3
