@@ -28,14 +28,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def solver_args(nx, k, init, precision, newton_steps, warmup, device=True):
-    args = ["--nx", str(nx), "--ny", str(nx), "--k", str(k), "--method", "dg", "--integrator", "be",
+def solver_args(nx, k, init, precision, newton_steps, warmup, device=True, method="dg"):
+    args = ["--nx", str(nx), "--ny", str(nx), "--k", str(k), "--method", method, "--integrator", "be",
             "--steps", "1", "--newton-steps", str(newton_steps), "--warmup-updates", str(warmup), "--cfl", "5",
             "--ntol", "1e-12", "--ltol", "1e-12", "--restart", "20", "--max-linear", "1500", "--safety", "1.1",
             "--timing-only", "--testcase", init]
     if device:
         args += ["--seed", "2025", "--precision", precision]
     return args
+
+
+def run_hdg(args, nx, k, init):
+    """Condensed HDG (FP64) at the same workload: Newton updates/s and Schur
+    Krylov iterations (the reference's second method, config 3)."""
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    with DeviceContext(solver_args(nx, k, init, "fp64", args.warmup + args.steps, args.warmup,
+                                   method="hdg")) as dev:
+        u = dev.initial()
+        stats = dev.advance(u)
+    measured = stats["measured_newton_seconds"]
+    upd = stats["updates"]
+    log(f"[bench] hdg fp64 {init}: updates={len(upd)} mean_s={measured / max(1, stats['measured_updates']):.4f}")
+    for x in upd:
+        log(f"   update {x['iteration']}: rms={x['residual_physical_rms']:.6e} krylov={x['krylov_iterations']} "
+            f"t={x['iteration_s']:.4f}s")
+    return {"updates_per_s": stats["measured_updates"] / measured if measured > 0 else None,
+            "krylov_per_update": sum(x["krylov_iterations"] for x in upd[args.warmup:]) / max(1, len(upd) - args.warmup),
+            "rms": [x["residual_physical_rms"] for x in upd]}
 
 
 def run_device(args, nx, k, init, precision):
@@ -151,6 +171,7 @@ def main():
     ap.add_argument("--init", default="diagonal", choices=["seeded", "diagonal"])
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed", "mixed-bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-hdg", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X; multi-GPU runs are not part of this build "
@@ -190,6 +211,7 @@ def main():
         equal = {"rms_fp64": a, "rms_mixed": b,
                  "max_rel_diff_first_two_updates": max(abs(x - y) / abs(x) for x, y in zip(a[:2], b[:2])),
                  "speedup_mixed_over_fp64": forms_out["mixed"]["updates_per_s"] / forms_out["fp64"]["updates_per_s"]}
+    hdg = None if args.no_hdg or args.precision != "both" else run_hdg(args, args.nx, args.k, args.init)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.nx, args.k, args.init, forms_out.get("fp64", forms_out[head])["krylov_per_update"])
@@ -215,6 +237,7 @@ def main():
                      "algorithmic_bytes": bytes_bj, "avg_launch_s": dev["bj_avg_s"]},
         "forms": forms_out,
         "equal_residual": equal,
+        "hdg_fp64": hdg,
         "jv": {"dof_per_s": g["size"] / dev["jv_avg_s"], "achieved_GBs": jv_bytes(g) / dev["jv_avg_s"] / 1e9,
                "avg_launch_s": dev["jv_avg_s"]},
         "cpu_baseline": None if cpu is None else {k: v for k, v in cpu.items() if k != "seconds_per_update"},
