@@ -161,6 +161,46 @@ def cpu_baseline(nx_full, k, init, krylov_per_update):
                 seconds_per_update=t_update)
 
 
+def probe(args):
+    """Each hot kernel a few times on one 256^2 k=8 linearisation, beside a
+    plain device copy of known size (the scale for hardware byte counters):
+    the program that counter runs (rocprofv3 --pmc) profile."""
+    import torch
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[probe] copy 1 GiB read + 1 GiB write x3")
+    del src, dst
+    with DeviceContext(solver_args(args.nx, args.k, args.init, "mixed", 1, 0)) as dev:
+        u = dev.initial()
+        rate = dev.freeze(u)
+        dev.set_step(5.0 / rate, u)
+        dev.dg_refresh(u)
+        x = torch.randn(dev.size, dtype=torch.float64, device="cuda")
+        y = dev.zeros()
+        for _ in range(3):
+            dev.dg_apply(x, y)
+        for _ in range(3):
+            dev.dg_apply_transpose(x, y)
+        log("[probe] jv x3, jtv x3")
+        dev.bj_refresh()
+        for _ in range(3):
+            dev.bj_apply(x, y)
+        log("[probe] bj_apply fp64 x3")
+        dev.bj_refresh_mixed()
+        x32 = x.to(torch.float32)
+        y32 = torch.zeros_like(x32)
+        for _ in range(3):
+            dev.bj_apply_f32(x32, y32)
+        torch.cuda.synchronize()
+        log("[probe] bj_apply fp32 x3")
+    print(json.dumps({"probe": "done", "nx": args.nx, "k": args.k}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,12 +212,28 @@ def main():
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed", "mixed-bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-hdg", action="store_true")
+    ap.add_argument("--only-hdg", action="store_true", help="run and report the condensed HDG form alone")
+    ap.add_argument("--probe", action="store_true", help="run each hot kernel 3x for counter profiles")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X; multi-GPU runs are not part of this build "
               "(the path would shard by 2D element tiles, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.probe:
+        probe(args)
+        return
+    if args.only_hdg:
+        hdg = run_hdg(args, args.nx, args.k, args.init)
+        print(json.dumps({
+            "metric": "2D HDG k=8 256² Newton updates/s (FP64)", "value": hdg["updates_per_s"],
+            "unit": "Newton updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 / hdg["updates_per_s"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": f"synthetic ({args.init} state)",
+            "config": {"workload": f"2D condensed HDG BE Newton update, k={args.k}, {args.nx}x{args.nx}, CFL 5, "
+                                   f"Schur GMRES(20) + face block Jacobi, ntol=ltol=1e-12"},
+            "hdg_fp64": hdg}), flush=True)
+        return
     forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     runs = {p: run_device(args, args.nx, args.k, args.init, p) for p in forms}
     head = forms[-1]  # the headline form: mixed precision when both are run

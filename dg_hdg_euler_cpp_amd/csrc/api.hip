@@ -184,6 +184,14 @@ struct dgh_ctx {
     // Newton
     DevBuf<double> state, trial, residual, trial_residual, direction, rhs, equation_error, step_state, stage_base,
         slopes;
+    // Discrete adjoint: the recorded trajectory (converged stage states and the
+    // wave speeds each stage froze), the step sizes, and the reverse-sweep work
+    enum Trajectory { PLAIN, RECORD, REPLAY };
+    Trajectory trajectory = PLAIN;
+    DevBuf<double> ck_states, ck_speeds, adj_kbar, adj_lambda, adj_rhs, adj_base;
+    std::vector<double> ck_step, ck_rate;
+    int ck_stages = 0;
+    bool adj_bj_ready = false;
 
     explicit dgh_ctx(const dgh_config &c) : cfg(c) {
         n = c.degree + 1;
@@ -236,7 +244,8 @@ struct dgh_ctx {
                                   &factors, &kbasis, &kw, &kz, &kwork, &kres, &ky, &state, &trial, &residual,
                                   &trial_residual, &direction, &rhs, &equation_error, &step_state, &stage_base,
                                   &slopes, &check_dir, &hX, &hY, &hS, &hfres, &hloc, &hdiag, &hrhs, &itm,
-                                  &face_factors, &trace_dir, &hlambda, &indep};
+                                  &face_factors, &trace_dir, &hlambda, &indep, &ck_states, &ck_speeds,
+                                  &adj_kbar, &adj_lambda, &adj_rhs, &adj_base};
         for (auto *b : bufs) b->release(bytes);
         DevBuf<float> *fbufs[] = {&factors_f32, &ibasis, &iw, &it, &is, &ires, &ib, &zbasis};
         for (auto *b : fbufs) b->release(bytes);
@@ -370,6 +379,7 @@ struct dgh_ctx {
         HIPCHECK(launch_face_cache(n, tr_u.p, speeds.p, fcache.p, basis.p, geo, stream));
         refreshed = true;
         bj_ready = false;
+        adj_bj_ready = false;
         bj32_ready = false;
         hdg_ready = false;
         face_bj_ready = false;
@@ -406,6 +416,27 @@ struct dgh_ctx {
         require(bj_ready, "Refresh the block Jacobi preconditioner first");
         timed("bj_apply", [&] { HIPCHECK(launch_bj_apply(m, factors.p, rowsrc.p, r, z, cells, stream)); });
     }
+    // Adjoint block Jacobi: LU of each transposed diagonal block B_i^T, so
+    // that the apply returns B_i^-T r with the same factor/apply kernels.
+    // Shares the factor storage with bj_refresh.
+    void bj_refresh_transposed() {
+        require(refreshed, "Refresh the volume Jacobian before assembling a block");
+        factors.alloc(static_cast<size_t>(cells) * m * m, bytes);
+        rowsrc.alloc(static_cast<size_t>(cells) * m, bytes);
+        bj_ready = false;
+        set_flag(2, 0);
+        timed("assemble", [&] {
+            HIPCHECK(launch_assemble_blocks_transposed(n, vcache.p, fcache.p, factors.p, inv_step, basis.p, geo,
+                                                       stream));
+        });
+        timed("lu", [&] { HIPCHECK(launch_lu(m, factors.p, rowsrc.p, cells, flags.p + 2, stream)); });
+        require(read_flag(2) == 0, "Singular block Jacobi factor");
+        adj_bj_ready = true;
+    }
+    void bj_apply_transposed(const double *r, double *z) {
+        require(adj_bj_ready, "Refresh the transposed block Jacobi preconditioner first");
+        timed("bj_apply", [&] { HIPCHECK(launch_bj_apply(m, factors.p, rowsrc.p, r, z, cells, stream)); });
+    }
 
     // ---- GMRES (gmres.cpp:17-86): restarted, right-preconditioned, weighted
     //      CGS2, Givens, true residual after each cycle. Vectors of length N
@@ -415,6 +446,14 @@ struct dgh_ctx {
     Linear gmres(const double *rhs, double *x, int limit, double tolerance) {
         return gmres_generic(D, nullptr, [&](const double *a, double *b) { dg_apply(a, b); },
                              [&](const double *a, double *b) { bj_apply(a, b); }, rhs, x, limit, tolerance);
+    }
+    // GMRES on J^T lambda = rhs, right-preconditioned by the transposed block
+    // Jacobi; the residual of J^T lambda is a weak-form vector like the
+    // primal one, so the same M^-1 metric applies
+    Linear gmres_transposed(const double *rhs, double *x, int limit, double tolerance) {
+        return gmres_generic(D, nullptr, [&](const double *a, double *b) { dg_apply_transpose(a, b); },
+                             [&](const double *a, double *b) { bj_apply_transposed(a, b); }, rhs, x, limit,
+                             tolerance);
     }
     double vec_norm(const double *x, long long N, const double *weights) {
         return std::sqrt(std::max(0.0, weights ? reduce_sq(x, N, 3, weights) : reduce_sq(x, N, 0)));
@@ -843,11 +882,26 @@ struct dgh_ctx {
         stage_base.alloc(D, bytes);
         if (stages > 1) slopes.alloc(static_cast<size_t>(4) * D, bytes);
         auto slope = [&](int s) { return slopes.p + static_cast<size_t>(s) * D; };
+        const long long speed_count = 2LL * faces * n;
+        if (trajectory == RECORD) {
+            require(c.steps > 0, "Recording a trajectory for the adjoint needs a fixed step count (--steps N)");
+            const size_t records = static_cast<size_t>(c.steps) * stages;
+            ck_states.alloc(records * D, bytes);
+            ck_speeds.alloc(records * speed_count, bytes);
+            ck_step.clear();
+            ck_rate.clear();
+            ck_stages = stages;
+        }
+        if (trajectory == REPLAY)
+            require(!ck_step.empty() && ck_stages == stages, "Record a trajectory before replaying it");
+        const int fixed_steps = trajectory == REPLAY ? static_cast<int>(ck_step.size()) : c.steps;
         const auto begin = Clock::now();
-        while (c.steps > 0 ? stats.steps < c.steps : stats.final_time < c.final_time) {
-            const double rate = freeze(solution);
-            double step = c.cfl / rate;
-            if (c.steps == 0) {
+        while (fixed_steps > 0 ? stats.steps < fixed_steps : stats.final_time < c.final_time) {
+            // a replay takes the recorded step and frozen speeds instead of freezing again
+            const double rate = trajectory == REPLAY ? (load_frozen(stats.steps * stages), ck_rate[stats.steps])
+                                                     : freeze(solution);
+            double step = trajectory == REPLAY ? ck_step[stats.steps] : c.cfl / rate;
+            if (fixed_steps == 0) {
                 const double remaining = c.final_time - stats.final_time;
                 step = std::min(step, remaining);
                 if (remaining - step <= 16 * std::numeric_limits<double>::epsilon() * std::max(remaining, c.final_time))
@@ -861,7 +915,11 @@ struct dgh_ctx {
                 const double diagonal = stages == 1 ? 1. : a[stage][stage];
                 copy(stage_base.p, step_state.p, D);
                 for (int j = 0; j < stage; ++j) axpby(1.0, stage_base.p, step * a[stage][j], slope(j), stage_base.p, D);
-                if (stage > 0) freeze(state.p);
+                const size_t record = static_cast<size_t>(stats.steps) * stages + stage;
+                if (stage > 0) {
+                    if (trajectory == REPLAY) load_frozen(record);
+                    else freeze(state.p);
+                }
                 tau = diagonal * step;
                 copy(old.p, stage_base.p, D);
                 refreshed = false;
@@ -988,7 +1046,15 @@ struct dgh_ctx {
                     updates.push_back(u);
                 }
                 require(converged || c.newton_steps > 0, "Newton iteration limit reached");
+                if (trajectory == RECORD) {
+                    copy(ck_states.p + record * D, state.p, D);
+                    copy(ck_speeds.p + record * speed_count, speeds.p, speed_count);
+                }
                 if (stages > 1) HIPCHECK(launch_sub_div(state.p, stage_base.p, tau, slope(stage), D, stream));
+            }
+            if (trajectory == RECORD) {
+                ck_step.push_back(step);
+                ck_rate.push_back(rate);
             }
             if (stages == 1) copy(solution, state.p, D);
             else {
@@ -1008,6 +1074,80 @@ struct dgh_ctx {
         }
         sync();
         stats.seconds = elapsed(begin);
+    }
+
+    // ---- Discrete adjoint of BE / DIRK3 on a recorded trajectory
+    //
+    // Frozen schedule: the HLL speeds and the step sizes of the recorded run
+    // are constants of the map u_0 -> u_N (the reference never differentiates
+    // the speeds: its Jacobians use the frozen ones), so the gradient is that
+    // of a replay with the same schedule. Stage s of a step of size h solves
+    //   M k_s + F_s(U_s) = 0,  U_s = base_s + tau_s k_s,  tau_s = a_ss h,
+    //   base_s = u_n + h sum_{j<s} a_sj k_j,  u_{n+1} = u_n + h sum_s b_s k_s
+    // (BE: one stage, a = b = 1). With A_s = M / tau_s + dF_s/dU (the Newton
+    // operator at the checkpointed U_s) the reverse sweep is
+    //   kbar_s += h b_s ubar;  A_s^T lambda = kbar_s / tau_s;
+    //   basebar = (M lambda - kbar_s) / tau_s;  ubar += basebar;
+    //   kbar_j += h a_sj basebar (j < s)
+    // with each A_s rebuilt from its checkpoint (recompute, not storage).
+    void load_frozen(size_t record) {
+        const long long speed_count = 2LL * faces * n;
+        require(record < ck_step.size() * static_cast<size_t>(ck_stages), "Trajectory record out of range");
+        copy(speeds.p, ck_speeds.p + record * speed_count, speed_count);
+        frozen = true;
+        refreshed = false;
+    }
+    struct AdjointSweep { int krylov; int solves; double max_relative; };
+    AdjointSweep adjoint_backward(const double *final_gradient, double *gradient) {
+        require(!ck_step.empty() && ck_stages > 0, "Record a trajectory before the adjoint sweep");
+        constexpr double alpha = .24169426078821, beta = alpha / 4., eta = .12915286960590;
+        constexpr double dirk_a[4][4] = {{alpha, 0, 0, 0}, {-alpha, alpha, 0, 0}, {0, 1 - alpha, alpha, 0},
+                                         {beta, eta, .5 - beta - eta - alpha, alpha}};
+        constexpr double dirk_b[4] = {0, 1. / 6, 1. / 6, 2. / 3};
+        const int stages = ck_stages;
+        auto a = [&](int s, int j) { return stages == 1 ? 1.0 : dirk_a[s][j]; };
+        auto b = [&](int s) { return stages == 1 ? 1.0 : dirk_b[s]; };
+        adj_kbar.alloc(static_cast<size_t>(stages) * D, bytes);
+        adj_lambda.alloc(D, bytes);
+        adj_rhs.alloc(D, bytes);
+        adj_base.alloc(D, bytes);
+        auto kbar = [&](int s) { return adj_kbar.p + static_cast<size_t>(s) * D; };
+        AdjointSweep out{0, 0, 0};
+        copy(gradient, final_gradient, D);
+        for (int step = static_cast<int>(ck_step.size()) - 1; step >= 0; --step) {
+            const double h = ck_step[step];
+            for (int s = 0; s < stages; ++s) axpby(h * b(s), gradient, 0.0, nullptr, kbar(s), D);
+            for (int s = stages - 1; s >= 0; --s) {
+                const size_t record = static_cast<size_t>(step) * stages + s;
+                load_frozen(record);
+                tau = a(s, s) * h;
+                dg_refresh(ck_states.p + record * D);
+                bj_refresh_transposed();
+                axpby(1.0 / tau, kbar(s), 0.0, nullptr, adj_rhs.p, D);
+                const Linear lin = gmres_transposed(adj_rhs.p, adj_lambda.p, cfg.max_linear, cfg.linear_tolerance);
+                out.krylov += lin.iterations;
+                out.solves += 1;
+                out.max_relative = std::max(out.max_relative, lin.relative);
+                HIPCHECK(launch_mass_combine(n, adj_lambda.p, kbar(s), tau, adj_base.p, D, basis.p, stream));
+                axpby(1.0, gradient, 1.0, adj_base.p, gradient, D);
+                for (int j = 0; j < s; ++j)
+                    if (a(s, j) != 0) axpby(1.0, kbar(j), h * a(s, j), adj_base.p, kbar(j), D);
+            }
+        }
+        sync();
+        return out;
+    }
+    // density integral over the quadrant x > 0, y > 0 and its gradient
+    double quadrant_density(const double *u, double *gradient) {
+        adj_rhs.alloc(D, bytes);
+        double *g = gradient ? gradient : adj_rhs.p;
+        HIPCHECK(launch_quadrant_weights(n, g, basis.p, geo, stream));
+        HIPCHECK(launch_dot(g, u, D, partials.p, stream));
+        HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 1, scalars.p + 200, stream));
+        double out = 0;
+        HIPCHECK(hipMemcpyAsync(&out, scalars.p + 200, sizeof(double), hipMemcpyDeviceToHost, stream));
+        sync();
+        return out;
     }
 };
 
@@ -1346,6 +1486,80 @@ int dgh_advance(dgh_ctx *ctx, double *u, dgh_run_stats *run, dgh_update_stats *u
         if (updates)
             for (int k = 0; k < capacity && k < static_cast<int>(list.size()); ++k) updates[k] = list[k];
         if (count) *count = static_cast<int>(list.size());
+    });
+}
+int dgh_bj_refresh_transposed(dgh_ctx *ctx) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->bj_refresh_transposed();
+        ctx->sync();
+    });
+}
+int dgh_bj_apply_transposed(dgh_ctx *ctx, const double *r, double *z) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->bj_apply_transposed(r, z);
+        ctx->sync();
+    });
+}
+int dgh_dg_gmres_transposed(dgh_ctx *ctx, const double *rhs, double *x, int max_iterations, double tolerance,
+                            int *iterations, double *relative_residual, double *residual) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        const auto r = ctx->gmres_transposed(rhs, x, max_iterations, tolerance);
+        if (residual) ctx->copy(residual, ctx->kres.p, ctx->D);
+        ctx->sync();
+        if (iterations) *iterations = r.iterations;
+        if (relative_residual) *relative_residual = r.relative;
+    });
+}
+namespace {
+int run_trajectory(dgh_ctx *ctx, dgh_ctx::Trajectory mode, double *u, dgh_run_stats *run, dgh_update_stats *updates,
+                   int capacity, int *count) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        dgh_run_stats stats{};
+        std::vector<dgh_update_stats> list;
+        ctx->trajectory = mode;
+        try {
+            ctx->advance(u, stats, list);
+        } catch (...) {
+            ctx->trajectory = dgh_ctx::PLAIN;
+            throw;
+        }
+        ctx->trajectory = dgh_ctx::PLAIN;
+        if (run) *run = stats;
+        if (updates)
+            for (int k = 0; k < capacity && k < static_cast<int>(list.size()); ++k) updates[k] = list[k];
+        if (count) *count = static_cast<int>(list.size());
+    });
+}
+}  // namespace
+int dgh_trajectory_record(dgh_ctx *ctx, double *u, dgh_run_stats *run, dgh_update_stats *updates, int capacity,
+                          int *count) {
+    return run_trajectory(ctx, dgh_ctx::RECORD, u, run, updates, capacity, count);
+}
+int dgh_trajectory_replay(dgh_ctx *ctx, double *u, dgh_run_stats *run, dgh_update_stats *updates, int capacity,
+                          int *count) {
+    return run_trajectory(ctx, dgh_ctx::REPLAY, u, run, updates, capacity, count);
+}
+int dgh_adjoint_backward(dgh_ctx *ctx, const double *final_gradient, double *initial_gradient,
+                         dgh_adjoint_stats *stats) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        const auto r = ctx->adjoint_backward(final_gradient, initial_gradient);
+        if (stats) {
+            stats->krylov_iterations = r.krylov;
+            stats->linear_solves = r.solves;
+            stats->max_linear_residual = r.max_relative;
+        }
+    });
+}
+int dgh_quadrant_density(dgh_ctx *ctx, const double *u, double *value, double *gradient) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        const double v = ctx->quadrant_density(u, gradient);
+        if (value) *value = v;
     });
 }
 int dgh_kernel_time(dgh_ctx *ctx, const char *name, double *total_ms, long long *launches) {

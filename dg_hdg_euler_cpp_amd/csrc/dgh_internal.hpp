@@ -133,6 +133,17 @@ hipError_t launch_scale_div(const double *x, double d, float *out, long long cou
 // out = (x - y) / d
 hipError_t launch_sub_div(const double *x, const double *y, double d, double *out, long long count, hipStream_t s);
 
+// Transposed element blocks B^T (adjoint block Jacobi).
+hipError_t launch_assemble_blocks_transposed(int n, const double *vcache, const double *fcache, double *blocks,
+                                             double inv_step, const Basis *b, const Geo &g, hipStream_t s);
+// gradient of the density integral over the quadrant x > 0, y > 0
+hipError_t launch_quadrant_weights(int n, double *g, const Basis *b, const Geo &geo, hipStream_t s);
+// partials[REDUCE_BLOCKS] of sum a_i b_i
+hipError_t launch_dot(const double *a, const double *b, long long count, double *partials, hipStream_t s);
+// out = (M lambda - k) / tau
+hipError_t launch_mass_combine(int n, const double *lambda, const double *k, double tau, double *out, long long count,
+                               const Basis *b, hipStream_t s);
+
 // Statically condensed HDG (hybrid.cpp:31-121).
 hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache, double *blocks, double inv_step,
                                const Basis *b, const Geo &g, int first, int count, hipStream_t s);

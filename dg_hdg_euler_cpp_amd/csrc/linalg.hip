@@ -5,6 +5,8 @@
 #include <hip/hip_bf16.h>
 
 #include <cfloat>
+#include <cstdint>
+#include <type_traits>
 
 namespace dgh {
 namespace {
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish_sum(const double *__restrict__
 // With HDG = true the face terms are those of HybridOperator::assemble_cell
 // (hybrid.cpp:31-55): H_s = w_q ((+-1) A_phys + tau I) on every side with the
 // same shapes, added with sign +1 (face_jacobian.cpp:77-82).
-template <int N, typename T, bool HDG>
+template <int N, typename T, bool HDG, bool TRANSPOSE = false>
 __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ vc, const double *__restrict__ fc,
                                                     T *__restrict__ blocks, double inv_step,
                                                     const Basis *__restrict__ B, Geo g, int first) {
@@ -193,7 +195,8 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ v
                 val += (B->left[jy] * hs[(2 * N + i) * 16 + vw]) * B->left[j];
                 val += (B->right[jy] * hs[(3 * N + i) * 16 + vw]) * B->right[j];
             }
-            out[static_cast<long long>(col) * M + row] = static_cast<T>(val);
+            if (TRANSPOSE) out[static_cast<long long>(row) * M + col] = static_cast<T>(val);
+            else out[static_cast<long long>(col) * M + row] = static_cast<T>(val);
         }
     }
 }
@@ -395,17 +398,24 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
 // block_jacobi.hpp:24-39: z = U^-1 L^-1 P r per block; one wavefront per block,
 // lane l owns rows l, l+64, ...; the value of each pivot row is broadcast with
 // readlane and the factor columns stream once from HBM (column-major: one
-// coalesced load per column and row chunk). The loads of column j+1 are issued
-// before the FMAs of column j so every wave keeps two columns in flight.
+// coalesced load per column and row chunk). A ring of U column buffers keeps
+// the loads of the next U - 1 columns in flight while a column is applied;
+// the ring slot of a column is fixed at compile time (the chunk loops are
+// unrolled), so no registers move.
+#ifndef DGH_BJ_RING
+#define DGH_BJ_RING 4
+#endif
+constexpr int BJ_RING = DGH_BJ_RING;  // columns in flight per wave in k_bj_apply
 __device__ __forceinline__ float to_compute(__hip_bfloat16 v) { return __bfloat162float(v); }
 __device__ __forceinline__ float to_compute(float v) { return v; }
 __device__ __forceinline__ double to_compute(double v) { return v; }
 
 // TF: stored factor type (double, float or bfloat16); TV: vector and
 // arithmetic type (double for the FP64 path, float for the inner FP32 solve).
-template <int M, typename TF, typename TV>
+template <int M, typename TF, typename TV, int U>
 __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ factors, const int *__restrict__ rowsrc,
                                                     const TV *__restrict__ r, TV *__restrict__ z, int nblocks) {
+    static_assert(64 % U == 0, "ring depth must divide the chunk width");
     using T = TF;
     constexpr int NA = (M + 63) / 64;
     const int lane = threadIdx.x % 64;
@@ -418,64 +428,74 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
         const int row = a * 64 + lane;
         x[a] = row < M ? r[blk * M + rowsrc[blk * M + row]] : TV(0);
     }
-    // forward substitution with the unit lower L: x_i -= L_ij x_j for i > j
-    T cur[NA], nxt[NA];
-#pragma unroll
-    for (int a = 0; a < NA; ++a) {
-        const int row = a * 64 + lane;
-        nxt[a] = (row > 0 && row < M) ? F[row] : T(0);
-    }
-#pragma unroll
-    for (int ja = 0; ja < NA; ++ja) {
-        const int jend = min(64, M - ja * 64);
-        for (int jl = 0; jl < jend; ++jl) {
-            const int j = ja * 64 + jl;
-#pragma unroll
-            for (int a = ja; a < NA; ++a) cur[a] = nxt[a];
-            if (j + 1 < M) {
-                const T *colp = F + static_cast<long long>(j + 1) * M;
-#pragma unroll
-                for (int a = ja; a < NA; ++a) {
-                    const int row = a * 64 + lane;
-                    nxt[a] = (row > j + 1 && row < M) ? colp[row] : T(0);
-                }
-            }
-            const TV xj = readlane_d(x[ja], jl);
-#pragma unroll
-            for (int a = ja; a < NA; ++a) x[a] -= static_cast<TV>(to_compute(cur[a])) * xj;
-        }
-    }
-    // backward substitution with U: x_j /= U_jj, x_i -= U_ij x_j for i < j
-    {
-        const T *colp = F + static_cast<long long>(M - 1) * M;
+    T buf[U][NA];
+    // strictly lower part of column c, chunks a >= a0
+    auto load_lower = [&](T(&b)[NA], int c, int a0) {
+        const T *colp = F + static_cast<long long>(c) * M;
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int row = a * 64 + lane;
-            nxt[a] = row < M ? colp[row] : T(0);
+            if (a >= a0) b[a] = (row > c && row < M) ? colp[row] : T(0);
+        }
+    };
+    // upper part (diagonal included) of column c, chunks a <= a1
+    auto load_upper = [&](T(&b)[NA], int c, int a1) {
+        const T *colp = F + static_cast<long long>(c) * M;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            const int row = a * 64 + lane;
+            if (a <= a1) b[a] = row <= c ? colp[row] : T(0);
+        }
+    };
+    // forward substitution with the unit lower L: x_i -= L_ij x_j for i > j;
+    // column j sits in ring slot j % U
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (u < M) load_lower(buf[u], u, 0);
+#pragma unroll
+    for (int ja = 0; ja < NA; ++ja) {
+        const int jend = min(64, M - ja * 64);
+#pragma clang loop unroll(disable)
+        for (int jl = 0; jl < jend; jl += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (jl + u < jend) {
+                    const int j = ja * 64 + jl + u;
+                    const TV xj = readlane_d(x[ja], jl + u);
+#pragma unroll
+                    for (int a = ja; a < NA; ++a) x[a] -= static_cast<TV>(to_compute(buf[u][a])) * xj;
+                    if (j + U < M) load_lower(buf[u], j + U, ja);
+                }
+            }
         }
     }
+    // backward substitution with U: x_j /= U_jj, x_i -= U_ij x_j for i < j;
+    // column j sits in ring slot (M - 1 - j) % U
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (M - 1 - u >= 0) load_upper(buf[u], M - 1 - u, NA - 1);
 #pragma unroll
     for (int ja = NA - 1; ja >= 0; --ja) {
         const int jend = min(64, M - ja * 64);
-        for (int jl = jend - 1; jl >= 0; --jl) {
-            const int j = ja * 64 + jl;
+        const int t0 = M - ja * 64 - jend;  // ring position of the chunk's last column
+#pragma clang loop unroll(disable)
+        for (int g = 0; g < jend; g += U) {
 #pragma unroll
-            for (int a = 0; a <= ja; ++a) cur[a] = nxt[a];
-            if (j > 0) {
-                const T *colp = F + static_cast<long long>(j - 1) * M;
+            for (int u = 0; u < U; ++u) {
+                const int jl = jend - 1 - g - u;
+                if (jl >= 0) {
+                    const int j = ja * 64 + jl;
+                    T(&cur)[NA] = buf[(t0 + u) % U];
+                    const TV diag = readlane_d(static_cast<TV>(to_compute(cur[ja])), jl);
+                    const TV xj = readlane_d(x[ja], jl) / diag;
+                    if (lane == jl) x[ja] = xj;
 #pragma unroll
-                for (int a = 0; a <= ja; ++a) {
-                    const int row = a * 64 + lane;
-                    nxt[a] = row <= j - 1 ? colp[row] : T(0);
+                    for (int a = 0; a <= ja; ++a) {
+                        const int row = a * 64 + lane;
+                        if (row < j) x[a] -= static_cast<TV>(to_compute(cur[a])) * xj;
+                    }
+                    if (j - U >= 0) load_upper(cur, j - U, ja);
                 }
-            }
-            const TV diag = readlane_d(static_cast<TV>(to_compute(cur[ja])), jl);
-            const TV xj = readlane_d(x[ja], jl) / diag;
-            if (lane == jl) x[ja] = xj;
-#pragma unroll
-            for (int a = 0; a <= ja; ++a) {
-                const int row = a * 64 + lane;
-                if (row < j) x[a] -= static_cast<TV>(to_compute(cur[a])) * xj;
             }
         }
     }
@@ -504,80 +524,50 @@ constexpr int MAXV = 32;  // restart + 1 <= 32 basis vectors per launch
 template <typename T> struct PtrsT { const T *p[MAXV]; };
 using Ptrs = PtrsT<double>;
 
-// partials[b][k] = sum_i V_k[i] * (W_i * w_i); W = inv_mass(node) (metric
-// weights == nullptr) or weights[i] (gmres.cpp:39-43 projection).
-template <int N, typename TV>
-__global__ __launch_bounds__(BLOCK) void k_multi_dot(PtrsT<TV> V, int nvec, const TV *__restrict__ w,
-                                                     const double *__restrict__ weights, long long count,
-                                                     double *partials, const Basis *__restrict__ B) {
-    constexpr int P = N * N;
-    __shared__ double smem[BLOCK / 64][MAXV];
-    double acc[MAXV];
-#pragma unroll
-    for (int k = 0; k < MAXV; ++k) acc[k] = 0;
-    for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
-         i += static_cast<long long>(gridDim.x) * BLOCK) {
-        const double ww = (weights ? weights[i] : B->inv_mass[i % P]) * static_cast<double>(w[i]);
-#pragma unroll
-        for (int k = 0; k < MAXV; ++k)
-            if (k < nvec) acc[k] += static_cast<double>(V.p[k][i]) * ww;
+// ---- vectorised Arnoldi passes with a compile-time basis count NV
+// Each thread takes W consecutive entries per step with one 16-byte load per
+// vector (W = 2 doubles or 4 floats); the basis values stay in registers
+// between the update of w and the next projection; the DG metric's inverse
+// mass comes from an LDS copy indexed by the node (entry mod P).
+template <typename TV> struct Pack;
+template <> struct Pack<double> {
+    using T = double2;
+    static constexpr int W = 2;
+    __device__ static void get(const T &a, double *o) { o[0] = a.x; o[1] = a.y; }
+    __device__ static double at(const T &a, int e) { return e == 0 ? a.x : a.y; }
+    __device__ static T put(const double *o) { return make_double2(o[0], o[1]); }
+};
+template <> struct Pack<float> {
+    using T = float4;
+    static constexpr int W = 4;
+    __device__ static void get(const T &a, double *o) { o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w; }
+    __device__ static double at(const T &a, int e) { return e == 0 ? a.x : e == 1 ? a.y : e == 2 ? a.z : a.w; }
+    __device__ static T put(const double *o) {
+        return make_float4(static_cast<float>(o[0]), static_cast<float>(o[1]), static_cast<float>(o[2]),
+                           static_cast<float>(o[3]));
     }
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-#pragma unroll
-    for (int k = 0; k < MAXV; ++k) {
-        if (k < nvec) {
-            const double t = wave_sum(acc[k]);
-            if (lane == 0) smem[wave][k] = t;
+};
+
+template <typename TV>
+__device__ __forceinline__ void metric(const double *__restrict__ weights, const double *im, int P, int i, int w,
+                                       double *out) {
+    if (weights) {
+        for (int e = 0; e < w; ++e) out[e] = weights[i + e];
+    } else {
+        int r = i % P;
+        for (int e = 0; e < w; ++e) {
+            out[e] = im[r];
+            r = r + 1 == P ? 0 : r + 1;
         }
-    }
-    __syncthreads();
-    if (threadIdx.x < nvec) {
-        double t = 0;
-        for (int wv = 0; wv < BLOCK / 64; ++wv) t += smem[wv][threadIdx.x];
-        partials[static_cast<long long>(blockIdx.x) * nvec + threadIdx.x] = t;
     }
 }
 
-// w -= sum_k V_k h_k (gmres.cpp:44-45), then mode 0: partial projections of
-// the updated w (next CGS pass), mode 1: partial weighted sum of w^2, mode 2: none.
-template <int N, typename TV>
-__global__ __launch_bounds__(BLOCK) void k_update_dot(PtrsT<TV> V, int nvec, const double *__restrict__ h,
-                                                      TV *__restrict__ w, const double *__restrict__ weights,
-                                                      long long count, double *partials, int mode,
-                                                      const Basis *__restrict__ B) {
-    constexpr int P = N * N;
-    __shared__ double smem[BLOCK / 64][MAXV];
-    double hk[MAXV], acc[MAXV];
-#pragma unroll
-    for (int k = 0; k < MAXV; ++k) {
-        hk[k] = k < nvec ? h[k] : 0.0;
-        acc[k] = 0;
-    }
-    for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
-         i += static_cast<long long>(gridDim.x) * BLOCK) {
-        double s = 0;
-#pragma unroll
-        for (int k = 0; k < MAXV; ++k)
-            if (k < nvec) s += static_cast<double>(V.p[k][i]) * hk[k];
-        const TV wt = static_cast<TV>(static_cast<double>(w[i]) - s);
-        w[i] = wt;
-        const double wi = static_cast<double>(wt);
-        if (mode != 2) {
-            const double ww = (weights ? weights[i] : B->inv_mass[i % P]) * wi;
-            if (mode == 0) {
-#pragma unroll
-                for (int k = 0; k < MAXV; ++k)
-                    if (k < nvec) acc[k] += static_cast<double>(V.p[k][i]) * ww;
-            } else {
-                acc[0] += wi * ww;
-            }
-        }
-    }
-    if (mode == 2) return;
-    const int width = mode == 0 ? nvec : 1;
+template <int NV>
+__device__ __forceinline__ void store_partials(double (&acc)[NV], int width, double *partials) {
+    __shared__ double smem[BLOCK / 64][NV];
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
 #pragma unroll
-    for (int k = 0; k < MAXV; ++k) {
+    for (int k = 0; k < NV; ++k) {
         if (k < width) {
             const double t = wave_sum(acc[k]);
             if (lane == 0) smem[wave][k] = t;
@@ -589,6 +579,123 @@ __global__ __launch_bounds__(BLOCK) void k_update_dot(PtrsT<TV> V, int nvec, con
         for (int wv = 0; wv < BLOCK / 64; ++wv) t += smem[wv][threadIdx.x];
         partials[static_cast<long long>(blockIdx.x) * width + threadIdx.x] = t;
     }
+}
+
+// partials[b][k] = sum_i V_k[i] * (W_i * w_i), W = inverse mass (weights ==
+// nullptr) or weights[i]: the projections of gmres.cpp:39-43
+template <typename TV, int NV>
+__global__ __launch_bounds__(BLOCK) void k_mdot(PtrsT<TV> V, const TV *__restrict__ w,
+                                                const double *__restrict__ weights, int count, int P,
+                                                double *partials, const Basis *__restrict__ B) {
+    using PK = Pack<TV>;
+    constexpr int W = PK::W;
+    __shared__ double im[MAXN * MAXN];
+    if (!weights)
+        for (int t = threadIdx.x; t < P; t += BLOCK) im[t] = B->inv_mass[t];
+    __syncthreads();
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0;
+    const int groups = count / W;
+    for (int gi = blockIdx.x * BLOCK + threadIdx.x; gi < groups; gi += gridDim.x * BLOCK) {
+        const int i = gi * W;
+        double wv[W], mw[W];
+        PK::get(*reinterpret_cast<const typename PK::T *>(w + i), wv);
+        metric<TV>(weights, im, P, i, W, mw);
+#pragma unroll
+        for (int e = 0; e < W; ++e) wv[e] *= mw[e];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double vk[W];
+            PK::get(*reinterpret_cast<const typename PK::T *>(V.p[k] + i), vk);
+#pragma unroll
+            for (int e = 0; e < W; ++e) acc[k] += vk[e] * wv[e];
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < count - groups * W) {  // tail entries
+        const int i = groups * W + threadIdx.x;
+        double mw;
+        metric<TV>(weights, im, P, i, 1, &mw);
+        const double ww = mw * static_cast<double>(w[i]);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += static_cast<double>(V.p[k][i]) * ww;
+    }
+    store_partials<NV>(acc, NV, partials);
+}
+
+// w -= sum_k V_k h_k (gmres.cpp:44-45); then mode 0: projections of the
+// updated w on V (the second CGS pass), mode 1: its weighted square norm,
+// mode 2: nothing
+template <typename TV, int NV>
+__global__ __launch_bounds__(BLOCK) void k_mupdate(PtrsT<TV> V, const double *__restrict__ h, TV *__restrict__ w,
+                                                   const double *__restrict__ weights, int count, int P,
+                                                   double *partials, int mode, const Basis *__restrict__ B) {
+    using PK = Pack<TV>;
+    constexpr int W = PK::W;
+    __shared__ double im[MAXN * MAXN];
+    if (!weights)
+        for (int t = threadIdx.x; t < P; t += BLOCK) im[t] = B->inv_mass[t];
+    __syncthreads();
+    double hk[NV], acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        hk[k] = h[k];
+        acc[k] = 0;
+    }
+    const int groups = count / W;
+    for (int gi = blockIdx.x * BLOCK + threadIdx.x; gi < groups; gi += gridDim.x * BLOCK) {
+        const int i = gi * W;
+        typename PK::T vk[NV];  // kept in the stored type: 16 bytes per vector
+        double wv[W];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) vk[k] = *reinterpret_cast<const typename PK::T *>(V.p[k] + i);
+        PK::get(*reinterpret_cast<const typename PK::T *>(w + i), wv);
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+            double s = 0;
+#pragma unroll
+            for (int k = 0; k < NV; ++k) s += PK::at(vk[k], e) * hk[k];
+            wv[e] = static_cast<double>(static_cast<TV>(wv[e] - s));
+        }
+        *reinterpret_cast<typename PK::T *>(w + i) = PK::put(wv);
+        if (mode != 2) {
+            double mw[W];
+            metric<TV>(weights, im, P, i, W, mw);
+            if (mode == 0) {
+#pragma unroll
+                for (int k = 0; k < NV; ++k)
+#pragma unroll
+                    for (int e = 0; e < W; ++e) acc[k] += PK::at(vk[k], e) * (mw[e] * wv[e]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < W; ++e) acc[0] += wv[e] * (mw[e] * wv[e]);
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < count - groups * W) {  // tail entries
+        const int i = groups * W + threadIdx.x;
+        double s = 0, vk[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            vk[k] = static_cast<double>(V.p[k][i]);
+            s += vk[k] * hk[k];
+        }
+        const TV wt = static_cast<TV>(static_cast<double>(w[i]) - s);
+        w[i] = wt;
+        const double wi = static_cast<double>(wt);
+        if (mode != 2) {
+            double mw;
+            metric<TV>(weights, im, P, i, 1, &mw);
+            if (mode == 0) {
+#pragma unroll
+                for (int k = 0; k < NV; ++k) acc[k] += vk[k] * (mw * wi);
+            } else {
+                acc[0] += wi * (mw * wi);
+            }
+        }
+    }
+    if (mode == 2) return;
+    store_partials<NV>(acc, mode == 0 ? NV : 1, partials);
 }
 
 // out = sum_k V_k y_k (gmres.cpp:75)
@@ -620,6 +727,45 @@ __global__ __launch_bounds__(BLOCK) void k_sub_div(const double *__restrict__ x,
     for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
          i += static_cast<long long>(gridDim.x) * BLOCK)
         out[i] = (x[i] - y[i]) / d;
+}
+
+// partials[b] = sum a_i b_i
+__global__ __launch_bounds__(BLOCK) void k_dot(const double *__restrict__ a, const double *__restrict__ b,
+                                               long long count, double *partials) {
+    __shared__ double smem[BLOCK / 64];
+    double acc = 0;
+    for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
+         i += static_cast<long long>(gridDim.x) * BLOCK)
+        acc += a[i] * b[i];
+    const double t = block_sum(acc, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+// out = (M lambda - k) / tau with the DG mass M(node)
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_mass_combine(const double *__restrict__ lambda,
+                                                        const double *__restrict__ k, double tau,
+                                                        double *__restrict__ out, long long count,
+                                                        const Basis *__restrict__ B) {
+    constexpr int P = N * N;
+    for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
+         i += static_cast<long long>(gridDim.x) * BLOCK)
+        out[i] = (B->mass[i % P] * lambda[i] - k[i]) / tau;
+}
+
+// g = d/du of the density integral over the cells whose centre lies in x > 0,
+// y > 0 (the exact quadrant for even nx, ny): the node mass on density nodes
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_quadrant_weights(double *__restrict__ g, const Basis *__restrict__ B,
+                                                            Geo geo) {
+    constexpr int P = N * N, M = 4 * P;
+    const long long count = static_cast<long long>(geo.cells) * M;
+    for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
+         i += static_cast<long long>(gridDim.x) * BLOCK) {
+        const int cell = static_cast<int>(i / M), local = static_cast<int>(i % M);
+        const int cx = cell % geo.nx, cy = cell / geo.nx;
+        const bool inside = 2 * cx + 1 > geo.nx && 2 * cy + 1 > geo.ny;
+        g[i] = (inside && local < P) ? B->mass[local] : 0.0;
+    }
 }
 
 template <typename T>
@@ -696,6 +842,12 @@ hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fca
         k_assemble<N, float, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
     })
 }
+hipError_t launch_assemble_blocks_transposed(int n, const double *vcache, const double *fcache, double *blocks,
+                                             double inv_step, const Basis *b, const Geo &g, hipStream_t s) {
+    DGH_DISPATCH_N(n, {
+        k_assemble<N, double, false, true><<<g.cells, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, 0);
+    })
+}
 hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache, double *blocks, double inv_step,
                                const Basis *b, const Geo &g, int first, int count, hipStream_t s) {
     DGH_DISPATCH_N(n, {
@@ -711,20 +863,20 @@ hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, 
 hipError_t launch_bj_apply(int m, const double *factors, const int *rowsrc, const double *r, double *z,
                            int nblocks, hipStream_t s) {
     DGH_DISPATCH_M(m, {
-        k_bj_apply<M, double, double><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks);
+        k_bj_apply<M, double, double, BJ_RING><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks);
     })
 }
 hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const float *r, float *z, int nblocks,
                            hipStream_t s) {
     DGH_DISPATCH_M(m, {
-        k_bj_apply<M, float, float><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks);
+        k_bj_apply<M, float, float, BJ_RING><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks);
     })
 }
 hipError_t launch_bj_apply_bf16(int m, const void *factors, const int *rowsrc, const float *r, float *z,
                                 int nblocks, hipStream_t s) {
     const auto *f = static_cast<const __hip_bfloat16 *>(factors);
     DGH_DISPATCH_M(m, {
-        k_bj_apply<M, __hip_bfloat16, float><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, rowsrc, r, z, nblocks);
+        k_bj_apply<M, __hip_bfloat16, float, BJ_RING><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, rowsrc, r, z, nblocks);
     })
 }
 hipError_t launch_convert(const double *src, float *dst, long long count, hipStream_t s) {
@@ -740,6 +892,28 @@ hipError_t launch_to_bf16(const float *src, void *dst, long long count, hipStrea
     return hipGetLastError();
 }
 
+// calls f(std::integral_constant<int, nv>) for nv in [NV, MAXV]
+template <int NV, class F> static hipError_t nv_dispatch(int nv, F &&f) {
+    if constexpr (NV > MAXV) {
+        (void)nv;
+        (void)f;
+        return hipErrorInvalidValue;
+    } else {
+        if (nv == NV) {
+            f(std::integral_constant<int, NV>{});
+            return hipGetLastError();
+        }
+        return nv_dispatch<NV + 1>(nv, f);
+    }
+}
+
+template <typename T> static bool aligned16(const T *p) { return (reinterpret_cast<std::uintptr_t>(p) & 15) == 0; }
+template <typename T> static bool basis_aligned(const T *const *basis, int nvec, const T *w) {
+    for (int k = 0; k < nvec; ++k)
+        if (!aligned16(basis[k])) return false;
+    return aligned16(w);
+}
+
 template <typename T> static PtrsT<T> make_ptrs(const T *const *basis, int nvec) {
     PtrsT<T> p{};
     for (int k = 0; k < nvec && k < MAXV; ++k) p.p[k] = basis[k];
@@ -750,8 +924,13 @@ template <typename TV>
 static hipError_t multi_dot_any(int n, const TV *const *basis, int nvec, const TV *w, const double *weights,
                                 long long count, double *partials, const Basis *b, hipStream_t s) {
     if (nvec > MAXV) return hipErrorInvalidValue;
+    if (nvec < 1 || count >= (1LL << 31) || n < 1 || n > MAXN || !basis_aligned(basis, nvec, w))
+        return hipErrorInvalidValue;
     const auto p = make_ptrs(basis, nvec);
-    DGH_DISPATCH_N(n, { k_multi_dot<N, TV><<<REDUCE_BLOCKS, BLOCK, 0, s>>>(p, nvec, w, weights, count, partials, b); })
+    return nv_dispatch<1>(nvec, [&](auto c) {
+        constexpr int NV = decltype(c)::value;
+        k_mdot<TV, NV><<<REDUCE_BLOCKS, BLOCK, 0, s>>>(p, w, weights, static_cast<int>(count), n * n, partials, b);
+    });
 }
 hipError_t launch_multi_dot(int n, const double *const *basis, int nvec, const double *w, const double *weights,
                             long long count, double *partials, const Basis *b, hipStream_t s) {
@@ -766,10 +945,14 @@ static hipError_t update_dot_any(int n, const TV *const *basis, int nvec, const 
                                  const double *weights, long long count, double *partials, int mode,
                                  const Basis *b, hipStream_t s) {
     if (nvec > MAXV) return hipErrorInvalidValue;
+    if (nvec < 1 || count >= (1LL << 31) || n < 1 || n > MAXN || !basis_aligned(basis, nvec, w))
+        return hipErrorInvalidValue;
     const auto p = make_ptrs(basis, nvec);
-    DGH_DISPATCH_N(n, {
-        k_update_dot<N, TV><<<REDUCE_BLOCKS, BLOCK, 0, s>>>(p, nvec, h, w, weights, count, partials, mode, b);
-    })
+    return nv_dispatch<1>(nvec, [&](auto c) {
+        constexpr int NV = decltype(c)::value;
+        k_mupdate<TV, NV><<<REDUCE_BLOCKS, BLOCK, 0, s>>>(p, h, w, weights, static_cast<int>(count), n * n, partials,
+                                                         mode, b);
+    });
 }
 hipError_t launch_update_dot(int n, const double *const *basis, int nvec, const double *h, double *w,
                              const double *weights, long long count, double *partials, int mode, const Basis *b,
@@ -820,6 +1003,17 @@ hipError_t launch_scale_div(const float *x, double d, float *out, long long coun
 hipError_t launch_scale_div(const double *x, double d, float *out, long long count, hipStream_t s) {
     k_scale_div<double, float><<<4096, BLOCK, 0, s>>>(x, d, out, count);
     return hipGetLastError();
+}
+hipError_t launch_quadrant_weights(int n, double *g, const Basis *b, const Geo &geo, hipStream_t s) {
+    DGH_DISPATCH_N(n, { k_quadrant_weights<N><<<4096, BLOCK, 0, s>>>(g, b, geo); })
+}
+hipError_t launch_dot(const double *a, const double *b, long long count, double *partials, hipStream_t s) {
+    k_dot<<<REDUCE_BLOCKS, BLOCK, 0, s>>>(a, b, count, partials);
+    return hipGetLastError();
+}
+hipError_t launch_mass_combine(int n, const double *lambda, const double *k, double tau, double *out, long long count,
+                               const Basis *b, hipStream_t s) {
+    DGH_DISPATCH_N(n, { k_mass_combine<N><<<4096, BLOCK, 0, s>>>(lambda, k, tau, out, count, b); })
 }
 hipError_t launch_sub_div(const double *x, const double *y, double d, double *out, long long count, hipStream_t s) {
     k_sub_div<<<4096, BLOCK, 0, s>>>(x, y, d, out, count);

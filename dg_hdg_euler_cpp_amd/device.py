@@ -37,6 +37,10 @@ class RunStats(C.Structure):
                 ("krylov_iterations", C.c_int), ("measured_updates", C.c_int)]
 
 
+class AdjointStats(C.Structure):
+    _fields_ = [("krylov_iterations", C.c_int), ("linear_solves", C.c_int), ("max_linear_residual", C.c_double)]
+
+
 def load_library():
     """Load libdgh_device.so (raises if it has not been built)."""
     global _lib
@@ -306,15 +310,56 @@ class DeviceContext:
                    C.byref(its), C.byref(rel), _ptr(res))
         return x, its.value, rel.value, res
 
-    def advance(self, u, capacity=100000):
+    def _run(self, entry, u, capacity):
         run = RunStats()
         upd = (UpdateStats * capacity)()
         count = C.c_int()
-        self._call("dgh_advance", _ptr(u), C.byref(run), upd, C.c_int(capacity), C.byref(count))
+        self._call(entry, _ptr(u), C.byref(run), upd, C.c_int(capacity), C.byref(count))
         stats = {f: getattr(run, f) for f, _ in RunStats._fields_}
         stats["updates"] = [{f: getattr(upd[i], f) for f, _ in UpdateStats._fields_}
                             for i in range(min(count.value, capacity))]
         return stats
+
+    def advance(self, u, capacity=100000):
+        return self._run("dgh_advance", u, capacity)
+
+    # ---- discrete adjoint
+    def bj_refresh_transposed(self):
+        self._call("dgh_bj_refresh_transposed")
+
+    def bj_apply_transposed(self, r, out=None):
+        z = self.zeros() if out is None else out
+        self._call("dgh_bj_apply_transposed", _ptr(r), _ptr(z))
+        return z
+
+    def dg_gmres_transposed(self, rhs, max_iterations, tolerance):
+        x = self.zeros()
+        res = self.zeros()
+        its = C.c_int()
+        rel = C.c_double()
+        self._call("dgh_dg_gmres_transposed", _ptr(rhs), _ptr(x), C.c_int(max_iterations), C.c_double(tolerance),
+                   C.byref(its), C.byref(rel), _ptr(res))
+        return x, its.value, rel.value, res
+
+    def trajectory_record(self, u, capacity=100000):
+        """advance() that checkpoints every stage for adjoint_backward."""
+        return self._run("dgh_trajectory_record", u, capacity)
+
+    def trajectory_replay(self, u, capacity=100000):
+        """advance() on the recorded step sizes and frozen speeds."""
+        return self._run("dgh_trajectory_replay", u, capacity)
+
+    def adjoint_backward(self, final_gradient):
+        grad = self.zeros()
+        st = AdjointStats()
+        self._call("dgh_adjoint_backward", _ptr(final_gradient), _ptr(grad), C.byref(st))
+        return grad, {f: getattr(st, f) for f, _ in AdjointStats._fields_}
+
+    def quadrant_density(self, u, with_gradient=True):
+        value = C.c_double()
+        g = self.zeros() if with_gradient else None
+        self._call("dgh_quadrant_density", _ptr(u), C.byref(value), _ptr(g) if g is not None else None)
+        return (value.value, g) if with_gradient else value.value
 
     # ---- timing hooks
     def set_timing(self, enabled):

@@ -90,6 +90,13 @@ typedef struct dgh_run_stats {
     int steps, newton_iterations, krylov_iterations, measured_updates;
 } dgh_run_stats;
 
+/* Reverse sweep of the discrete adjoint: transposed linear solves, one per
+ * stage, their GMRES iterations and the largest final relative residual. */
+typedef struct dgh_adjoint_stats {
+    int krylov_iterations, linear_solves;
+    double max_linear_residual;
+} dgh_adjoint_stats;
+
 /* ---- configuration (config.cpp:6-122) ---------------------------------- */
 size_t dgh_config_size(void);
 int dgh_config_default(dgh_config *out);
@@ -207,6 +214,35 @@ int dgh_hdg_gmres(dgh_ctx *ctx, const double *rhs, double *x, int max_iterations
  * configured method. u is advanced in place. updates may be NULL. */
 int dgh_advance(dgh_ctx *ctx, double *u, dgh_run_stats *run, dgh_update_stats *updates, int capacity,
                 int *count);
+
+/* ---- discrete adjoint (no reference counterpart) -------------------------- */
+/* The reference only linearises forward (Jv); these entry points are the
+ * reverse of the same operators, built on dgh_dg_apply_transpose.
+ * Transposed block Jacobi: LU of every transposed diagonal block B_i^T, so the
+ * apply returns B_i^-T r (shares the factor storage with dgh_bj_refresh). */
+int dgh_bj_refresh_transposed(dgh_ctx *ctx);
+int dgh_bj_apply_transposed(dgh_ctx *ctx, const double *r, double *z);
+/* GMRES on J^T x = rhs with the transposed block Jacobi, metric M^-1. */
+int dgh_dg_gmres_transposed(dgh_ctx *ctx, const double *rhs, double *x, int max_iterations, double tolerance,
+                            int *iterations, double *relative_residual, double *residual);
+/* dgh_advance (BE or DIRK3, --steps N) that also checkpoints every converged
+ * stage state and the wave speeds that stage froze, with each step size. */
+int dgh_trajectory_record(dgh_ctx *ctx, double *u, dgh_run_stats *run, dgh_update_stats *updates, int capacity,
+                          int *count);
+/* dgh_advance from a new initial state on the recorded schedule: the recorded
+ * step sizes and frozen speeds are reused instead of freezing again. This is
+ * the map whose gradient dgh_adjoint_backward returns. */
+int dgh_trajectory_replay(dgh_ctx *ctx, double *u, dgh_run_stats *run, dgh_update_stats *updates, int capacity,
+                          int *count);
+/* Reverse sweep over the recorded trajectory: given final_gradient = dJ/du_N,
+ * writes initial_gradient = dJ/du_0; every stage's Newton operator is rebuilt
+ * from its checkpoint and solved transposed. */
+int dgh_adjoint_backward(dgh_ctx *ctx, const double *final_gradient, double *initial_gradient,
+                         dgh_adjoint_stats *stats);
+/* Output functional: integral of density over the cells whose centre lies in
+ * x > 0, y > 0 (the exact quadrant for even nx, ny), and its gradient (may be
+ * NULL). */
+int dgh_quadrant_density(dgh_ctx *ctx, const double *u, double *value, double *gradient);
 
 /* ---- timing hooks --------------------------------------------------------- */
 /* Accumulated device time (HIP events on the context stream) and launch count
