@@ -58,6 +58,20 @@ def run_hdg(args, nx, k, init):
             "rms": [x["residual_physical_rms"] for x in upd]}
 
 
+def run_precision_check(args, nx, k, init, precision):
+    """Untimed: every update of the mixed form also solves its Newton system
+    through the FP64 path and reports ||du_mixed - du_fp64||_M / ||du_fp64||_M."""
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    with DeviceContext(solver_args(nx, k, init, precision, 2, 0) + ["--check-update"]) as dev:
+        u = dev.initial()
+        stats = dev.advance(u)
+    out = [{"update": x["iteration"], "rms": x["residual_physical_rms"], "precision_error": x["precision_error"]}
+           for x in stats["updates"]]
+    log(f"[bench] {precision} vs fp64 update error: " + ", ".join(f"{o['precision_error']:.3e}" for o in out))
+    return out
+
+
 def run_device(args, nx, k, init, precision):
     import torch
     from dg_hdg_euler_cpp_amd import DeviceContext
@@ -212,6 +226,7 @@ def main():
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed", "mixed-bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-hdg", action="store_true")
+    ap.add_argument("--no-check", action="store_true", help="skip the untimed mixed-vs-FP64 update comparison")
     ap.add_argument("--only-hdg", action="store_true", help="run and report the condensed HDG form alone")
     ap.add_argument("--probe", action="store_true", help="run each hot kernel 3x for counter profiles")
     args = ap.parse_args()
@@ -268,6 +283,7 @@ def main():
                  "max_rel_diff_first_two_updates": max(abs(x - y) / abs(x) for x, y in zip(a[:2], b[:2])),
                  "speedup_mixed_over_fp64": forms_out["mixed"]["updates_per_s"] / forms_out["fp64"]["updates_per_s"]}
     hdg = None if args.no_hdg or args.precision != "both" else run_hdg(args, args.nx, args.k, args.init)
+    check = None if args.no_check or head == "fp64" else run_precision_check(args, args.nx, args.k, args.init, head)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.nx, args.k, args.init, forms_out.get("fp64", forms_out[head])["krylov_per_update"])
@@ -293,6 +309,7 @@ def main():
                      "algorithmic_bytes": bytes_bj, "avg_launch_s": dev["bj_avg_s"]},
         "forms": forms_out,
         "equal_residual": equal,
+        "update_error_vs_fp64": check,
         "hdg_fp64": hdg,
         "jv": {"dof_per_s": g["size"] / dev["jv_avg_s"], "achieved_GBs": jv_bytes(g) / dev["jv_avg_s"] / 1e9,
                "avg_launch_s": dev["jv_avg_s"]},

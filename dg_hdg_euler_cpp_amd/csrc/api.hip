@@ -978,12 +978,21 @@ struct dgh_ctx {
                     }
                     axpby(-1.0, kres.p, 0.0, nullptr, equation_error.p, D);
                     if (c.precision != DGH_PRECISION_FP64 && c.check_update) {
-                        // the same Newton system through the FP64 path
+                        // The same Newton system through the FP64 path. The two
+                        // factor pools are never resident together (at 512^2, k=8
+                        // they are 221 GB + 111 GB): the FP32/BF16 factors are
+                        // released first and the FP64 ones after the check.
+                        sync();
+                        factors_f32.release(bytes);
+                        factors_bf16.release(bytes);
+                        bj32_ready = false;
                         check_dir.alloc(D, bytes);
                         bj_refresh();
                         gmres(rhs.p, check_dir.p, c.max_linear, c.linear_tolerance);
                         axpby(1.0, direction.p, -1.0, check_dir.p, kwork.p, D);
                         precision_error = state_norm(kwork.p) / state_norm(check_dir.p);
+                        factors.release(bytes);
+                        bj_ready = false;
                     }
                     }
                     const double error_norm = dual_norm(equation_error.p);

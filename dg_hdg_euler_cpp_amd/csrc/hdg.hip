@@ -67,7 +67,17 @@ __device__ void hdg_matrix(const double *__restrict__ fc, long long nodes, int f
 // by all 16 waves (L1 hits) and applied to all of the pass's columns.
 // B columns (hybrid.cpp:41-42): column (s, v, q) is -w tau l_s(i) at rows
 // (v, face_node(s, q, i)); column t is the element's DG residual.
-constexpr int SOLVE_WAVES = 16, SOLVE_COLS = 4;  // 64 columns per pass
+#ifndef DGH_SOLVE_WAVES
+#define DGH_SOLVE_WAVES 8
+#endif
+#ifndef DGH_SOLVE_COLS
+#define DGH_SOLVE_COLS 8
+#endif
+#ifndef DGH_SOLVE_RING
+#define DGH_SOLVE_RING 2
+#endif
+constexpr int SOLVE_WAVES = DGH_SOLVE_WAVES, SOLVE_COLS = DGH_SOLVE_COLS;  // columns per pass: waves x cols
+constexpr int SOLVE_RING = DGH_SOLVE_RING;  // factor columns in flight per wave
 
 template <int N>
 __global__ __launch_bounds__(SOLVE_WAVES * 64) void k_hdg_solve(const double *__restrict__ factors,
@@ -117,49 +127,78 @@ __global__ __launch_bounds__(SOLVE_WAVES * 64) void k_hdg_solve(const double *__
             x[cc][a] = val;
         }
     }
-    // forward substitution with the unit lower L
+    // The factor columns stream through a ring of SOLVE_RING buffers so the
+    // loads of the next columns are in flight while a column is applied; they
+    // are read once per pass, hence non-temporal.
+    double buf[SOLVE_RING][NA];
+    auto load_lower = [&](double(&b)[NA], int c, int a0) {  // rows > c, chunks >= a0
+        const double *colp = F + static_cast<long long>(c) * M;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            const int row = a * 64 + lane;
+            if (a >= a0) b[a] = (row > c && row < M) ? __builtin_nontemporal_load(colp + row) : 0.0;
+        }
+    };
+    auto load_upper = [&](double(&b)[NA], int c, int a1) {  // rows <= c, chunks <= a1
+        const double *colp = F + static_cast<long long>(c) * M;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            const int row = a * 64 + lane;
+            if (a <= a1) b[a] = row <= c ? __builtin_nontemporal_load(colp + row) : 0.0;
+        }
+    };
+    // forward substitution with the unit lower L (column j in ring slot j % SOLVE_RING)
+#pragma unroll
+    for (int u = 0; u < SOLVE_RING; ++u)
+        if (u < M) load_lower(buf[u], u, 0);
 #pragma unroll
     for (int ja = 0; ja < NA; ++ja) {
         const int jend = min(64, M - ja * 64);
-        for (int jl = 0; jl < jend; ++jl) {
-            const int j = ja * 64 + jl;
-            const double *colp = F + static_cast<long long>(j) * M;
-            double l[NA];
+#pragma clang loop unroll(disable)
+        for (int jl = 0; jl < jend; jl += SOLVE_RING) {
 #pragma unroll
-            for (int a = ja; a < NA; ++a) {
-                const int row = a * 64 + lane;
-                l[a] = (row > j && row < M) ? colp[row] : 0.0;
-            }
+            for (int u = 0; u < SOLVE_RING; ++u) {
+                if (jl + u < jend) {
+                    const int j = ja * 64 + jl + u;
 #pragma unroll
-            for (int cc = 0; cc < SOLVE_COLS; ++cc) {
-                const double xj = readlane_d(x[cc][ja], jl);
+                    for (int cc = 0; cc < SOLVE_COLS; ++cc) {
+                        const double xj = readlane_d(x[cc][ja], jl + u);
 #pragma unroll
-                for (int a = ja; a < NA; ++a) x[cc][a] -= l[a] * xj;
+                        for (int a = ja; a < NA; ++a) x[cc][a] -= buf[u][a] * xj;
+                    }
+                    if (j + SOLVE_RING < M) load_lower(buf[u], j + SOLVE_RING, ja);
+                }
             }
         }
     }
-    // backward substitution with U
+    // backward substitution with U (column j in ring slot (M - 1 - j) % SOLVE_RING)
+#pragma unroll
+    for (int u = 0; u < SOLVE_RING; ++u)
+        if (M - 1 - u >= 0) load_upper(buf[u], M - 1 - u, NA - 1);
 #pragma unroll
     for (int ja = NA - 1; ja >= 0; --ja) {
         const int jend = min(64, M - ja * 64);
-        for (int jl = jend - 1; jl >= 0; --jl) {
-            const int j = ja * 64 + jl;
-            const double *colp = F + static_cast<long long>(j) * M;
-            double l[NA];
+        const int t0 = M - ja * 64 - jend;
+#pragma clang loop unroll(disable)
+        for (int g0 = 0; g0 < jend; g0 += SOLVE_RING) {
 #pragma unroll
-            for (int a = 0; a <= ja; ++a) {
-                const int row = a * 64 + lane;
-                l[a] = row <= j ? colp[row] : 0.0;
-            }
-            const double diag = readlane_d(l[ja], jl);
+            for (int u = 0; u < SOLVE_RING; ++u) {
+                const int jl = jend - 1 - g0 - u;
+                if (jl >= 0) {
+                    const int j = ja * 64 + jl;
+                    double(&cur)[NA] = buf[(t0 + u) % SOLVE_RING];
+                    const double diag = readlane_d(cur[ja], jl);
 #pragma unroll
-            for (int cc = 0; cc < SOLVE_COLS; ++cc) {
-                const double xj = readlane_d(x[cc][ja], jl) / diag;
-                if (lane == jl) x[cc][ja] = xj;
+                    for (int cc = 0; cc < SOLVE_COLS; ++cc) {
+                        const double xj = readlane_d(x[cc][ja], jl) / diag;
+                        if (lane == jl) x[cc][ja] = xj;
 #pragma unroll
-                for (int a = 0; a <= ja; ++a) {
-                    const int row = a * 64 + lane;
-                    if (row < j) x[cc][a] -= l[a] * xj;
+                        for (int a = 0; a <= ja; ++a) {
+                            const int row = a * 64 + lane;
+                            if (row < j) x[cc][a] -= cur[a] * xj;
+                        }
+                    }
+                    if (j - SOLVE_RING >= 0) load_upper(cur, j - SOLVE_RING, ja);
                 }
             }
         }
@@ -275,7 +314,7 @@ __global__ __launch_bounds__(BLOCK) void k_hdg_schur(const double *__restrict__ 
 #pragma unroll
             for (int a = 0; a < NA; ++a) {
                 const int r = a * 64 + lane;
-                if (r < T) acc[a] += Sk[static_cast<long long>(c) * T + r] * xc;
+                if (r < T) acc[a] += __builtin_nontemporal_load(Sk + static_cast<long long>(c) * T + r) * xc;
             }
         }
     }

@@ -226,8 +226,8 @@ __device__ __forceinline__ float readlane_d(float v, int lane) {
 //    in registers (readlane broadcasts) and writes the updated column back;
 //  * interchanges of later panels are applied to the finished L columns in one
 //    final gather pass.
-template <int M, int NB, typename T>
-__global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__restrict__ rowsrc, int nblocks,
+template <int M, int NB, typename T, int LT>
+__global__ __launch_bounds__(LT) void k_lu(T *__restrict__ blocks, int *__restrict__ rowsrc, int nblocks,
                                               int *flag) {
     constexpr int NA = (M + 63) / 64;  // row chunks per lane
     __shared__ T panel[NB * M];         // [c][r], rows k0..M-1 of the current panel
@@ -235,20 +235,20 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
     __shared__ int piv[M];              // row interchanged with row k at step k
     __shared__ int sigma[M];            // gather map of the current panel's interchanges
     __shared__ int tmp[M];
-    __shared__ double red_val[BLOCK / 64];
-    __shared__ int red_idx[BLOCK / 64];
+    __shared__ double red_val[LT / 64];
+    __shared__ int red_idx[LT / 64];
     __shared__ int bad;
     const int blk = blockIdx.x;
     T *A = blocks + static_cast<long long>(blk) * M * M;
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const double tiny = sizeof(T) == 8 ? DBL_MIN : FLT_MIN;
-    for (int r = threadIdx.x; r < M; r += BLOCK) perm[r] = r;
+    for (int r = threadIdx.x; r < M; r += LT) perm[r] = r;
     if (threadIdx.x == 0) bad = 0;
     __syncthreads();
     for (int k0 = 0; k0 < M; k0 += NB) {
         const int nb = min(NB, M - k0), rows = M - k0;
         // (a) panel columns k0..k0+nb-1, rows k0..M-1 (already in the current row order)
-        for (int idx = threadIdx.x; idx < nb * rows; idx += BLOCK) {
+        for (int idx = threadIdx.x; idx < nb * rows; idx += LT) {
             const int c = idx / rows, r = k0 + idx % rows;
             panel[c * M + r] = A[static_cast<long long>(k0 + c) * M + r];
         }
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
             const int col = k0 + c;
             double best = -1.0;
             int bidx = M;
-            for (int r = col + threadIdx.x; r < M; r += BLOCK) {
+            for (int r = col + threadIdx.x; r < M; r += LT) {
                 const double a = fabs(static_cast<double>(panel[c * M + r]));
                 if (a > best || (a == best && r < bidx)) { best = a; bidx = r; }
             }
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
             if (threadIdx.x == 0) {
                 double b = red_val[0];
                 int bi = red_idx[0];
-                for (int w = 1; w < BLOCK / 64; ++w)
+                for (int w = 1; w < LT / 64; ++w)
                     if (red_val[w] > b || (red_val[w] == b && red_idx[w] < bi)) { b = red_val[w]; bi = red_idx[w]; }
                 if (bi >= M) bi = col;  // all-NaN column: keep the diagonal, flagged below
                 piv[col] = bi;
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
             __syncthreads();
             const int p = piv[col];
             if (p != col)
-                for (int cc = threadIdx.x; cc < nb; cc += BLOCK) {
+                for (int cc = threadIdx.x; cc < nb; cc += LT) {
                     const T t = panel[cc * M + col];
                     panel[cc * M + col] = panel[cc * M + p];
                     panel[cc * M + p] = t;
@@ -290,16 +290,16 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
             __syncthreads();
             const T pivot = panel[c * M + col];
             if (pivot != T(0))
-                for (int r = col + 1 + threadIdx.x; r < M; r += BLOCK) panel[c * M + r] /= pivot;
+                for (int r = col + 1 + threadIdx.x; r < M; r += LT) panel[c * M + r] /= pivot;
             __syncthreads();
-            for (int idx = threadIdx.x; idx < (nb - c - 1) * (M - col - 1); idx += BLOCK) {
+            for (int idx = threadIdx.x; idx < (nb - c - 1) * (M - col - 1); idx += LT) {
                 const int cc = c + 1 + idx / (M - col - 1), r = col + 1 + idx % (M - col - 1);
                 panel[cc * M + r] -= panel[c * M + r] * panel[cc * M + col];
             }
             __syncthreads();
         }
         // (c) panel write-back; (d) gather map of the panel's interchanges
-        for (int idx = threadIdx.x; idx < nb * rows; idx += BLOCK) {
+        for (int idx = threadIdx.x; idx < nb * rows; idx += LT) {
             const int c = idx / rows, r = k0 + idx % rows;
             const T val = panel[c * M + r];
             if (!isfinite(static_cast<double>(val)) || (r == k0 + c && !(fabs(static_cast<double>(val)) > tiny)))
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
         __syncthreads();
         // (e) per trailing column j: gather rows through sigma, U12 segment by
         //     forward substitution with the unit L11, A22 -= L21 U12, write back.
-        for (int j = k0 + nb + wave; j < M; j += BLOCK / 64) {
+        for (int j = k0 + nb + wave; j < M; j += LT / 64) {
             T *colp = A + static_cast<long long>(j) * M;
             T x[NA];
 #pragma unroll
@@ -355,13 +355,13 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
     }
     // (f) apply the interchanges of later panels to the finished L columns:
     //     tau = gather map of all interchanges after the panel, built backwards.
-    for (int r = threadIdx.x; r < M; r += BLOCK) sigma[r] = r;
+    for (int r = threadIdx.x; r < M; r += LT) sigma[r] = r;
     __syncthreads();
     const int npanels = (M + NB - 1) / NB;
     for (int q = npanels - 1; q >= 0; --q) {
         const int k0 = q * NB, nb = min(NB, M - k0), lo = k0 + nb;
         if (q < npanels - 1)
-            for (int c = k0 + wave; c < k0 + nb; c += BLOCK / 64) {
+            for (int c = k0 + wave; c < k0 + nb; c += LT / 64) {
                 T *colp = A + static_cast<long long>(c) * M;
                 T x[NA];
 #pragma unroll
@@ -384,13 +384,13 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
             }
         }
         __syncthreads();
-        for (int r = k0 + threadIdx.x; r < M; r += BLOCK) {
+        for (int r = k0 + threadIdx.x; r < M; r += LT) {
             const int src = sigma[r];
             sigma[r] = tmp[src];
         }
         __syncthreads();
     }
-    for (int r = threadIdx.x; r < M; r += BLOCK) rowsrc[static_cast<long long>(blk) * M + r] = perm[r];
+    for (int r = threadIdx.x; r < M; r += LT) rowsrc[static_cast<long long>(blk) * M + r] = perm[r];
     if (threadIdx.x == 0 && bad) atomicOr(flag, 8);
 }
 
@@ -402,13 +402,39 @@ __global__ __launch_bounds__(BLOCK) void k_lu(T *__restrict__ blocks, int *__res
 // the loads of the next U - 1 columns in flight while a column is applied;
 // the ring slot of a column is fixed at compile time (the chunk loops are
 // unrolled), so no registers move.
-#ifndef DGH_BJ_RING
-#define DGH_BJ_RING 4
+// columns in flight per wave in k_bj_apply, by stored factor type
+#ifndef DGH_BJ_RING64
+#define DGH_BJ_RING64 4
 #endif
-constexpr int BJ_RING = DGH_BJ_RING;  // columns in flight per wave in k_bj_apply
+#ifndef DGH_BJ_RING32
+#define DGH_BJ_RING32 4
+#endif
+#ifndef DGH_BJ_RING16
+#define DGH_BJ_RING16 4
+#endif
+constexpr int BJ_RING64 = DGH_BJ_RING64, BJ_RING32 = DGH_BJ_RING32, BJ_RING16 = DGH_BJ_RING16;
 __device__ __forceinline__ float to_compute(__hip_bfloat16 v) { return __bfloat162float(v); }
 __device__ __forceinline__ float to_compute(float v) { return v; }
 __device__ __forceinline__ double to_compute(double v) { return v; }
+
+// Factor loads: each factor is read once per apply (55 GB in FP64 at 256^2,
+// k=8) and never reused before it is evicted, so the loads are marked
+// non-temporal (measured: FP64 apply 9.98 -> 9.18 ms, FP32 5.33 -> 4.87 ms).
+#ifndef DGH_BJ_NONTEMPORAL
+#define DGH_BJ_NONTEMPORAL 1
+#endif
+template <typename T> __device__ __forceinline__ T factor_load(const T *p) {
+    if constexpr (!DGH_BJ_NONTEMPORAL) {
+        return *p;
+    } else if constexpr (std::is_same<T, __hip_bfloat16>::value) {
+        const unsigned short bits = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+        T out;
+        __builtin_memcpy(&out, &bits, sizeof(out));
+        return out;
+    } else {
+        return __builtin_nontemporal_load(p);
+    }
+}
 
 // TF: stored factor type (double, float or bfloat16); TV: vector and
 // arithmetic type (double for the FP64 path, float for the inner FP32 solve).
@@ -435,7 +461,7 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int row = a * 64 + lane;
-            if (a >= a0) b[a] = (row > c && row < M) ? colp[row] : T(0);
+            if (a >= a0) b[a] = (row > c && row < M) ? factor_load(colp + row) : T(0);
         }
     };
     // upper part (diagonal included) of column c, chunks a <= a1
@@ -444,7 +470,7 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int row = a * 64 + lane;
-            if (a <= a1) b[a] = row <= c ? colp[row] : T(0);
+            if (a <= a1) b[a] = row <= c ? factor_load(colp + row) : T(0);
         }
     };
     // forward substitution with the unit lower L: x_i -= L_ij x_j for i > j;
@@ -854,29 +880,51 @@ hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache
         k_assemble<N, double, true><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
     })
 }
+// Panel width and workgroup size of k_lu. A wider panel halves the passes
+// over the trailing matrix but its LDS panel caps the resident workgroups
+// per CU; measured at 256^2, k=8 the 16-column panel wins (NB = 32: FP64
+// 1066 ms, FP32 338 ms; NB = 8: 376 / 211 ms; NB = 16: see the thread counts).
+#ifndef DGH_LU_NB64
+#define DGH_LU_NB64 16
+#endif
+#ifndef DGH_LU_NB32
+#define DGH_LU_NB32 16
+#endif
+// FP64 blocks factor faster with 8 waves per workgroup, FP32 blocks with 4
+// (measured, 256^2 k=8: 245 vs 353 ms and 179 vs 206 ms)
+#ifndef DGH_LU_THREADS64
+#define DGH_LU_THREADS64 512
+#endif
+#ifndef DGH_LU_THREADS32
+#define DGH_LU_THREADS32 256
+#endif
 hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s) {
-    DGH_DISPATCH_M(m, { k_lu<M, 16, double><<<nblocks, BLOCK, 0, s>>>(blocks, rowsrc, nblocks, flag); })
+    DGH_DISPATCH_M(m, {
+        k_lu<M, DGH_LU_NB64, double, DGH_LU_THREADS64><<<nblocks, DGH_LU_THREADS64, 0, s>>>(blocks, rowsrc, nblocks, flag);
+    })
 }
 hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s) {
-    DGH_DISPATCH_M(m, { k_lu<M, 16, float><<<nblocks, BLOCK, 0, s>>>(blocks, rowsrc, nblocks, flag); })
+    DGH_DISPATCH_M(m, {
+        k_lu<M, DGH_LU_NB32, float, DGH_LU_THREADS32><<<nblocks, DGH_LU_THREADS32, 0, s>>>(blocks, rowsrc, nblocks, flag);
+    })
 }
 hipError_t launch_bj_apply(int m, const double *factors, const int *rowsrc, const double *r, double *z,
                            int nblocks, hipStream_t s) {
     DGH_DISPATCH_M(m, {
-        k_bj_apply<M, double, double, BJ_RING><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks);
+        k_bj_apply<M, double, double, BJ_RING64><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks);
     })
 }
 hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const float *r, float *z, int nblocks,
                            hipStream_t s) {
     DGH_DISPATCH_M(m, {
-        k_bj_apply<M, float, float, BJ_RING><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks);
+        k_bj_apply<M, float, float, BJ_RING32><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks);
     })
 }
 hipError_t launch_bj_apply_bf16(int m, const void *factors, const int *rowsrc, const float *r, float *z,
                                 int nblocks, hipStream_t s) {
     const auto *f = static_cast<const __hip_bfloat16 *>(factors);
     DGH_DISPATCH_M(m, {
-        k_bj_apply<M, __hip_bfloat16, float, BJ_RING><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, rowsrc, r, z, nblocks);
+        k_bj_apply<M, __hip_bfloat16, float, BJ_RING16><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, rowsrc, r, z, nblocks);
     })
 }
 hipError_t launch_convert(const double *src, float *dst, long long count, hipStream_t s) {
