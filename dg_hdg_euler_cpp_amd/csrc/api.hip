@@ -190,6 +190,13 @@ struct dgh_ctx {
     Trajectory trajectory = PLAIN;
     DevBuf<double> ck_states, ck_speeds, adj_kbar, adj_lambda, adj_rhs, adj_base;
     std::vector<double> ck_step, ck_rate;
+    // Graph-captured GMRES (cfg.graph): one executable graph per Arnoldi step
+    // index, captured within a linear solve on first use; the Hessenberg
+    // matrix, rotations and rotated right-hand side live on the device and
+    // three scalars per step come back through pinned, mapped host memory.
+    std::vector<hipGraphExec_t> step_graphs;
+    DevBuf<double> gH, gcs, gsn, gg, gnext;
+    double *gout_host = nullptr, *gout_dev = nullptr;
     int ck_stages = 0;
     bool adj_bj_ready = false;
 
@@ -257,9 +264,19 @@ struct dgh_ctx {
         flags.release(bytes);
         rate_bits.release(bytes);
         basis.release(bytes);
+        drop_step_graphs();
+        if (gout_host) (void)hipHostFree(gout_host);
+        DevBuf<double> *gbufs[] = {&gH, &gcs, &gsn, &gg, &gnext};
+        for (auto *b : gbufs) b->release(bytes);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void drop_step_graphs() {
+        for (auto &e : step_graphs)
+            if (e) (void)hipGraphExecDestroy(e);
+        step_graphs.clear();
     }
 
     // ---- timing of kernel groups on the context stream
@@ -479,6 +496,32 @@ struct dgh_ctx {
         auto Hm = [&](int i, int j) -> double & { return H[i + (restart + 1) * j]; };
         int iterations = 0;
         double host_scalars[65];
+        // every captured argument (operators, caches, step size, buffers) is
+        // fixed for the duration of this solve, so graphs live for one solve
+        const bool use_graph = cfg.graph != 0 && !timing;
+        if (use_graph) {
+            drop_step_graphs();
+            step_graphs.assign(restart, nullptr);
+            gH.alloc(static_cast<size_t>(restart + 1) * restart, bytes);
+            gcs.alloc(restart, bytes);
+            gsn.alloc(restart, bytes);
+            gg.alloc(restart + 1, bytes);
+            gnext.alloc(1, bytes);
+            if (!gout_host) {
+                HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&gout_host), 4 * sizeof(double), hipHostMallocMapped));
+                HIPCHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&gout_dev), gout_host, 0));
+            }
+        }
+        auto arnoldi_ortho = [&](int j) {
+            HIPCHECK(launch_multi_dot(n, V.data(), j + 1, kw.p, weights, D, partials.p, basis.p, stream));
+            HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p, stream));
+            HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p, kw.p, weights, D, partials.p, 0, basis.p,
+                                       stream));
+            HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p + 32, stream));
+            HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p + 32, kw.p, weights, D, partials.p, 1,
+                                       basis.p, stream));
+            HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 1, scalars.p + 64, stream));
+        };
         while (iterations < limit) {
             const double beta = vec_norm(kres.p, D, weights);
             if (beta <= tolerance * rhs_norm) return {iterations, beta / rhs_norm};
@@ -487,18 +530,40 @@ struct dgh_ctx {
             std::fill(g.begin(), g.end(), 0.0);
             g[0] = beta;
             int count = 0;
-            for (int j = 0; j < restart && iterations < limit; ++j) {
+            if (use_graph) {
+                HIPCHECK(hipMemcpyAsync(gg.p, g.data(), (restart + 1) * sizeof(double), hipMemcpyHostToDevice, stream));
+                for (int j = 0; j < restart && iterations < limit; ++j) {
+                    if (!step_graphs[j]) {
+                        hipGraph_t graph = nullptr;
+                        HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                        M(Vk(j), kz.p);
+                        A(kz.p, kw.p);
+                        arnoldi_ortho(j);
+                        HIPCHECK(launch_arnoldi_givens(j, restart, scalars.p, gH.p, gcs.p, gsn.p, gg.p, gnext.p,
+                                                       gout_dev, stream));
+                        HIPCHECK(launch_scale_div_dev(kw.p, gnext.p, Vk(j + 1), D, stream));
+                        HIPCHECK(hipStreamEndCapture(stream, &graph));
+                        const hipError_t e = hipGraphInstantiate(&step_graphs[j], graph, nullptr, nullptr, 0);
+                        (void)hipGraphDestroy(graph);
+                        HIPCHECK(e);
+                    }
+                    HIPCHECK(hipGraphLaunch(step_graphs[j], stream));
+                    sync();
+                    const double gabs = gout_host[0], next = gout_host[1], d = gout_host[2];
+                    require(std::isfinite(d) && d > 0, "GMRES Arnoldi breakdown");
+                    ++iterations;
+                    count = j + 1;
+                    if (gabs <= tolerance * rhs_norm || next <= std::numeric_limits<double>::min()) break;
+                }
+                HIPCHECK(hipMemcpyAsync(H.data(), gH.p, H.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+                HIPCHECK(hipMemcpyAsync(g.data(), gg.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+                sync();
+            }
+            for (int j = 0; !use_graph && j < restart && iterations < limit; ++j) {
                 M(Vk(j), kz.p);
                 A(kz.p, kw.p);
                 timed("gmres_ortho", [&] {
-                    HIPCHECK(launch_multi_dot(n, V.data(), j + 1, kw.p, weights, D, partials.p, basis.p, stream));
-                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p, stream));
-                    HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p, kw.p, weights, D, partials.p, 0,
-                                               basis.p, stream));
-                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p + 32, stream));
-                    HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p + 32, kw.p, weights, D, partials.p, 1,
-                                               basis.p, stream));
-                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 1, scalars.p + 64, stream));
+                    arnoldi_ortho(j);
                     HIPCHECK(hipMemcpyAsync(host_scalars, scalars.p, 65 * sizeof(double), hipMemcpyDeviceToHost,
                                             stream));
                     sync();

@@ -150,6 +150,7 @@ dgh_config parse_config(int argc, const char *const *argv) {
         } else if (option == "--inner-restart") c.inner_restart = integer();
         else if (option == "--inner-max") c.inner_max = integer();
         else if (option == "--inner-tol") c.inner_tolerance = real();
+        else if (option == "--graph") c.graph = 1;
         else if (option == "--seed") {
             auto text = value();
             size_t end;
@@ -165,7 +166,7 @@ dgh_config parse_config(int argc, const char *const *argv) {
                          "--integrator ssp-dirk3|be --newton-steps N --warmup-updates N\n"
                          "--timing-only --check-update --no-save-solution\n"
                          "--device gpu --precision fp64|mixed|mixed-bf16 --inner-restart N --inner-max N\n"
-                         "--inner-tol R --seed S\n";
+                         "--inner-tol R --seed S --graph\n";
             throw std::runtime_error("help");
         } else
             throw std::runtime_error("Unknown option: " + option);

@@ -136,6 +136,11 @@ hipError_t launch_sub_div(const double *x, const double *y, double d, double *ou
 // Transposed element blocks B^T (adjoint block Jacobi).
 hipError_t launch_assemble_blocks_transposed(int n, const double *vcache, const double *fcache, double *blocks,
                                              double inv_step, const Basis *b, const Geo &g, hipStream_t s);
+// Graph-captured GMRES: one Arnoldi step's Hessenberg / Givens update on the
+// device, and out = x / *beta with beta in device memory.
+hipError_t launch_arnoldi_givens(int j, int restart, const double *scalars, double *H, double *cs, double *sn,
+                                 double *g, double *next, double *out, hipStream_t s);
+hipError_t launch_scale_div_dev(const double *x, const double *beta, double *out, long long count, hipStream_t s);
 // gradient of the density integral over the quadrant x > 0, y > 0
 hipError_t launch_quadrant_weights(int n, double *g, const Basis *b, const Geo &geo, hipStream_t s);
 // partials[REDUCE_BLOCKS] of sum a_i b_i

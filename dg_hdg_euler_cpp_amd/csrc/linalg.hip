@@ -724,6 +724,49 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate(PtrsT<TV> V, const double *__
     store_partials<NV>(acc, mode == 0 ? NV : 1, partials);
 }
 
+// One Arnoldi step's scalar work on the device (gmres.cpp:44-66), for the
+// graph-captured GMRES: H column j from the two CGS passes, the new norm, the
+// earlier Givens rotations, the new rotation and the rotated right-hand side.
+// H is (R+1) x R column-major; s holds the pass-1 / pass-2 projections at
+// [0..j] / [32..32+j] and the squared norm at [64]. out (host-visible)
+// receives |g_{j+1}|, the new norm and the rotation's d.
+__global__ void k_arnoldi_givens(int j, int R, const double *__restrict__ s, double *H, double *cs, double *sn,
+                                 double *g, double *next_out, double *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    auto Hm = [&](int i, int k) -> double & { return H[i + (R + 1) * k]; };
+    for (int k = 0; k <= j; ++k) Hm(k, j) = (0.0 + s[k]) + s[32 + k];
+    const double next = sqrt(fmax(0.0, s[64]));
+    Hm(j + 1, j) = next;
+    *next_out = next;
+    for (int i = 0; i < j; ++i) {
+        const double a = Hm(i, j), b = Hm(i + 1, j);
+        Hm(i, j) = cs[i] * a + sn[i] * b;
+        Hm(i + 1, j) = -sn[i] * a + cs[i] * b;
+    }
+    const double d = hypot(Hm(j, j), Hm(j + 1, j));
+    if (isfinite(d) && d > 0) {
+        cs[j] = Hm(j, j) / d;
+        sn[j] = Hm(j + 1, j) / d;
+        Hm(j, j) = d;
+        Hm(j + 1, j) = 0;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] *= cs[j];
+    }
+    out[0] = fabs(g[j + 1]);
+    out[1] = next;
+    out[2] = d;
+}
+
+// out = x / *beta when *beta > 0 (basis_.col(j + 1) = column_ / next)
+__global__ __launch_bounds__(BLOCK) void k_scale_div_dev(const double *__restrict__ x, const double *__restrict__ beta,
+                                                         double *__restrict__ out, long long count) {
+    const double b = *beta;
+    if (!(b > 0)) return;
+    for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
+         i += static_cast<long long>(gridDim.x) * BLOCK)
+        out[i] = x[i] / b;
+}
+
 // out = sum_k V_k y_k (gmres.cpp:75)
 template <typename TV, typename TO>
 __global__ __launch_bounds__(BLOCK) void k_combine(PtrsT<TV> V, int nvec, const double *__restrict__ y,
@@ -1054,6 +1097,15 @@ hipError_t launch_scale_div(const double *x, double d, float *out, long long cou
 }
 hipError_t launch_quadrant_weights(int n, double *g, const Basis *b, const Geo &geo, hipStream_t s) {
     DGH_DISPATCH_N(n, { k_quadrant_weights<N><<<4096, BLOCK, 0, s>>>(g, b, geo); })
+}
+hipError_t launch_arnoldi_givens(int j, int restart, const double *scalars, double *H, double *cs, double *sn,
+                                 double *g, double *next, double *out, hipStream_t s) {
+    k_arnoldi_givens<<<1, 64, 0, s>>>(j, restart, scalars, H, cs, sn, g, next, out);
+    return hipGetLastError();
+}
+hipError_t launch_scale_div_dev(const double *x, const double *beta, double *out, long long count, hipStream_t s) {
+    k_scale_div_dev<<<4096, BLOCK, 0, s>>>(x, beta, out, count);
+    return hipGetLastError();
 }
 hipError_t launch_dot(const double *a, const double *b, long long count, double *partials, hipStream_t s) {
     k_dot<<<REDUCE_BLOCKS, BLOCK, 0, s>>>(a, b, count, partials);

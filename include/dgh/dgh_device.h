@@ -67,6 +67,7 @@ typedef struct dgh_config {
     int inner_max;           /* mixed: iteration cap of one inner solve */
     double inner_tolerance;  /* mixed: relative tolerance of one inner solve */
     unsigned long long seed;
+    int graph;               /* 1: each GMRES Arnoldi step replays a captured hipGraph */
     char prefix[512];
 } dgh_config;
 

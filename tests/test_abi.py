@@ -41,7 +41,7 @@ class Cfg(C.Structure):
                 ("timing_only", C.c_int), ("check_update", C.c_int), ("save_solution", C.c_int),
                 ("integrator", C.c_int), ("method", C.c_int), ("testcase", C.c_int), ("precision", C.c_int),
                 ("inner_restart", C.c_int), ("inner_max", C.c_int), ("inner_tolerance", C.c_double),
-                ("seed", C.c_ulonglong), ("prefix", C.c_char * 512)]
+                ("seed", C.c_ulonglong), ("graph", C.c_int), ("prefix", C.c_char * 512)]
 
 
 def test_config_struct_layout_and_defaults():
@@ -57,6 +57,7 @@ def test_config_struct_layout_and_defaults():
     c = Cfg.from_buffer_copy(parse(["--nx", "256", "--k", "8", "--integrator", "be", "--steps", "1",
                                     "--precision", "mixed", "--device", "gpu", "--restart", "20"]))
     assert (c.nx, c.degree, c.integrator, c.steps, c.precision, c.restart) == (256, 8, 0, 1, 1, 20)
+    assert c.graph == 0 and Cfg.from_buffer_copy(parse(["--graph"])).graph == 1
 
 
 @pytest.mark.parametrize("args,message", [

@@ -1,0 +1,66 @@
+"""GMRES with graph-captured Arnoldi steps (--graph) against the plain host-
+driven loop: the same Krylov counts and the same solutions to rounding (the
+Givens rotations run on the device instead of the host), across several
+restart cycles (so every step graph is replayed), for the DG and HDG systems
+and through whole Newton updates."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _solve(args, graph, torch):
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    print(f"\n[case] {args} graph={graph}", flush=True)
+    with DeviceContext(args + (["--graph"] if graph else [])) as dev:
+        u = dev.initial()
+        rate = dev.freeze(u)
+        dev.set_step(5.0 / rate, u)
+        r = dev.dg_residual(u)
+        dev.dg_refresh(u)
+        dev.bj_refresh()
+        x, its, relres, _ = dev.dg_gmres(-r, 1500, 1e-12)
+        out = {"x": x.cpu().numpy(), "its": its, "rel": relres}
+        dev.hdg_refresh_projected(u, r)
+        dev.hdg_bj_refresh()
+        xl, its_h, rel_h, _ = dev.hdg_gmres(dev.hdg_rhs(), 1500, 1e-12)
+        out.update(xh=xl.cpu().numpy(), its_h=its_h, rel_h=rel_h)
+    return out
+
+
+@pytest.mark.parametrize("args", [["--nx", "16", "--ny", "16", "--k", "8"], ["--nx", "12", "--ny", "8", "--k", "4"]],
+                         ids=["k8-16x16", "k4-12x8"])
+def test_graph_gmres_matches_host_loop(args, torch_cuda):
+    base = args + ["--cfl", "5", "--integrator", "be", "--restart", "20"]
+    plain = _solve(base, False, torch_cuda)
+    graph = _solve(base, True, torch_cuda)
+    print(f"DG its {plain['its']} / {graph['its']}, HDG its {plain['its_h']} / {graph['its_h']}")
+    assert plain["its"] > 20 and plain["its_h"] > 20   # several restart cycles
+    assert abs(graph["its"] - plain["its"]) <= 1 and abs(graph["its_h"] - plain["its_h"]) <= 1
+    assert graph["rel"] <= 1e-12 and graph["rel_h"] <= 1e-12
+    # both solutions meet the same 1e-12 residual; they differ by rounding of
+    # the rotations, amplified by the conditioning of the Newton operator
+    assert rel(graph["x"], plain["x"]) <= 1e-9
+    assert rel(graph["xh"], plain["xh"]) <= 1e-9
+
+
+def test_graph_newton_updates_match(torch_cuda):
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    args = ["--nx", "32", "--ny", "32", "--k", "8", "--method", "dg", "--integrator", "be", "--cfl", "5",
+            "--steps", "1", "--newton-steps", "3", "--restart", "20"]
+    runs = []
+    for graph in (False, True):
+        with DeviceContext(args + (["--graph"] if graph else [])) as dev:
+            u = dev.initial()
+            runs.append(dev.advance(u))
+    a = [x["residual_physical_rms"] for x in runs[0]["updates"]]
+    b = [x["residual_physical_rms"] for x in runs[1]["updates"]]
+    print("rms", a, b)
+    for k, (x, y) in enumerate(zip(a, b)):
+        assert abs(x - y) <= [1e-13, 1e-10, 1e-6][k] * x + 1e-12 * a[0]
