@@ -22,6 +22,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+GRAPH = False  # --graph: GMRES Arnoldi steps replay captured hipGraphs
 
 
 def log(*a):
@@ -35,6 +36,8 @@ def solver_args(nx, k, init, precision, newton_steps, warmup, device=True, metho
             "--timing-only", "--testcase", init]
     if device:
         args += ["--seed", "2025", "--precision", precision]
+        if GRAPH:
+            args += ["--graph"]
     return args
 
 
@@ -227,9 +230,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-hdg", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the untimed mixed-vs-FP64 update comparison")
+    ap.add_argument("--graph", action="store_true", help="GMRES Arnoldi steps as captured hipGraphs")
     ap.add_argument("--only-hdg", action="store_true", help="run and report the condensed HDG form alone")
     ap.add_argument("--probe", action="store_true", help="run each hot kernel 3x for counter profiles")
     args = ap.parse_args()
+    global GRAPH
+    GRAPH = args.graph
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X; multi-GPU runs are not part of this build "
               "(the path would shard by 2D element tiles, see DESIGN.md)", file=sys.stderr)
