@@ -76,8 +76,9 @@ std::vector<double> lagrange_at(const std::vector<double> &x, double z) {
     return v;
 }
 
-// nodal_basis.hpp:34-65 for the collocated rule (points == nodes).
-Basis make_basis(int n, double hx, double hy) {
+// nodal_basis.hpp:34-65: the nodal rule, plus the Q-point volume rule when
+// over-integrating (Q > n).
+Basis make_basis(int n, double hx, double hy, int Q = 0) {
     Basis b{};
     std::vector<double> x, w;
     gauss_rule(n, x, w);
@@ -114,6 +115,21 @@ Basis make_basis(int n, double hx, double hy) {
     }
     for (int i = 0; i < n; ++i)
         for (int q = 0; q < eq; ++q) b.proj[i + n * q] = ((1.0 / w[i]) * b.ev[q + eq * i]) * ew[q];
+    if (Q > n) {
+        std::vector<double> qx, qw;
+        gauss_rule(Q, qx, qw);
+        for (int q = 0; q < Q; ++q) {
+            b.qw[q] = qw[q];
+            const auto e = lagrange_at(x, qx[q]);
+            for (int k = 0; k < n; ++k) b.vq[q + Q * k] = e[k];
+        }
+        for (int q = 0; q < Q; ++q)
+            for (int j = 0; j < n; ++j) {
+                double s = 0;
+                for (int k = 0; k < n; ++k) s += (qw[q] * b.vq[q + Q * k]) * D[k + n * j];
+                b.dq[q + Q * j] = s;
+            }
+    }
     return b;
 }
 
@@ -216,6 +232,7 @@ struct dgh_ctx {
         geo.n = n;
         geo.cells = cells;
         geo.eq = std::max(20, 2 * n + 3);
+        geo.nq = c.quadrature > n ? c.quadrature : n;
         geo.testcase = c.testcase;
         geo.lx = c.testcase == DGH_DIAGONAL ? 2. / cosine : 2.;
         geo.ly = c.testcase == DGH_DIAGONAL ? 2. / sine : 2.;
@@ -225,7 +242,7 @@ struct dgh_ctx {
         geo.safety = c.safety;
         geo.cosine = cosine;
         geo.sine = sine;
-        hbasis = make_basis(n, geo.hx, geo.hy);
+        hbasis = make_basis(n, geo.hx, geo.hy, geo.nq);
         HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIPCHECK(hipEventCreate(&ev0));
         HIPCHECK(hipEventCreate(&ev1));
@@ -237,7 +254,7 @@ struct dgh_ctx {
         tr_x.alloc(16LL * cells * n, bytes);
         fq.alloc(static_cast<size_t>(T), bytes);
         gtr.alloc(2 * static_cast<size_t>(T), bytes);
-        vcache.alloc(3LL * cells * P, bytes);
+        vcache.alloc(3LL * cells * geo.nq * geo.nq, bytes);
         fcache.alloc(static_cast<size_t>(F_NFIELDS) * faces * n, bytes);
         partials.alloc(REDUCE_BLOCKS * 32, bytes);
         scalars.alloc(256, bytes);
@@ -296,6 +313,7 @@ struct dgh_ctx {
         t.launches += 1;
     }
     void sync() { HIPCHECK(hipStreamSynchronize(stream)); }
+    bool oi() const { return geo.nq > n; }  // over-integrated volume rule (--quad Q > k+1)
 
     // ---- reductions
     double reduce_sq(const double *x, long long count, int mode, const double *weights = nullptr) {
@@ -350,7 +368,8 @@ struct dgh_ctx {
         HIPCHECK(hipMemsetAsync(rate_bits.p, 0, sizeof(unsigned long long), stream));
         HIPCHECK(launch_restrict(n, u, tr_u.p, basis.p, geo, stream));
         HIPCHECK(launch_freeze_speeds(n, tr_u.p, speeds.p, flags.p, geo, stream));
-        HIPCHECK(launch_freeze_rate(n, u, speeds.p, rate_bits.p, flags.p, geo, stream));
+        if (oi()) HIPCHECK(launch_oi_freeze_rate(n, u, speeds.p, rate_bits.p, flags.p, basis.p, geo, stream));
+        else HIPCHECK(launch_freeze_rate(n, u, speeds.p, rate_bits.p, flags.p, geo, stream));
         copy(old.p, u, D);
         unsigned long long bits = 0;
         HIPCHECK(hipMemcpyAsync(&bits, rate_bits.p, sizeof(bits), hipMemcpyDeviceToHost, stream));
@@ -368,12 +387,14 @@ struct dgh_ctx {
         timed("residual", [&] {
             HIPCHECK(launch_restrict(n, u, tr_u.p, basis.p, geo, stream));
             HIPCHECK(launch_residual_flux(n, tr_u.p, speeds.p, fq.p, basis.p, geo, stream));
-            HIPCHECK(launch_element_residual(n, u, old.p, fq.p, r, tau, basis.p, geo, stream));
+            if (oi()) HIPCHECK(launch_oi_element_residual(n, u, old.p, fq.p, r, tau, basis.p, geo, stream));
+            else HIPCHECK(launch_element_residual(n, u, old.p, fq.p, r, tau, basis.p, geo, stream));
         });
     }
     bool admissible(const double *u) {
         set_flag(1, 1);
         HIPCHECK(launch_admissible(n, u, flags.p + 1, basis.p, geo, stream));
+        if (oi()) HIPCHECK(launch_oi_admissible(n, u, flags.p + 1, basis.p, geo, stream));
         return read_flag(1) != 0;
     }
     std::array<double, 4> error_l2(const double *u, double time) {
@@ -391,7 +412,8 @@ struct dgh_ctx {
         require(tau > 0, "Set the time step before refreshing the Jacobian");
         require(frozen, "Freeze the wave speeds before refreshing the Jacobian");
         inv_step = 1 / tau;
-        HIPCHECK(launch_volume_cache(n, u, vcache.p, geo, stream));
+        if (oi()) HIPCHECK(launch_oi_volume_cache(n, u, vcache.p, basis.p, geo, stream));
+        else HIPCHECK(launch_volume_cache(n, u, vcache.p, geo, stream));
         HIPCHECK(launch_restrict(n, u, tr_u.p, basis.p, geo, stream));
         HIPCHECK(launch_face_cache(n, tr_u.p, speeds.p, fcache.p, basis.p, geo, stream));
         refreshed = true;
@@ -406,7 +428,8 @@ struct dgh_ctx {
         timed("jv", [&] {
             HIPCHECK(launch_restrict(n, x, tr_x.p, basis.p, geo, stream));
             HIPCHECK(launch_jv_flux(n, tr_x.p, fcache.p, fq.p, geo, stream));
-            HIPCHECK(launch_element_jv(n, x, vcache.p, fq.p, y, inv_step, basis.p, geo, stream));
+            if (oi()) HIPCHECK(launch_oi_element_jv(n, x, vcache.p, fq.p, y, inv_step, basis.p, geo, stream));
+            else HIPCHECK(launch_element_jv(n, x, vcache.p, fq.p, y, inv_step, basis.p, geo, stream));
         });
     }
     void dg_apply_transpose(const double *w, double *z) {
@@ -414,7 +437,8 @@ struct dgh_ctx {
         timed("jtv", [&] {
             HIPCHECK(launch_restrict(n, w, tr_x.p, basis.p, geo, stream));
             HIPCHECK(launch_adjoint_face(n, tr_x.p, fcache.p, gtr.p, geo, stream));
-            HIPCHECK(launch_element_jtv(n, w, vcache.p, gtr.p, z, inv_step, basis.p, geo, stream));
+            if (oi()) HIPCHECK(launch_oi_element_jtv(n, w, vcache.p, gtr.p, z, inv_step, basis.p, geo, stream));
+            else HIPCHECK(launch_element_jtv(n, w, vcache.p, gtr.p, z, inv_step, basis.p, geo, stream));
         });
     }
     void bj_refresh() {
@@ -511,6 +535,10 @@ struct dgh_ctx {
                 HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&gout_host), 4 * sizeof(double), hipHostMallocMapped));
                 HIPCHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&gout_dev), gout_host, 0));
             }
+            // one eager application of both operators, so that any buffer they
+            // allocate on first use exists before a capture begins
+            M(rhs, kz.p);
+            A(kz.p, kw.p);
         }
         auto arnoldi_ortho = [&](int j) {
             HIPCHECK(launch_multi_dot(n, V.data(), j + 1, kw.p, weights, D, partials.p, basis.p, stream));
@@ -536,12 +564,18 @@ struct dgh_ctx {
                     if (!step_graphs[j]) {
                         hipGraph_t graph = nullptr;
                         HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-                        M(Vk(j), kz.p);
-                        A(kz.p, kw.p);
-                        arnoldi_ortho(j);
-                        HIPCHECK(launch_arnoldi_givens(j, restart, scalars.p, gH.p, gcs.p, gsn.p, gg.p, gnext.p,
-                                                       gout_dev, stream));
-                        HIPCHECK(launch_scale_div_dev(kw.p, gnext.p, Vk(j + 1), D, stream));
+                        try {
+                            M(Vk(j), kz.p);
+                            A(kz.p, kw.p);
+                            arnoldi_ortho(j);
+                            HIPCHECK(launch_arnoldi_givens(j, restart, scalars.p, gH.p, gcs.p, gsn.p, gg.p, gnext.p,
+                                                           gout_dev, stream));
+                            HIPCHECK(launch_scale_div_dev(kw.p, gnext.p, Vk(j + 1), D, stream));
+                        } catch (...) {
+                            // leave the stream out of capture mode before reporting
+                            if (hipStreamEndCapture(stream, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+                            throw;
+                        }
                         HIPCHECK(hipStreamEndCapture(stream, &graph));
                         const hipError_t e = hipGraphInstantiate(&step_graphs[j], graph, nullptr, nullptr, 0);
                         (void)hipGraphDestroy(graph);
@@ -721,7 +755,8 @@ struct dgh_ctx {
         timed("jv", [&] {
             HIPCHECK(launch_restrict(n, x, tr_x.p, basis.p, geo, stream));
             HIPCHECK(launch_jv_flux(n, tr_x.p, fcache.p, fq.p, geo, stream));
-            HIPCHECK(launch_element_jv(n, x, vcache.p, fq.p, y, inv_step, basis.p, geo, stream));
+            if (oi()) HIPCHECK(launch_oi_element_jv(n, x, vcache.p, fq.p, y, inv_step, basis.p, geo, stream));
+            else HIPCHECK(launch_element_jv(n, x, vcache.p, fq.p, y, inv_step, basis.p, geo, stream));
         });
     }
     void dg_apply_mixed(const float *x, double *y) {
@@ -729,7 +764,8 @@ struct dgh_ctx {
         timed("jv", [&] {
             HIPCHECK(launch_restrict(n, x, tr_x.p, basis.p, geo, stream));
             HIPCHECK(launch_jv_flux(n, tr_x.p, fcache.p, fq.p, geo, stream));
-            HIPCHECK(launch_element_jv(n, x, vcache.p, fq.p, y, inv_step, basis.p, geo, stream));
+            if (oi()) HIPCHECK(launch_oi_element_jv(n, x, vcache.p, fq.p, y, inv_step, basis.p, geo, stream));
+            else HIPCHECK(launch_element_jv(n, x, vcache.p, fq.p, y, inv_step, basis.p, geo, stream));
         });
     }
     double dot32(const float *a) {
@@ -1268,8 +1304,7 @@ int dgh_create(const dgh_config *cfg, dgh_ctx **out) {
     return guarded(nullptr, [&] {
         require(cfg != nullptr && out != nullptr, "Null argument");
         validate_config(*cfg);
-        require(cfg->quadrature == 0 || cfg->quadrature == cfg->degree + 1,
-                "Over-integration (--quad > k+1) is not available on the device path");
+        require(cfg->quadrature <= MAXQ, "Over-integration above 16 points per direction is not built on the device");
         *out = new dgh_ctx(*cfg);
     });
 }

@@ -11,6 +11,7 @@ namespace dgh {
 constexpr int NVAR = 4;
 constexpr int MAXN = 9;    // degree <= 8 (reference config.cpp:7)
 constexpr int MAXEQ = 21;  // max(20, 2n+3) error/projection points
+constexpr int MAXQ = 16;   // over-integration: volume quadrature points per direction
 
 // Reference-element tables of one context (nodal_basis.hpp:34-65), uploaded
 // once; kernels read them through uniform (scalar-cache) loads.
@@ -22,11 +23,16 @@ struct Basis {
     double proj[MAXN * MAXEQ];   // L2 projection n x EQ, column-major [i + n*q]
     double ev[MAXEQ * MAXN];     // basis at error points EQ x n, [q + EQ*i]
     double ex[MAXEQ], ew[MAXEQ];
+    // over-integration (quad Q > n, nodal_basis.hpp:34-65): Gauss weights of
+    // the Q-point rule, basis values vq(q, k) and weighted derivatives
+    // dq(q, i) = sum_k qw_q vq(q, k) D(k, i), both at [q + Q*k]
+    double qw[MAXQ], vq[MAXQ * MAXN], dq[MAXQ * MAXN];
 };
 
 // Mesh / physics scalars passed by value to every kernel.
 struct Geo {
     int nx, ny, n, cells, eq;
+    int nq;        // volume quadrature points per direction (n when collocated)
     int testcase;  // 0 diagonal, 1 wave, 2 axis, 3 seeded
     double hx, hy, lx, ly, gamma, safety, cosine, sine;
 };
@@ -61,6 +67,23 @@ hipError_t launch_element_jv(int n, const float *x, const double *vcache, const 
 hipError_t launch_element_jv(int n, const float *x, const double *vcache, const double *flux, double *y,
                              double inv_step, const Basis *b, const Geo &g, hipStream_t s);
 // Adjoint: z = J^T w, volume part and the face-flux adjoint.
+// Over-integrated volume terms (quad Q = g.nq > n): residual (volume + mass +
+// lifts), (u, v, H) cache at the Q^2 points ([cell][3][Q^2]), Jv / J^T w
+// element parts, the CFL rate and the point admissibility check.
+hipError_t launch_oi_element_residual(int n, const double *u, const double *old, const double *fq, double *r,
+                                      double tau, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_oi_volume_cache(int n, const double *u, double *vcache, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_oi_element_jv(int n, const double *x, const double *vcache, const double *fq, double *y,
+                                double inv_step, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_oi_element_jv(int n, const float *x, const double *vcache, const double *fq, float *y,
+                                double inv_step, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_oi_element_jv(int n, const float *x, const double *vcache, const double *fq, double *y,
+                                double inv_step, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_oi_element_jtv(int n, const double *w, const double *vcache, const double *gtraces, double *z,
+                                 double inv_step, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_oi_freeze_rate(int n, const double *u, const double *speeds, unsigned long long *rate_bits,
+                                 int *flag, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_oi_admissible(int n, const double *u, int *flag, const Basis *b, const Geo &g, hipStream_t s);
 hipError_t launch_adjoint_face(int n, const double *wtraces, const double *fcache, double *gtraces,
                                const Geo &g, hipStream_t s);
 hipError_t launch_element_jtv(int n, const double *w, const double *vcache, const double *gtraces, double *z,

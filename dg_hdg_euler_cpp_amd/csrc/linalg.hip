@@ -108,31 +108,13 @@ __global__ __launch_bounds__(BLOCK) void k_finish_sum(const double *__restrict__
 // With HDG = true the face terms are those of HybridOperator::assemble_cell
 // (hybrid.cpp:31-55): H_s = w_q ((+-1) A_phys + tau I) on every side with the
 // same shapes, added with sign +1 (face_jacobian.cpp:77-82).
-template <int N, typename T, bool HDG, bool TRANSPOSE = false>
-__global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ vc, const double *__restrict__ fc,
-                                                    T *__restrict__ blocks, double inv_step,
-                                                    const Basis *__restrict__ B, Geo g, int first) {
-    constexpr int P = N * N, M = 4 * P;
-    __shared__ double ax[P * 16], ay[P * 16];  // [node][v*4+w]
-    __shared__ double hs[4 * N * 16];          // [side][q][v*4+w]
-    const int cell = first + blockIdx.x;
+// Face terms of an element block (dg_operator.cpp:22-48 / face_jacobian.cpp
+// :34, hybrid.cpp:33-38 for the HDG variant): the 4x4 matrix per side and
+// face node, [side][q][v*4+w], computed by the whole block.
+template <int N, bool HDG>
+__device__ void face_block_terms(int cell, const double *__restrict__ fc, double *hs, const Basis *__restrict__ B,
+                                 const Geo &g) {
     const double gamma = g.gamma, gm = gamma - 1.;
-    for (int node = threadIdx.x; node < P; node += BLOCK) {
-        const double u = vc[static_cast<long long>(cell) * 3 * P + node];
-        const double v = vc[static_cast<long long>(cell) * 3 * P + P + node];
-        const double H = vc[static_cast<long long>(cell) * 3 * P + 2 * P + node];
-        const double s2 = u * u + v * v;
-        double *a = ax + node * 16, *b = ay + node * 16;
-        // physics.cpp:42-57, axis 0 then axis 1, row-major [v][w]
-        a[0] = 0; a[1] = 1.; a[2] = 0; a[3] = 0;
-        a[4] = .5 * (gamma - 3) * u * u + .5 * gm * v * v; a[5] = (3 - gamma) * u; a[6] = -gm * v; a[7] = gm;
-        a[8] = -u * v; a[9] = v; a[10] = u; a[11] = 0.;
-        a[12] = u * (.5 * gm * s2 - H); a[13] = H - gm * u * u; a[14] = -gm * u * v; a[15] = gamma * u;
-        b[0] = 0; b[1] = 0; b[2] = 1.; b[3] = 0;
-        b[4] = -u * v; b[5] = v; b[6] = u; b[7] = 0.;
-        b[8] = .5 * gm * u * u + .5 * (gamma - 3) * v * v; b[9] = -gm * u; b[10] = (3 - gamma) * v; b[11] = gm;
-        b[12] = v * (.5 * gm * s2 - H); b[13] = -gm * u * v; b[14] = H - gm * v * v; b[15] = gamma * v;
-    }
     const long long nodes = 2LL * g.cells * N;
     for (int t = threadIdx.x; t < 4 * N; t += BLOCK) {
         const int side = t / N, q = t % N;
@@ -173,6 +155,34 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ v
         }
         for (int k = 0; k < 16; ++k) hs[(side * N + q) * 16 + k] = a[k];
     }
+}
+
+template <int N, typename T, bool HDG, bool TRANSPOSE = false>
+__global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ vc, const double *__restrict__ fc,
+                                                    T *__restrict__ blocks, double inv_step,
+                                                    const Basis *__restrict__ B, Geo g, int first) {
+    constexpr int P = N * N, M = 4 * P;
+    __shared__ double ax[P * 16], ay[P * 16];  // [node][v*4+w]
+    __shared__ double hs[4 * N * 16];          // [side][q][v*4+w]
+    const int cell = first + blockIdx.x;
+    const double gamma = g.gamma, gm = gamma - 1.;
+    for (int node = threadIdx.x; node < P; node += BLOCK) {
+        const double u = vc[static_cast<long long>(cell) * 3 * P + node];
+        const double v = vc[static_cast<long long>(cell) * 3 * P + P + node];
+        const double H = vc[static_cast<long long>(cell) * 3 * P + 2 * P + node];
+        const double s2 = u * u + v * v;
+        double *a = ax + node * 16, *b = ay + node * 16;
+        // physics.cpp:42-57, axis 0 then axis 1, row-major [v][w]
+        a[0] = 0; a[1] = 1.; a[2] = 0; a[3] = 0;
+        a[4] = .5 * (gamma - 3) * u * u + .5 * gm * v * v; a[5] = (3 - gamma) * u; a[6] = -gm * v; a[7] = gm;
+        a[8] = -u * v; a[9] = v; a[10] = u; a[11] = 0.;
+        a[12] = u * (.5 * gm * s2 - H); a[13] = H - gm * u * u; a[14] = -gm * u * v; a[15] = gamma * u;
+        b[0] = 0; b[1] = 0; b[2] = 1.; b[3] = 0;
+        b[4] = -u * v; b[5] = v; b[6] = u; b[7] = 0.;
+        b[8] = .5 * gm * u * u + .5 * (gamma - 3) * v * v; b[9] = -gm * u; b[10] = (3 - gamma) * v; b[11] = gm;
+        b[12] = v * (.5 * gm * s2 - H); b[13] = -gm * u * v; b[14] = H - gm * v * v; b[15] = gamma * v;
+    }
+    face_block_terms<N, HDG>(cell, fc, hs, B, g);
     __syncthreads();
     T *out = blocks + static_cast<long long>(blockIdx.x) * M * M;
     const double sx = .5 * g.hy, sy = -.5 * g.hx;
@@ -194,6 +204,104 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ v
             if (i == jx) {
                 val += (B->left[jy] * hs[(2 * N + i) * 16 + vw]) * B->left[j];
                 val += (B->right[jy] * hs[(3 * N + i) * 16 + vw]) * B->right[j];
+            }
+            if (TRANSPOSE) out[static_cast<long long>(row) * M + col] = static_cast<T>(val);
+            else out[static_cast<long long>(col) * M + row] = static_cast<T>(val);
+        }
+    }
+}
+
+// Entry (v, w) of the cached flux Jacobian dF_axis/dU from (u, v, H)
+// (physics.cpp:42-57), row-major [v][w].
+__device__ __forceinline__ double jacobian_entry(double u, double v, double H, int axis, int r, int c, double gamma) {
+    const double gm = gamma - 1., s2 = u * u + v * v;
+    if (axis == 0) {
+        switch (r * 4 + c) {
+        case 1: return 1.;
+        case 4: return .5 * (gamma - 3) * u * u + .5 * gm * v * v;
+        case 5: return (3 - gamma) * u;
+        case 6: return -gm * v;
+        case 7: return gm;
+        case 8: return -u * v;
+        case 9: return v;
+        case 10: return u;
+        case 12: return u * (.5 * gm * s2 - H);
+        case 13: return H - gm * u * u;
+        case 14: return -gm * u * v;
+        case 15: return gamma * u;
+        default: return 0.;
+        }
+    }
+    switch (r * 4 + c) {
+    case 2: return 1.;
+    case 4: return -u * v;
+    case 5: return v;
+    case 6: return u;
+    case 8: return .5 * gm * u * u + .5 * (gamma - 3) * v * v;
+    case 9: return -gm * u;
+    case 10: return (3 - gamma) * v;
+    case 11: return gm;
+    case 12: return v * (.5 * gm * s2 - H);
+    case 13: return -gm * u * v;
+    case 14: return H - gm * v * v;
+    case 15: return gamma * v;
+    default: return 0.;
+    }
+}
+
+// Element block with an over-integrated volume part (volume_jacobian.cpp
+// :90-116 + dg_operator.cpp:22-48): per (v, w) pair, the line tables
+// lx[qb][i + n k] = -sum_qa d(qa, i) (A_x(qa, qb)[v][w] vq(qa, k)) and
+// ly[qb][i + n k] = -sum_qa (vq(qa, i) qw_qa) (A_y(qa, qb)[v][w] vq(qa, k)),
+// then entry (v,i,j; w,k,l) = sum_qb cx(qb, j, l) lx + cy(qb, j, l) ly with
+// cx = hy/2 qw_qb vq(qb, j) vq(qb, l), cy = hx/2 d(qb, j) vq(qb, l); the mass
+// diagonal and the face terms follow as in k_assemble.
+template <int N, typename T, bool HDG, bool TRANSPOSE = false>
+__global__ __launch_bounds__(BLOCK) void k_assemble_oi(const double *__restrict__ vc, const double *__restrict__ fc,
+                                                       T *__restrict__ blocks, double inv_step,
+                                                       const Basis *__restrict__ B, Geo g, int first) {
+    constexpr int P = N * N, M = 4 * P;
+    __shared__ double uvh[3 * MAXQ * MAXQ], hs[4 * N * 16], lx[MAXQ * P], ly[MAXQ * P];
+    const int cell = first + blockIdx.x, Q = g.nq, QQ = Q * Q;
+    for (int t = threadIdx.x; t < 3 * QQ; t += BLOCK) uvh[t] = vc[static_cast<long long>(cell) * 3 * QQ + t];
+    face_block_terms<N, HDG>(cell, fc, hs, B, g);
+    T *out = blocks + static_cast<long long>(blockIdx.x) * M * M;
+    for (int vw = 0; vw < 16; ++vw) {
+        const int v = vw / 4, w = vw % 4;
+        __syncthreads();
+        for (int t = threadIdx.x; t < Q * P; t += BLOCK) {
+            const int qb = t / P, r = t % P, i = r % N, k = r / N;
+            double sx = 0, sy = 0;
+            for (int qa = 0; qa < Q; ++qa) {
+                const int q = qb * Q + qa;
+                const double uu = uvh[q], vv = uvh[QQ + q], H = uvh[2 * QQ + q];
+                const double wx = jacobian_entry(uu, vv, H, 0, v, w, g.gamma) * B->vq[qa + Q * k];
+                const double wy = jacobian_entry(uu, vv, H, 1, v, w, g.gamma) * B->vq[qa + Q * k];
+                sx += B->dq[qa + Q * i] * wx;
+                sy += (B->vq[qa + Q * i] * B->qw[qa]) * wy;
+            }
+            lx[t] = -sx;
+            ly[t] = -sy;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < P * P; t += BLOCK) {
+            const int rnode = t % P, cnode = t / P;
+            const int i = rnode % N, j = rnode / N, kx = cnode % N, ly_ = cnode / N;
+            double val = 0;
+            for (int qb = 0; qb < Q; ++qb) {
+                const double cx = .5 * g.hy * B->qw[qb] * B->vq[qb + Q * j] * B->vq[qb + Q * ly_];
+                const double cy = .5 * g.hx * B->dq[qb + Q * j] * B->vq[qb + Q * ly_];
+                val += cx * lx[qb * P + i + N * kx] + cy * ly[qb * P + i + N * kx];
+            }
+            const int row = v * P + rnode, col = w * P + cnode;
+            if (row == col) val += B->mass[rnode] * inv_step;
+            if (j == ly_) {
+                val += (B->left[kx] * hs[(0 * N + j) * 16 + vw]) * B->left[i];
+                val += (B->right[kx] * hs[(1 * N + j) * 16 + vw]) * B->right[i];
+            }
+            if (i == kx) {
+                val += (B->left[ly_] * hs[(2 * N + i) * 16 + vw]) * B->left[j];
+                val += (B->right[ly_] * hs[(3 * N + i) * 16 + vw]) * B->right[j];
             }
             if (TRANSPOSE) out[static_cast<long long>(row) * M + col] = static_cast<T>(val);
             else out[static_cast<long long>(col) * M + row] = static_cast<T>(val);
@@ -901,26 +1009,30 @@ hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fca
                                   double inv_step, const Basis *b, const Geo &g, int first, int count,
                                   hipStream_t s) {
     DGH_DISPATCH_N(n, {
-        k_assemble<N, double, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
+        if (g.nq > N) k_assemble_oi<N, double, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
+        else k_assemble<N, double, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
     })
 }
 hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fcache, float *blocks,
                                   double inv_step, const Basis *b, const Geo &g, int first, int count,
                                   hipStream_t s) {
     DGH_DISPATCH_N(n, {
-        k_assemble<N, float, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
+        if (g.nq > N) k_assemble_oi<N, float, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
+        else k_assemble<N, float, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
     })
 }
 hipError_t launch_assemble_blocks_transposed(int n, const double *vcache, const double *fcache, double *blocks,
                                              double inv_step, const Basis *b, const Geo &g, hipStream_t s) {
     DGH_DISPATCH_N(n, {
-        k_assemble<N, double, false, true><<<g.cells, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, 0);
+        if (g.nq > N) k_assemble_oi<N, double, false, true><<<g.cells, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, 0);
+        else k_assemble<N, double, false, true><<<g.cells, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, 0);
     })
 }
 hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache, double *blocks, double inv_step,
                                const Basis *b, const Geo &g, int first, int count, hipStream_t s) {
     DGH_DISPATCH_N(n, {
-        k_assemble<N, double, true><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
+        if (g.nq > N) k_assemble_oi<N, double, true><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
+        else k_assemble<N, double, true><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
     })
 }
 // Panel width and workgroup size of k_lu. A wider panel halves the passes

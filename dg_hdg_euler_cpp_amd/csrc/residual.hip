@@ -526,6 +526,319 @@ __global__ __launch_bounds__(BLOCK) void k_element_jtv(const double *__restrict_
     }
 }
 
+// --------------------------------------------------------- over-integration
+// Volume terms on the Q-point rule (quad Q > n): interpolation to the points
+// (tensor_product.cpp:11-22), fluxes and their linearisation there, and the
+// weak form back to the nodes (tensor_product.cpp:54-72); the face terms are
+// those of the collocated kernels. One element per block; per-point work is
+// spread over the block's threads with the reference's summation orders.
+template <int N> struct OI {
+    static constexpr int P = N * N, M = 4 * P, ITEMS = (M + BLOCK - 1) / BLOCK;
+};
+
+// nodal [4][n^2] -> points [4][Q^2], through tmp [4][Q n]
+template <int N>
+__device__ void oi_interpolate(const double *nodal, double *tmp, double *pts, const Basis *__restrict__ B, int Q) {
+    for (int t = threadIdx.x; t < 4 * Q * N; t += BLOCK) {
+        const int v = t / (Q * N), r = t % (Q * N), q = r % Q, j = r / Q;
+        const double *in = nodal + v * N * N;
+        double s = 0;
+        for (int k = 0; k < N; ++k) s += in[k + N * j] * B->vq[q + Q * k];
+        tmp[v * Q * N + q + Q * j] = s;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 4 * Q * Q; t += BLOCK) {
+        const int v = t / (Q * Q), r = t % (Q * Q), qa = r % Q, qb = r / Q;
+        double s = 0;
+        for (int k = 0; k < N; ++k) s += B->vq[qb + Q * k] * tmp[v * Q * N + qa + Q * k];
+        pts[v * Q * Q + qa + Q * qb] = s;
+    }
+    __syncthreads();
+}
+
+// first pass of the weak form: scr_x[v][i + n qb] = sum_k X[v][k + Q qb] d(k, i),
+// scr_y[v][i + n qb] = sum_k Y[v][k + Q qb] (qw_k vq(k, i))
+template <int N>
+__device__ void oi_weak_pass1(const double *X, const double *Y, double *scr_x, double *scr_y,
+                              const Basis *__restrict__ B, int Q) {
+    for (int t = threadIdx.x; t < 4 * N * Q; t += BLOCK) {
+        const int v = t / (N * Q), r = t % (N * Q), i = r % N, qb = r / N;
+        const double *x = X + v * Q * Q, *y = Y + v * Q * Q;
+        double sx = 0, sy = 0;
+        for (int k = 0; k < Q; ++k) sx += x[k + Q * qb] * B->dq[k + Q * i];
+        for (int k = 0; k < Q; ++k) sy += y[k + Q * qb] * (B->qw[k] * B->vq[k + Q * i]);
+        scr_x[v * N * Q + i + N * qb] = sx;
+        scr_y[v * N * Q + i + N * qb] = sy;
+    }
+    __syncthreads();
+}
+
+// second pass for node (i, j) of variable v: res -= sx * sum_k (qw_k vq(k, j)) scr_x,
+// then res -= sy * sum_k d(k, j) scr_y
+template <int N>
+__device__ __forceinline__ double oi_weak_pass2(const double *scr_x, const double *scr_y, int v, int i, int j,
+                                                double res, const Basis *__restrict__ B, const Geo &g, int Q) {
+    double sx = 0, sy = 0;
+    for (int k = 0; k < Q; ++k) sx += (B->qw[k] * B->vq[k + Q * j]) * scr_x[v * N * Q + i + N * k];
+    res -= (.5 * g.hy) * sx;
+    for (int k = 0; k < Q; ++k) sy += B->dq[k + Q * j] * scr_y[v * N * Q + i + N * k];
+    res -= (.5 * g.hx) * sy;
+    return res;
+}
+
+#define DGH_OI_SHARED                                                                            \
+    __shared__ double nodal[OI<N>::M], tmp[4 * MAXQ * N], pts[4 * MAXQ * MAXQ];                   \
+    __shared__ double fxs[4 * MAXQ * MAXQ], fys[4 * MAXQ * MAXQ], scr_x[4 * N * MAXQ], scr_y[4 * N * MAXQ];
+
+// discretization.cpp:111-140 with tensor_product.cpp:54-72
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_oi_element_residual(const double *__restrict__ u,
+                                                               const double *__restrict__ old,
+                                                               const double *__restrict__ fq, double *__restrict__ r,
+                                                               double tau, const Basis *__restrict__ B, Geo g) {
+    using O = OI<N>;
+    DGH_OI_SHARED
+    const int cell = blockIdx.x, Q = g.nq, QQ = Q * Q;
+    const long long base = static_cast<long long>(cell) * O::M;
+    for (int t = threadIdx.x; t < O::M; t += BLOCK) nodal[t] = u[base + t];
+    __syncthreads();
+    double res[O::ITEMS];
+#pragma unroll
+    for (int it = 0; it < O::ITEMS; ++it) {
+        const int item = threadIdx.x + it * BLOCK;
+        if (item < O::M) res[it] = B->mass[item % O::P] * (nodal[item] - old[base + item]) / tau;
+    }
+    oi_interpolate<N>(nodal, tmp, pts, B, Q);
+    for (int q = threadIdx.x; q < QQ; q += BLOCK) {
+        double st[4], fx[4], fy[4];
+        for (int v = 0; v < 4; ++v) st[v] = pts[v * QQ + q];
+        const Prim p = primitive(st, g.gamma);
+        flux(st, p, 0, fx);
+        flux(st, p, 1, fy);
+        for (int v = 0; v < 4; ++v) {
+            fxs[v * QQ + q] = fx[v];
+            fys[v * QQ + q] = fy[v];
+        }
+    }
+    __syncthreads();
+    oi_weak_pass1<N>(fxs, fys, scr_x, scr_y, B, Q);
+#pragma unroll
+    for (int it = 0; it < O::ITEMS; ++it) {
+        const int item = threadIdx.x + it * BLOCK;
+        if (item >= O::M) continue;
+        const int v = item / O::P, node = item % O::P, i = node % N, j = node / N;
+        double acc = oi_weak_pass2<N>(scr_x, scr_y, v, i, j, res[it], B, g, Q);
+        acc = lift_faces<N>(fq, B, cell, i, j, v, g, acc);
+        r[base + item] = acc;
+    }
+}
+
+// volume_jacobian.cpp:13-20: (u, v, H) at the Q^2 points, [cell][3][Q^2]
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_oi_volume_cache(const double *__restrict__ u, double *__restrict__ vc,
+                                                           const Basis *__restrict__ B, Geo g) {
+    using O = OI<N>;
+    __shared__ double nodal[O::M], tmp[4 * MAXQ * N], pts[4 * MAXQ * MAXQ];
+    const int cell = blockIdx.x, Q = g.nq, QQ = Q * Q;
+    for (int t = threadIdx.x; t < O::M; t += BLOCK) nodal[t] = u[static_cast<long long>(cell) * O::M + t];
+    __syncthreads();
+    oi_interpolate<N>(nodal, tmp, pts, B, Q);
+    for (int q = threadIdx.x; q < QQ; q += BLOCK) {
+        double st[4];
+        for (int v = 0; v < 4; ++v) st[v] = pts[v * QQ + q];
+        const Prim p = primitive(st, g.gamma);
+        const long long vb = static_cast<long long>(cell) * 3 * QQ;
+        vc[vb + q] = p.u;
+        vc[vb + QQ + q] = p.v;
+        vc[vb + 2 * QQ + q] = p.H;
+    }
+}
+
+// volume_jacobian.cpp:43-71 on the points, plus the face lifts of k_element_jv
+template <int N, typename TX, typename TY>
+__global__ __launch_bounds__(BLOCK) void k_oi_element_jv(const TX *__restrict__ x, const double *__restrict__ vc,
+                                                         const double *__restrict__ fq, TY *__restrict__ y,
+                                                         double inv_step, const Basis *__restrict__ B, Geo g) {
+    using O = OI<N>;
+    DGH_OI_SHARED
+    const int cell = blockIdx.x, Q = g.nq, QQ = Q * Q;
+    const long long base = static_cast<long long>(cell) * O::M;
+    for (int t = threadIdx.x; t < O::M; t += BLOCK) nodal[t] = static_cast<double>(x[base + t]);
+    __syncthreads();
+    double res[O::ITEMS];
+#pragma unroll
+    for (int it = 0; it < O::ITEMS; ++it) {
+        const int item = threadIdx.x + it * BLOCK;
+        if (item < O::M) res[it] = B->mass[item % O::P] * nodal[item] * inv_step;
+    }
+    oi_interpolate<N>(nodal, tmp, pts, B, Q);
+    const double gm = g.gamma - 1.;
+    for (int q = threadIdx.x; q < QQ; q += BLOCK) {
+        double d[4], fx[4], fy[4];
+        for (int v = 0; v < 4; ++v) d[v] = pts[v * QQ + q];
+        const long long vb = static_cast<long long>(cell) * 3 * QQ;
+        normal_action(vc[vb + q], vc[vb + QQ + q], vc[vb + 2 * QQ + q], d, 0, gm, fx);
+        normal_action(vc[vb + q], vc[vb + QQ + q], vc[vb + 2 * QQ + q], d, 1, gm, fy);
+        for (int v = 0; v < 4; ++v) {
+            fxs[v * QQ + q] = fx[v];
+            fys[v * QQ + q] = fy[v];
+        }
+    }
+    __syncthreads();
+    oi_weak_pass1<N>(fxs, fys, scr_x, scr_y, B, Q);
+#pragma unroll
+    for (int it = 0; it < O::ITEMS; ++it) {
+        const int item = threadIdx.x + it * BLOCK;
+        if (item >= O::M) continue;
+        const int v = item / O::P, node = item % O::P, i = node % N, j = node / N;
+        double acc = oi_weak_pass2<N>(scr_x, scr_y, v, i, j, res[it], B, g, Q);
+        acc = lift_faces<N>(fq, B, cell, i, j, v, g, acc);
+        y[base + item] = static_cast<TY>(acc);
+    }
+}
+
+// Transpose of k_oi_element_jv's volume part: with the weak form
+// r(i,j) -= sx sum d(k,i) qw_b vq(b,j) X(k,b) + sy sum qw_k vq(k,i) d(b,j) Y(k,b),
+// w maps to GX(k,b) = sx qw_b sum_j vq(b,j) sum_i d(k,i) w(i,j) and
+// GY(k,b) = sy qw_k sum_j d(b,j) sum_i vq(k,i) w(i,j); then A^T per point and
+// the transposed interpolation back to the nodes. Face adjoint as k_element_jtv.
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_oi_element_jtv(const double *__restrict__ w, const double *__restrict__ vc,
+                                                          const double *__restrict__ gtr, double *__restrict__ z,
+                                                          double inv_step, const Basis *__restrict__ B, Geo g) {
+    using O = OI<N>;
+    DGH_OI_SHARED
+    const int cell = blockIdx.x, Q = g.nq, QQ = Q * Q;
+    const long long base = static_cast<long long>(cell) * O::M;
+    for (int t = threadIdx.x; t < O::M; t += BLOCK) nodal[t] = w[base + t];
+    __syncthreads();
+    // T1[v][k + Q j] = sum_i d(k, i) w(i, j) (in scr_x), T2 = sum_i vq(k, i) w(i, j) (in scr_y)
+    for (int t = threadIdx.x; t < 4 * Q * N; t += BLOCK) {
+        const int v = t / (Q * N), r = t % (Q * N), k = r % Q, j = r / Q;
+        const double *in = nodal + v * O::P;
+        double a = 0, b = 0;
+        for (int i = 0; i < N; ++i) {
+            a += B->dq[k + Q * i] * in[i + N * j];
+            b += B->vq[k + Q * i] * in[i + N * j];
+        }
+        scr_x[v * Q * N + k + Q * j] = a;
+        scr_y[v * Q * N + k + Q * j] = b;
+    }
+    __syncthreads();
+    const double sx = .5 * g.hy, sy = .5 * g.hx, gm = g.gamma - 1.;
+    for (int t = threadIdx.x; t < 4 * QQ; t += BLOCK) {
+        const int v = t / QQ, q = t % QQ, k = q % Q, b = q / Q;
+        double a = 0, c = 0;
+        for (int j = 0; j < N; ++j) {
+            a += B->vq[b + Q * j] * scr_x[v * Q * N + k + Q * j];
+            c += B->dq[b + Q * j] * scr_y[v * Q * N + k + Q * j];
+        }
+        fxs[v * QQ + q] = sx * B->qw[b] * a;
+        fys[v * QQ + q] = sy * B->qw[k] * c;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < QQ; q += BLOCK) {
+        double gx[4], gy[4], tx[4], ty[4];
+        for (int v = 0; v < 4; ++v) {
+            gx[v] = fxs[v * QQ + q];
+            gy[v] = fys[v * QQ + q];
+        }
+        const long long vb = static_cast<long long>(cell) * 3 * QQ;
+        normal_action_t(vc[vb + q], vc[vb + QQ + q], vc[vb + 2 * QQ + q], gx, 0, gm, g.gamma, tx);
+        normal_action_t(vc[vb + q], vc[vb + QQ + q], vc[vb + 2 * QQ + q], gy, 1, gm, g.gamma, ty);
+        for (int v = 0; v < 4; ++v) pts[v * QQ + q] = tx[v] + ty[v];
+    }
+    __syncthreads();
+    // transposed interpolation: T3[v][k + n b] = sum_a vq(a, k) P[v][a + Q b]
+    for (int t = threadIdx.x; t < 4 * N * Q; t += BLOCK) {
+        const int v = t / (N * Q), r = t % (N * Q), k = r % N, b = r / N;
+        double s = 0;
+        for (int a = 0; a < Q; ++a) s += B->vq[a + Q * k] * pts[v * QQ + a + Q * b];
+        tmp[v * N * Q + k + N * b] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < O::ITEMS; ++it) {
+        const int item = threadIdx.x + it * BLOCK;
+        if (item >= O::M) continue;
+        const int v = item / O::P, node = item % O::P, i = node % N, j = node / N;
+        double vol = 0;
+        for (int b = 0; b < Q; ++b) vol += B->vq[b + Q * j] * tmp[v * N * Q + i + N * b];
+        double acc = B->mass[node] * nodal[item] * inv_step - vol;
+        const long long f0 = neighbor(cell, 0, g.nx, g.ny), f1 = cell;
+        const long long f2 = g.cells + neighbor(cell, 2, g.nx, g.ny), f3 = g.cells + cell;
+        acc += B->right[i] * gtr[f1 * 8 * N + v * N + j];
+        acc += B->left[i] * gtr[f0 * 8 * N + 4 * N + v * N + j];
+        acc += B->right[j] * gtr[f3 * 8 * N + v * N + i];
+        acc += B->left[j] * gtr[f2 * 8 * N + 4 * N + v * N + i];
+        z[base + item] = acc;
+    }
+}
+
+// discretization.cpp:163-195 with the volume speeds at the Q^2 points
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_oi_freeze_rate(const double *__restrict__ u,
+                                                          const double *__restrict__ speeds,
+                                                          unsigned long long *rate_bits, int *flag,
+                                                          const Basis *__restrict__ B, Geo g) {
+    using O = OI<N>;
+    __shared__ double nodal[O::M], tmp[4 * MAXQ * N], pts[4 * MAXQ * MAXQ];
+    __shared__ unsigned long long alpha[2];
+    const int cell = blockIdx.x, Q = g.nq, QQ = Q * Q;
+    for (int t = threadIdx.x; t < O::M; t += BLOCK) nodal[t] = u[static_cast<long long>(cell) * O::M + t];
+    if (threadIdx.x < 2) alpha[threadIdx.x] = 0;
+    __syncthreads();
+    oi_interpolate<N>(nodal, tmp, pts, B, Q);
+    for (int q = threadIdx.x; q < QQ; q += BLOCK) {
+        double st[4];
+        for (int v = 0; v < 4; ++v) st[v] = pts[v * QQ + q];
+        const Prim p = primitive(st, g.gamma);
+        if (!(st[0] > 0 && p.p > 0)) atomicOr(flag, 2);
+        const double sound = sqrt(g.gamma * p.p * p.inv);
+        atomicMax(&alpha[0], static_cast<unsigned long long>(__double_as_longlong(fabs(p.u) + sound)));
+        atomicMax(&alpha[1], static_cast<unsigned long long>(__double_as_longlong(fabs(p.v) + sound)));
+    }
+    for (int t = threadIdx.x; t < 4 * N; t += BLOCK) {
+        const int side = t / N, q = t % N;
+        const int face = side == 0 ? neighbor(cell, 0, g.nx, g.ny)
+                       : side == 1 ? cell
+                       : side == 2 ? g.cells + neighbor(cell, 2, g.nx, g.ny)
+                                   : g.cells + cell;
+        const double a = speeds[2LL * (static_cast<long long>(face) * N + q)];
+        const double b = speeds[2LL * (static_cast<long long>(face) * N + q) + 1];
+        atomicMax(&alpha[side / 2], static_cast<unsigned long long>(__double_as_longlong(fmax(a, b))));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double ax = __longlong_as_double(static_cast<long long>(alpha[0]));
+        const double ay = __longlong_as_double(static_cast<long long>(alpha[1]));
+        atomicMax(rate_bits, static_cast<unsigned long long>(__double_as_longlong(ax / g.hx + ay / g.hy)));
+    }
+}
+
+// discretization.cpp:220-233: the quadrature points of an over-integrated rule
+// must be physical too (the nodes and faces are checked by k_admissible)
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_oi_admissible(const double *__restrict__ u, int *flag,
+                                                         const Basis *__restrict__ B, Geo g) {
+    using O = OI<N>;
+    __shared__ double nodal[O::M], tmp[4 * MAXQ * N], pts[4 * MAXQ * MAXQ];
+    const int cell = blockIdx.x, Q = g.nq, QQ = Q * Q;
+    for (int t = threadIdx.x; t < O::M; t += BLOCK) nodal[t] = u[static_cast<long long>(cell) * O::M + t];
+    __syncthreads();
+    oi_interpolate<N>(nodal, tmp, pts, B, Q);
+    for (int q = threadIdx.x; q < QQ; q += BLOCK) {
+        double st[4];
+        bool ok = true;
+        for (int v = 0; v < 4; ++v) {
+            st[v] = pts[v * QQ + q];
+            ok = ok && isfinite(st[v]);
+        }
+        ok = ok && st[0] > 1e-13 && pressure(st, g.gamma) > 1e-13;
+        if (!ok) atomicAnd(flag, 0);
+    }
+}
+
 // ------------------------------------------------------------ initial data
 // example.cpp:20-39 (rotated simple wave) and :12-16 (entropy wave), t = 0.
 __device__ void exact_state(double x, double y, const Geo &g, double out[4]) {
@@ -850,6 +1163,41 @@ hipError_t launch_element_jv(int n, const float *x, const double *vcache, const 
 hipError_t launch_element_jv(int n, const float *x, const double *vcache, const double *fq, double *y,
                              double inv_step, const Basis *b, const Geo &g, hipStream_t s) {
     return element_jv_any(n, x, vcache, fq, y, inv_step, b, g, s);
+}
+hipError_t launch_oi_element_residual(int n, const double *u, const double *old, const double *fq, double *r,
+                                      double tau, const Basis *b, const Geo &g, hipStream_t s) {
+    DGH_DISPATCH_N(n, { k_oi_element_residual<N><<<g.cells, BLOCK, 0, s>>>(u, old, fq, r, tau, b, g); })
+}
+hipError_t launch_oi_volume_cache(int n, const double *u, double *vcache, const Basis *b, const Geo &g, hipStream_t s) {
+    DGH_DISPATCH_N(n, { k_oi_volume_cache<N><<<g.cells, BLOCK, 0, s>>>(u, vcache, b, g); })
+}
+template <typename TX, typename TY>
+static hipError_t oi_jv_any(int n, const TX *x, const double *vcache, const double *fq, TY *y, double inv_step,
+                            const Basis *b, const Geo &g, hipStream_t s) {
+    DGH_DISPATCH_N(n, { k_oi_element_jv<N, TX, TY><<<g.cells, BLOCK, 0, s>>>(x, vcache, fq, y, inv_step, b, g); })
+}
+hipError_t launch_oi_element_jv(int n, const double *x, const double *vcache, const double *fq, double *y,
+                                double inv_step, const Basis *b, const Geo &g, hipStream_t s) {
+    return oi_jv_any(n, x, vcache, fq, y, inv_step, b, g, s);
+}
+hipError_t launch_oi_element_jv(int n, const float *x, const double *vcache, const double *fq, float *y,
+                                double inv_step, const Basis *b, const Geo &g, hipStream_t s) {
+    return oi_jv_any(n, x, vcache, fq, y, inv_step, b, g, s);
+}
+hipError_t launch_oi_element_jv(int n, const float *x, const double *vcache, const double *fq, double *y,
+                                double inv_step, const Basis *b, const Geo &g, hipStream_t s) {
+    return oi_jv_any(n, x, vcache, fq, y, inv_step, b, g, s);
+}
+hipError_t launch_oi_element_jtv(int n, const double *w, const double *vcache, const double *gtraces, double *z,
+                                 double inv_step, const Basis *b, const Geo &g, hipStream_t s) {
+    DGH_DISPATCH_N(n, { k_oi_element_jtv<N><<<g.cells, BLOCK, 0, s>>>(w, vcache, gtraces, z, inv_step, b, g); })
+}
+hipError_t launch_oi_freeze_rate(int n, const double *u, const double *speeds, unsigned long long *rate_bits,
+                                 int *flag, const Basis *b, const Geo &g, hipStream_t s) {
+    DGH_DISPATCH_N(n, { k_oi_freeze_rate<N><<<g.cells, BLOCK, 0, s>>>(u, speeds, rate_bits, flag, b, g); })
+}
+hipError_t launch_oi_admissible(int n, const double *u, int *flag, const Basis *b, const Geo &g, hipStream_t s) {
+    DGH_DISPATCH_N(n, { k_oi_admissible<N><<<g.cells, BLOCK, 0, s>>>(u, flag, b, g); })
 }
 hipError_t launch_adjoint_face(int n, const double *wtraces, const double *fcache, double *gtraces, const Geo &g,
                                hipStream_t s) {
