@@ -130,7 +130,8 @@ int main(int argc, char **argv) {
             summary << name << ',' << config.nx << ',' << config.ny << ',' << config.degree << ','
                     << (config.quadrature ? config.quadrature : config.degree + 1) << ',' << config.cfl << ','
                     << integrator_name(config.integrator) << ',' << (config.direct ? "direct" : "gmres") << ','
-                    << (config.direct ? "none" : "block-jacobi") << ',' << (config.direct ? "none" : "natural") << ','
+                    << (config.direct ? "none" : config.precond ? "block-ilu0" : "block-jacobi") << ','
+                    << (config.direct ? "none" : config.ordering == 2 ? "redblack" : "natural") << ','
                     << config.nonlinear_tolerance << ',' << config.linear_tolerance << ',' << config.safety << ','
                     << config.newton_steps << ',' << config.warmup_updates << ',' << stats.steps << ','
                     << stats.final_time << ',' << stats.newton_iterations << ',' << stats.krylov_iterations << ','

@@ -211,6 +211,11 @@ struct dgh_ctx {
     // matrix, rotations and rotated right-hand side live on the device and
     // three scalars per step come back through pinned, mapped host memory.
     std::vector<hipGraphExec_t> step_graphs;
+    // Block ILU(0) in red-black order (cfg.precond == 1): compact red factors
+    // in factors[0, cells/2), black ones in factors[cells/2, cells); the red
+    // couplings S [red cell][side][4n x 4n]; apply work vectors
+    DevBuf<double> ilu_S, ilu_z, ilu_w;
+    bool ilu_ready = false;
     DevBuf<double> gH, gcs, gsn, gg, gnext;
     double *gout_host = nullptr, *gout_dev = nullptr;
     int ck_stages = 0;
@@ -418,6 +423,7 @@ struct dgh_ctx {
         HIPCHECK(launch_face_cache(n, tr_u.p, speeds.p, fcache.p, basis.p, geo, stream));
         refreshed = true;
         bj_ready = false;
+        ilu_ready = false;
         adj_bj_ready = false;
         bj32_ready = false;
         hdg_ready = false;
@@ -452,6 +458,71 @@ struct dgh_ctx {
         timed("lu", [&] { HIPCHECK(launch_lu(m, factors.p, rowsrc.p, cells, flags.p + 2, stream)); });
         require(read_flag(2) == 0, "Singular block Jacobi factor");
         bj_ready = true;
+        ilu_ready = false;
+    }
+    // Block ILU(0) of the element graph in red-black order (block_ilu.cpp:48-61
+    // with red cells first): red LU, red couplings, black diagonal update,
+    // black LU. Shares the factor storage with bj_refresh.
+    void ilu_refresh() {
+        require(refreshed, "Refresh the volume Jacobian before assembling a block");
+        require(cfg.nx % 2 == 0 && cfg.ny % 2 == 0 && cfg.nx >= 4 && cfg.ny >= 4,
+                "Block ILU(0) on the device runs in red-black order: nx and ny must be even and >= 4");
+        const int half = cells / 2;
+        const size_t mm = static_cast<size_t>(m) * m;
+        factors.alloc(static_cast<size_t>(cells) * mm, bytes);
+        rowsrc.alloc(static_cast<size_t>(cells) * m, bytes);
+        ilu_S.alloc(static_cast<size_t>(half) * 4 * 16 * n * n, bytes);
+        if (!ilu_z.p) {
+            ilu_z.alloc(D, bytes);
+            ilu_w.alloc(D, bytes);
+            HIPCHECK(hipMemsetAsync(ilu_z.p, 0, D * sizeof(double), stream));
+        }
+        bj_ready = false;
+        adj_bj_ready = false;
+        ilu_ready = false;
+        set_flag(2, 0);
+        timed("assemble", [&] {
+            HIPCHECK(launch_assemble_colour(n, vcache.p, fcache.p, factors.p, inv_step, basis.p, geo, 0, stream));
+            HIPCHECK(launch_assemble_colour(n, vcache.p, fcache.p, factors.p + half * mm, inv_step, basis.p, geo, 1,
+                                            stream));
+        });
+        timed("lu", [&] { HIPCHECK(launch_lu(m, factors.p, rowsrc.p, half, flags.p + 2, stream)); });
+        timed("ilu_coupling", [&] { HIPCHECK(launch_ilu_coupling(n, factors.p, rowsrc.p, ilu_S.p, basis.p, geo, stream)); });
+        timed("ilu_update", [&] {
+            HIPCHECK(launch_ilu_update(n, factors.p + half * mm, ilu_S.p, fcache.p, basis.p, geo, stream));
+        });
+        timed("lu", [&] {
+            HIPCHECK(launch_lu(m, factors.p + half * mm, rowsrc.p + static_cast<size_t>(half) * m, half, flags.p + 2,
+                               stream));
+        });
+        require(read_flag(2) == 0, "Singular or nonfinite LU factors");
+        ilu_ready = true;
+    }
+    // block_ilu.cpp:62-84 in red-black order: z_r = A_rr^-1 r_r;
+    // w_b = r_b - sum A_br z_r; z_b = A_bb'^-1 w_b; z_r = A_rr^-1 (r_r - sum A_rb z_b)
+    void ilu_apply(const double *r, double *z) {
+        require(ilu_ready, "Refresh the block ILU(0) preconditioner first");
+        require(r != z, "The block ILU(0) apply needs distinct input and output vectors");
+        const int half = cells / 2;
+        const size_t mm = static_cast<size_t>(m) * m;
+        timed("ilu_apply", [&] {
+            HIPCHECK(launch_bj_apply_colour(m, factors.p, rowsrc.p, r, ilu_z.p, half, 0, cfg.nx, stream));
+            HIPCHECK(launch_restrict(n, ilu_z.p, tr_x.p, basis.p, geo, stream));
+            HIPCHECK(launch_ilu_offdiag(n, 1, tr_x.p, fcache.p, r, ilu_w.p, basis.p, geo, stream));
+            HIPCHECK(launch_bj_apply_colour(m, factors.p + half * mm, rowsrc.p + static_cast<size_t>(half) * m,
+                                            ilu_w.p, z, half, 1, cfg.nx, stream));
+            HIPCHECK(launch_restrict(n, z, tr_x.p, basis.p, geo, stream));
+            HIPCHECK(launch_ilu_offdiag(n, 0, tr_x.p, fcache.p, r, ilu_w.p, basis.p, geo, stream));
+            HIPCHECK(launch_bj_apply_colour(m, factors.p, rowsrc.p, ilu_w.p, z, half, 0, cfg.nx, stream));
+        });
+    }
+    void precond_refresh() {
+        if (cfg.precond == 1) ilu_refresh();
+        else bj_refresh();
+    }
+    void precond_apply(const double *r, double *z) {
+        if (cfg.precond == 1) ilu_apply(r, z);
+        else bj_apply(r, z);
     }
     void bj_apply(const double *r, double *z) {
         require(bj_ready, "Refresh the block Jacobi preconditioner first");
@@ -473,6 +544,7 @@ struct dgh_ctx {
         timed("lu", [&] { HIPCHECK(launch_lu(m, factors.p, rowsrc.p, cells, flags.p + 2, stream)); });
         require(read_flag(2) == 0, "Singular block Jacobi factor");
         adj_bj_ready = true;
+        ilu_ready = false;
     }
     void bj_apply_transposed(const double *r, double *z) {
         require(adj_bj_ready, "Refresh the transposed block Jacobi preconditioner first");
@@ -486,7 +558,7 @@ struct dgh_ctx {
     using Op = std::function<void(const double *, double *)>;
     Linear gmres(const double *rhs, double *x, int limit, double tolerance) {
         return gmres_generic(D, nullptr, [&](const double *a, double *b) { dg_apply(a, b); },
-                             [&](const double *a, double *b) { bj_apply(a, b); }, rhs, x, limit, tolerance);
+                             [&](const double *a, double *b) { precond_apply(a, b); }, rhs, x, limit, tolerance);
     }
     // GMRES on J^T lambda = rhs, right-preconditioned by the transposed block
     // Jacobi; the residual of J^T lambda is a weak-form vector like the
@@ -1071,7 +1143,7 @@ struct dgh_ctx {
                     dg_refresh(state.p);
                     axpby(-1.0, residual.p, 0.0, nullptr, rhs.p, D);
                     if (c.precision == DGH_PRECISION_FP64) {
-                        bj_refresh();
+                        precond_refresh();
                         linear = gmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance);
                     } else {
                         bj_refresh_mixed();
@@ -1460,6 +1532,20 @@ int dgh_bj_apply(dgh_ctx *ctx, const double *r, double *z) {
     return guarded(ctx, [&] {
         check_ctx(ctx);
         ctx->bj_apply(r, z);
+        ctx->sync();
+    });
+}
+int dgh_ilu_refresh(dgh_ctx *ctx) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->ilu_refresh();
+        ctx->sync();
+    });
+}
+int dgh_ilu_apply(dgh_ctx *ctx, const double *r, double *z) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->ilu_apply(r, z);
         ctx->sync();
     });
 }

@@ -64,6 +64,20 @@ void validate_config(const dgh_config &c) {
     require(c.inner_restart > 0 && c.inner_restart <= 31 && c.inner_max > 0, "Invalid iteration limit");
     require(std::isfinite(c.inner_tolerance) && c.inner_tolerance > 0 && c.inner_tolerance < 1,
             "Invalid solver tolerance or wave-speed safety");
+    require(c.precond == 0 || c.precond == 1, "Invalid block preconditioner");
+    require(c.ordering >= 0 && c.ordering <= 2, "Invalid ordering");
+    if (c.precond == 1) {
+        // The device block ILU(0) (block_ilu.cpp) is built for the DG element
+        // graph in red-black order, where all its updates are diagonal.
+        require(c.ordering == 2, "Block ILU(0) on the device runs in red-black order: use --ordering redblack");
+        require(c.nx % 2 == 0 && c.ny % 2 == 0 && c.nx >= 4 && c.ny >= 4,
+                "Block ILU(0) on the device runs in red-black order: nx and ny must be even and >= 4");
+        require(c.method == DGH_METHOD_DG, "Block ILU(0) on the device is built for --method dg");
+        require(c.precision == DGH_PRECISION_FP64, "Block ILU(0) on the device is built for --precision fp64");
+        require(!c.direct, "Block ILU(0) preconditions GMRES; --linear direct takes no preconditioner");
+    } else {
+        require(c.ordering != 2, "Red-black ordering applies to --precond block-ilu0");
+    }
 }
 
 dgh_config parse_config(int argc, const char *const *argv) {
@@ -133,11 +147,13 @@ dgh_config parse_config(int argc, const char *const *argv) {
             c.direct = name == "direct";
         } else if (option == "--ordering") {
             auto order = value();
-            require(order == "natural" || order == "colamd", "Invalid ordering");
+            require(order == "natural" || order == "colamd" || order == "redblack", "Invalid ordering");
             c.colamd = order == "colamd";
+            c.ordering = order == "natural" ? 0 : order == "colamd" ? 1 : 2;
         } else if (option == "--precond") {
             auto name = value();
-            require(name == "block-jacobi", "2D currently supports block-jacobi only");
+            require(name == "block-jacobi" || name == "block-ilu0", "Invalid block preconditioner");
+            c.precond = name == "block-ilu0";
         } else if (option == "--device") {
             auto name = value();
             require(name == "gpu", "Only --device gpu is built; the CPU path is the reference executable");
@@ -160,8 +176,8 @@ dgh_config parse_config(int argc, const char *const *argv) {
         } else if (option == "--help") {
             std::cout << "2D periodic tensor-product Euler DG / condensed HDG (MI355X device path)\n"
                          "--nx NX --ny NY --testcase diagonal|wave|axis|seeded --k K --quad Q --cfl C (--T T | --steps N)\n"
-                         "--method dg|hdg|both --linear direct|gmres --precond block-jacobi\n"
-                         "--ordering natural|colamd --restart N --max-linear N --max-newton N\n"
+                         "--method dg|hdg|both --linear direct|gmres --precond block-jacobi|block-ilu0\n"
+                         "--ordering natural|colamd|redblack --restart N --max-linear N --max-newton N\n"
                          "--ntol R --ltol R --safety S --prefix PATH\n"
                          "--integrator ssp-dirk3|be --newton-steps N --warmup-updates N\n"
                          "--timing-only --check-update --no-save-solution\n"

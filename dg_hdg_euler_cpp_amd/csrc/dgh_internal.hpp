@@ -41,6 +41,15 @@ struct Geo {
 // arrays over face nodes f*n + q.
 enum FaceField { F_UL, F_VL, F_HL, F_UR, F_VR, F_HR, F_BL, F_BR, F_PEN, F_TAUL, F_TAUR, F_NFIELDS };
 
+// Red-black colouring of an even nx x ny periodic grid (the block ILU(0) row
+// order): colour c holds the cells with (x + y) % 2 == c, numbered in natural
+// order; its k-th cell sits in row y = k / (nx/2) at x = 2 (k % (nx/2)) + ((y + c) & 1).
+__host__ __device__ inline int colour_cell(int k, int colour, int nx) {
+    const int half = nx / 2, y = k / half;
+    return 2 * (k % half) + ((y + colour) & 1) + nx * y;
+}
+__host__ __device__ inline int colour_index(int cell, int nx) { return (cell / nx) * (nx / 2) + (cell % nx) / 2; }
+
 // ------------------------------------------------------------ launchers
 // All launchers enqueue on `stream` and return hipError_t of the launch.
 // Vectors use the reference layout cell*4n^2 + v*n^2 + (i + n*j).
@@ -112,12 +121,20 @@ hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fca
 hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fcache, float *blocks,
                                   double inv_step, const Basis *b, const Geo &g, int first, int count,
                                   hipStream_t s);
+// Element blocks of the cells of one colour, compact: blocks[k * m*m] for the
+// colour's k-th cell.
+hipError_t launch_assemble_colour(int n, const double *vcache, const double *fcache, double *blocks,
+                                  double inv_step, const Basis *b, const Geo &g, int colour, hipStream_t s);
 hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s);
 hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s);
 hipError_t launch_bj_apply(int m, const double *factors, const int *rowsrc, const double *r, double *z,
                            int nblocks, hipStream_t s);
 hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const float *r, float *z, int nblocks,
                            hipStream_t s);
+// Compact factors of one colour's cells; r and z in the full vector layout,
+// only that colour's cells read and written.
+hipError_t launch_bj_apply_colour(int m, const double *factors, const int *rowsrc, const double *r, double *z,
+                                  int nblocks, int colour, int nx, hipStream_t s);
 // factors: bfloat16 storage (FP32 arithmetic)
 hipError_t launch_bj_apply_bf16(int m, const void *factors, const int *rowsrc, const float *r, float *z,
                                 int nblocks, hipStream_t s);
@@ -171,6 +188,20 @@ hipError_t launch_dot(const double *a, const double *b, long long count, double 
 // out = (M lambda - k) / tau
 hipError_t launch_mass_combine(int n, const double *lambda, const double *k, double tau, double *out, long long count,
                                const Basis *b, hipStream_t s);
+
+// Block ILU(0) of the DG element graph in red-black order (block_ilu.cpp:48-84
+// with the symmetric permutation red cells first). S holds, per red cell and
+// side s, the 4n x 4n coupling R_s A^-1 E_s of its factorised block (trace of
+// side s of the block solve against the side's lift columns).
+hipError_t launch_ilu_coupling(int n, const double *factors_red, const int *rowsrc_red, double *S, const Basis *b,
+                               const Geo &g, hipStream_t s);
+// A_bb -= sum over the four red neighbours of L_bj U_jb (rank-4n per side)
+hipError_t launch_ilu_update(int n, double *blocks_black, const double *S, const double *fcache, const Basis *b,
+                             const Geo &g, hipStream_t s);
+// out_c = rhs_c - sum over the four neighbours of A_{c,nb} x_nb for the cells of
+// one colour, with x given by its side traces
+hipError_t launch_ilu_offdiag(int n, int colour, const double *traces, const double *fcache, const double *rhs,
+                              double *out, const Basis *b, const Geo &g, hipStream_t s);
 
 // Statically condensed HDG (hybrid.cpp:31-121).
 hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache, double *blocks, double inv_step,

@@ -6,33 +6,13 @@
 // trace index face*(4n) + v*n + q (discretization.cpp:148). X is stored per
 // element column-major [m x t] (t = 16n), S_K column-major [t x t].
 #include "dgh_internal.hpp"
+#include "element_solve.hpp"
 
 namespace dgh {
 namespace {
 
 constexpr int BLOCK = 256;
 
-__device__ __forceinline__ int neighbor(int cell, int side, int nx, int ny) {
-    int x = cell % nx, y = cell / nx;
-    if (side < 2) x = (x + (side == 0 ? nx - 1 : 1)) % nx;
-    else y = (y + (side == 2 ? ny - 1 : 1)) % ny;
-    return x + nx * y;
-}
-__device__ __forceinline__ int cell_face(int cell, int side, const Geo &g) {
-    return side == 0 ? neighbor(cell, 0, g.nx, g.ny)
-         : side == 1 ? cell
-         : side == 2 ? g.cells + neighbor(cell, 2, g.nx, g.ny)
-                     : g.cells + cell;
-}
-__device__ __forceinline__ int face_node(int side, int q, int i, int n) { return side < 2 ? i + n * q : q + n * i; }
-__device__ __forceinline__ double readlane_d(double v, int lane) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double face_weight(int face, int q, const Basis *__restrict__ B, const Geo &g) {
-    return .5 * (face < g.cells ? g.hy : g.hx) * B->w[q];
-}
 
 // FaceJacobian::hdg (face_jacobian.cpp:77-82): w ((+-1) A_phys + tau I), row-major [v][w].
 __device__ void hdg_matrix(const double *__restrict__ fc, long long nodes, int face, int q, bool left,
@@ -127,82 +107,7 @@ __global__ __launch_bounds__(SOLVE_WAVES * 64) void k_hdg_solve(const double *__
             x[cc][a] = val;
         }
     }
-    // The factor columns stream through a ring of SOLVE_RING buffers so the
-    // loads of the next columns are in flight while a column is applied; they
-    // are read once per pass, hence non-temporal.
-    double buf[SOLVE_RING][NA];
-    auto load_lower = [&](double(&b)[NA], int c, int a0) {  // rows > c, chunks >= a0
-        const double *colp = F + static_cast<long long>(c) * M;
-#pragma unroll
-        for (int a = 0; a < NA; ++a) {
-            const int row = a * 64 + lane;
-            if (a >= a0) b[a] = (row > c && row < M) ? __builtin_nontemporal_load(colp + row) : 0.0;
-        }
-    };
-    auto load_upper = [&](double(&b)[NA], int c, int a1) {  // rows <= c, chunks <= a1
-        const double *colp = F + static_cast<long long>(c) * M;
-#pragma unroll
-        for (int a = 0; a < NA; ++a) {
-            const int row = a * 64 + lane;
-            if (a <= a1) b[a] = row <= c ? __builtin_nontemporal_load(colp + row) : 0.0;
-        }
-    };
-    // forward substitution with the unit lower L (column j in ring slot j % SOLVE_RING)
-#pragma unroll
-    for (int u = 0; u < SOLVE_RING; ++u)
-        if (u < M) load_lower(buf[u], u, 0);
-#pragma unroll
-    for (int ja = 0; ja < NA; ++ja) {
-        const int jend = min(64, M - ja * 64);
-#pragma clang loop unroll(disable)
-        for (int jl = 0; jl < jend; jl += SOLVE_RING) {
-#pragma unroll
-            for (int u = 0; u < SOLVE_RING; ++u) {
-                if (jl + u < jend) {
-                    const int j = ja * 64 + jl + u;
-#pragma unroll
-                    for (int cc = 0; cc < SOLVE_COLS; ++cc) {
-                        const double xj = readlane_d(x[cc][ja], jl + u);
-#pragma unroll
-                        for (int a = ja; a < NA; ++a) x[cc][a] -= buf[u][a] * xj;
-                    }
-                    if (j + SOLVE_RING < M) load_lower(buf[u], j + SOLVE_RING, ja);
-                }
-            }
-        }
-    }
-    // backward substitution with U (column j in ring slot (M - 1 - j) % SOLVE_RING)
-#pragma unroll
-    for (int u = 0; u < SOLVE_RING; ++u)
-        if (M - 1 - u >= 0) load_upper(buf[u], M - 1 - u, NA - 1);
-#pragma unroll
-    for (int ja = NA - 1; ja >= 0; --ja) {
-        const int jend = min(64, M - ja * 64);
-        const int t0 = M - ja * 64 - jend;
-#pragma clang loop unroll(disable)
-        for (int g0 = 0; g0 < jend; g0 += SOLVE_RING) {
-#pragma unroll
-            for (int u = 0; u < SOLVE_RING; ++u) {
-                const int jl = jend - 1 - g0 - u;
-                if (jl >= 0) {
-                    const int j = ja * 64 + jl;
-                    double(&cur)[NA] = buf[(t0 + u) % SOLVE_RING];
-                    const double diag = readlane_d(cur[ja], jl);
-#pragma unroll
-                    for (int cc = 0; cc < SOLVE_COLS; ++cc) {
-                        const double xj = readlane_d(x[cc][ja], jl) / diag;
-                        if (lane == jl) x[cc][ja] = xj;
-#pragma unroll
-                        for (int a = 0; a <= ja; ++a) {
-                            const int row = a * 64 + lane;
-                            if (row < j) x[cc][a] -= cur[a] * xj;
-                        }
-                    }
-                    if (j - SOLVE_RING >= 0) load_upper(cur, j - SOLVE_RING, ja);
-                }
-            }
-        }
-    }
+    lu_solve_columns<M, SOLVE_COLS, SOLVE_RING>(F, x, lane);
 #pragma unroll
     for (int cc = 0; cc < SOLVE_COLS; ++cc) {
         const int c = cols[cc];

@@ -160,11 +160,12 @@ __device__ void face_block_terms(int cell, const double *__restrict__ fc, double
 template <int N, typename T, bool HDG, bool TRANSPOSE = false>
 __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ vc, const double *__restrict__ fc,
                                                     T *__restrict__ blocks, double inv_step,
-                                                    const Basis *__restrict__ B, Geo g, int first) {
+                                                    const Basis *__restrict__ B, Geo g, int first,
+                                                    int colour = -1) {
     constexpr int P = N * N, M = 4 * P;
     __shared__ double ax[P * 16], ay[P * 16];  // [node][v*4+w]
     __shared__ double hs[4 * N * 16];          // [side][q][v*4+w]
-    const int cell = first + blockIdx.x;
+    const int cell = colour < 0 ? first + static_cast<int>(blockIdx.x) : colour_cell(blockIdx.x, colour, g.nx);
     const double gamma = g.gamma, gm = gamma - 1.;
     for (int node = threadIdx.x; node < P; node += BLOCK) {
         const double u = vc[static_cast<long long>(cell) * 3 * P + node];
@@ -259,10 +260,12 @@ __device__ __forceinline__ double jacobian_entry(double u, double v, double H, i
 template <int N, typename T, bool HDG, bool TRANSPOSE = false>
 __global__ __launch_bounds__(BLOCK) void k_assemble_oi(const double *__restrict__ vc, const double *__restrict__ fc,
                                                        T *__restrict__ blocks, double inv_step,
-                                                       const Basis *__restrict__ B, Geo g, int first) {
+                                                       const Basis *__restrict__ B, Geo g, int first,
+                                                       int colour = -1) {
     constexpr int P = N * N, M = 4 * P;
     __shared__ double uvh[3 * MAXQ * MAXQ], hs[4 * N * 16], lx[MAXQ * P], ly[MAXQ * P];
-    const int cell = first + blockIdx.x, Q = g.nq, QQ = Q * Q;
+    const int cell = colour < 0 ? first + static_cast<int>(blockIdx.x) : colour_cell(blockIdx.x, colour, g.nx);
+    const int Q = g.nq, QQ = Q * Q;
     for (int t = threadIdx.x; t < 3 * QQ; t += BLOCK) uvh[t] = vc[static_cast<long long>(cell) * 3 * QQ + t];
     face_block_terms<N, HDG>(cell, fc, hs, B, g);
     T *out = blocks + static_cast<long long>(blockIdx.x) * M * M;
@@ -548,7 +551,8 @@ template <typename T> __device__ __forceinline__ T factor_load(const T *p) {
 // arithmetic type (double for the FP64 path, float for the inner FP32 solve).
 template <int M, typename TF, typename TV, int U>
 __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ factors, const int *__restrict__ rowsrc,
-                                                    const TV *__restrict__ r, TV *__restrict__ z, int nblocks) {
+                                                    const TV *__restrict__ r, TV *__restrict__ z, int nblocks,
+                                                    int colour, int nx) {
     static_assert(64 % U == 0, "ring depth must divide the chunk width");
     using T = TF;
     constexpr int NA = (M + 63) / 64;
@@ -556,11 +560,13 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
     const long long blk = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + threadIdx.x / 64;
     if (blk >= nblocks) return;
     const T *F = factors + blk * M * M;
+    // vector position of the block: its cell (compact colour storage) or itself
+    const long long vb = colour < 0 ? blk : colour_cell(static_cast<int>(blk), colour, nx);
     TV x[NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
         const int row = a * 64 + lane;
-        x[a] = row < M ? r[blk * M + rowsrc[blk * M + row]] : TV(0);
+        x[a] = row < M ? r[vb * M + rowsrc[blk * M + row]] : TV(0);
     }
     T buf[U][NA];
     // strictly lower part of column c, chunks a >= a0
@@ -636,7 +642,7 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
         const int row = a * 64 + lane;
-        if (row < M) z[blk * M + row] = x[a];
+        if (row < M) z[vb * M + row] = x[a];
     }
 }
 
@@ -1021,6 +1027,14 @@ hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fca
         else k_assemble<N, float, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
     })
 }
+hipError_t launch_assemble_colour(int n, const double *vcache, const double *fcache, double *blocks,
+                                  double inv_step, const Basis *b, const Geo &g, int colour, hipStream_t s) {
+    const int count = g.cells / 2;
+    DGH_DISPATCH_N(n, {
+        if (g.nq > N) k_assemble_oi<N, double, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, 0, colour);
+        else k_assemble<N, double, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, 0, colour);
+    })
+}
 hipError_t launch_assemble_blocks_transposed(int n, const double *vcache, const double *fcache, double *blocks,
                                              double inv_step, const Basis *b, const Geo &g, hipStream_t s) {
     DGH_DISPATCH_N(n, {
@@ -1066,20 +1080,26 @@ hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, 
 hipError_t launch_bj_apply(int m, const double *factors, const int *rowsrc, const double *r, double *z,
                            int nblocks, hipStream_t s) {
     DGH_DISPATCH_M(m, {
-        k_bj_apply<M, double, double, BJ_RING64><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks);
+        k_bj_apply<M, double, double, BJ_RING64><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks, -1, 0);
+    })
+}
+hipError_t launch_bj_apply_colour(int m, const double *factors, const int *rowsrc, const double *r, double *z,
+                                  int nblocks, int colour, int nx, hipStream_t s) {
+    DGH_DISPATCH_M(m, {
+        k_bj_apply<M, double, double, BJ_RING64><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks, colour, nx);
     })
 }
 hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const float *r, float *z, int nblocks,
                            hipStream_t s) {
     DGH_DISPATCH_M(m, {
-        k_bj_apply<M, float, float, BJ_RING32><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks);
+        k_bj_apply<M, float, float, BJ_RING32><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks, -1, 0);
     })
 }
 hipError_t launch_bj_apply_bf16(int m, const void *factors, const int *rowsrc, const float *r, float *z,
                                 int nblocks, hipStream_t s) {
     const auto *f = static_cast<const __hip_bfloat16 *>(factors);
     DGH_DISPATCH_M(m, {
-        k_bj_apply<M, __hip_bfloat16, float, BJ_RING16><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, rowsrc, r, z, nblocks);
+        k_bj_apply<M, __hip_bfloat16, float, BJ_RING16><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, rowsrc, r, z, nblocks, -1, 0);
     })
 }
 hipError_t launch_convert(const double *src, float *dst, long long count, hipStream_t s) {

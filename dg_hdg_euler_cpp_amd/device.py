@@ -228,6 +228,14 @@ class DeviceContext:
         self._call("dgh_bj_apply", _ptr(r), _ptr(z))
         return z
 
+    def ilu_refresh(self):
+        self._call("dgh_ilu_refresh")
+
+    def ilu_apply(self, r, out=None):
+        z = self.zeros() if out is None else out
+        self._call("dgh_ilu_apply", _ptr(r), _ptr(z))
+        return z
+
     def bj_refresh_mixed(self):
         self._call("dgh_bj_refresh_mixed")
 

@@ -68,6 +68,8 @@ typedef struct dgh_config {
     double inner_tolerance;  /* mixed: relative tolerance of one inner solve */
     unsigned long long seed;
     int graph;               /* 1: each GMRES Arnoldi step replays a captured hipGraph */
+    int precond;             /* 0 block-jacobi, 1 block-ilu0 (block_ilu.cpp; DG, FP64, red-black order) */
+    int ordering;            /* 0 natural, 1 colamd, 2 red-black (block-ilu0 on the device) */
     char prefix[512];
 } dgh_config;
 
@@ -166,6 +168,17 @@ int dgh_dg_block(dgh_ctx *ctx, int row, double *out_host);
 int dgh_bj_refresh(dgh_ctx *ctx);
 /* BlockJacobi::apply (block_jacobi.hpp:24-39): z = P^-1 r. */
 int dgh_bj_apply(dgh_ctx *ctx, const double *r, double *z);
+
+/* ---- block ILU(0) (BlockPreconditioner with Preconditioner::BlockILU0) ----- */
+/* BlockPreconditioner::refresh + factorize (block_ilu.hpp:14-19,
+ * block_ilu.cpp:48-61) on the DG element graph (BlockGraph::cartesian,
+ * block_graph.cpp:20-29) in red-black order: the cells with x + y even first.
+ * The reference's 2D CLI accepts only block-jacobi (config.cpp:100-104); its
+ * 1D solver runs this preconditioner (time_integrator.cpp:29-35 of dg_hdg_1d).
+ * Needs even nx, ny >= 4. Shares the factor storage with dgh_bj_refresh. */
+int dgh_ilu_refresh(dgh_ctx *ctx);
+/* BlockPreconditioner::apply (block_ilu.cpp:62-84): z = (L U)^-1 r; r != z. */
+int dgh_ilu_apply(dgh_ctx *ctx, const double *r, double *z);
 
 /* ---- mixed precision (no reference counterpart) ---------------------------- */
 /* Block Jacobi factors in FP32 (and BF16 storage for DGH_PRECISION_MIXED_BF16)

@@ -75,7 +75,7 @@ void validate(const Config &c) {
 // dg_hdg_2d/src/config.cpp:23-122 (flags the oracle does not need are parsed and ignored)
 Config parse(int argc, const char *const *argv) {
     Config c;
-    bool time_seen = false, steps_seen = false;
+    bool time_seen = false, steps_seen = false, ilu = false, red_black = false;
     for (int i = 1; i < argc; ++i) {
         std::string option = argv[i];
         auto value = [&]() {
@@ -124,14 +124,20 @@ Config parse(int argc, const char *const *argv) {
             c.direct = name == "direct";
         } else if (option == "--ordering") {
             auto order = value();
-            require(order == "natural" || order == "colamd", "Invalid ordering");
+            require(order == "natural" || order == "colamd" || order == "redblack", "Invalid ordering");
             c.colamd = order == "colamd";
+            red_black = order == "redblack";
         } else if (option == "--precond") {
             auto name = value();
-            require(name == "block-jacobi", "2D currently supports block-jacobi only");
+            require(name == "block-jacobi" || name == "block-ilu0", "2D currently supports block-jacobi only");
+            ilu = name == "block-ilu0";
         } else
             throw std::runtime_error("Unknown option: " + option);
     }
+    require(!ilu || red_black, "2D currently supports block-jacobi only");
+    require(!red_black || ilu, "Invalid ordering");
+    require(!ilu || c.method == "dg", "Block ILU(0) is built for --method dg");
+    c.ilu_red_black = ilu;
     require(!(time_seen && steps_seen), "Choose either --T or --steps");
     require(!steps_seen || c.steps > 0, "steps must be positive");
     validate(c);
@@ -1172,6 +1178,127 @@ void BlockJacobi::apply(const Vec &rhs, Vec &result) const {
     }
 }
 
+// ------------------------------------------------------------- block ILU(0)
+// dense_lu.hpp:38-45: for each row x of X solve y U = x (U^T lower), z L = y
+// (L^T unit upper), then X(row, rowsrc[i]) = z[i] (the P^T of PA = LU).
+void right_solve(const DenseLU &lu, Mat &x) {
+    const int n = lu.n;
+    Vec y(n);
+    for (int r = 0; r < x.rows; ++r) {
+        for (int k = 0; k < n; ++k) {
+            double s = x(r, k);
+            const double *c = lu.lu.col(k);
+            for (int i = 0; i < k; ++i) s -= c[i] * y[i];
+            y[k] = s / c[k];
+        }
+        for (int k = n - 1; k >= 0; --k) {
+            double s = y[k];
+            const double *c = lu.lu.col(k);
+            for (int i = k + 1; i < n; ++i) s -= c[i] * y[i];
+            y[k] = s;
+        }
+        for (int i = 0; i < n; ++i) x(r, lu.rowsrc[i]) = y[i];
+    }
+}
+std::vector<int> red_black_order(int nx, int ny) {
+    std::vector<int> order;
+    for (int colour = 0; colour < 2; ++colour)
+        for (int cell = 0; cell < nx * ny; ++cell)
+            if ((cell % nx + cell / nx) % 2 == colour) order.push_back(cell);
+    return order;
+}
+// block_ilu.cpp:3-14 (permuted_graph) and :15-47 (symbolic update schedule)
+BlockILU0::BlockILU0(const std::vector<std::vector<int>> &adjacency, int width, std::vector<int> order)
+    : rows_(static_cast<int>(adjacency.size())), width_(width), order_(std::move(order)) {
+    require(static_cast<int>(order_.size()) == rows_, "Block ordering size mismatch");
+    std::vector<int> inverse(rows_, -1);
+    for (int i = 0; i < rows_; ++i) inverse[order_[i]] = i;
+    for (int i = 0; i < rows_; ++i) require(inverse[i] >= 0, "Block ordering is not a permutation");
+    row_ptr_.assign(rows_ + 1, 0);
+    for (int i = 0; i < rows_; ++i) {
+        std::vector<int> row{i};
+        for (int c : adjacency[order_[i]]) row.push_back(inverse[c]);
+        std::sort(row.begin(), row.end());
+        row.erase(std::unique(row.begin(), row.end()), row.end());
+        col_idx_.insert(col_idx_.end(), row.begin(), row.end());
+        row_ptr_[i + 1] = static_cast<int>(col_idx_.size());
+    }
+    blocks_.resize(col_idx_.size());
+    diagonal_.resize(rows_);
+    work_.resize(static_cast<size_t>(rows_) * width_);
+    diagonal_slots_.resize(rows_);
+    update_offsets_.resize(col_idx_.size() + 1);
+    for (int row = 0; row < rows_; ++row) {
+        diagonal_slots_[row] = find(row, row);
+        for (int slot = row_ptr_[row]; slot < row_ptr_[row + 1]; ++slot) {
+            update_offsets_[slot] = static_cast<int>(update_targets_.size());
+            const int eliminated = col_idx_[slot];
+            if (eliminated >= row) continue;
+            for (int upper = row_ptr_[eliminated]; upper < row_ptr_[eliminated + 1]; ++upper) {
+                const int column = col_idx_[upper];
+                if (column <= eliminated) continue;
+                const int target = find(row, column);
+                if (target >= 0) {
+                    update_targets_.push_back(target);
+                    update_upper_.push_back(upper);
+                }
+            }
+        }
+    }
+    update_offsets_.back() = static_cast<int>(update_targets_.size());
+}
+int BlockILU0::find(int row, int column) const {  // block_graph.cpp:37-42
+    auto begin = col_idx_.begin() + row_ptr_[row], end = col_idx_.begin() + row_ptr_[row + 1];
+    auto found = std::lower_bound(begin, end, column);
+    return found != end && *found == column ? static_cast<int>(found - col_idx_.begin()) : -1;
+}
+// block_ilu.cpp:48-61; subtract_product is dense_lu.hpp:50-55
+void BlockILU0::factorize() {
+    for (int i = 0; i < rows_; ++i) {
+        for (int p = row_ptr_[i]; p < diagonal_slots_[i]; ++p) {
+            right_solve(diagonal_[col_idx_[p]], blocks_[p]);
+            const Mat &left = blocks_[p];
+            for (int u = update_offsets_[p]; u < update_offsets_[p + 1]; ++u) {
+                Mat &dest = blocks_[update_targets_[u]];
+                const Mat &right = blocks_[update_upper_[u]];
+                for (int j = 0; j < right.cols; ++j)
+                    for (int k = 0; k < left.cols; ++k) {
+                        const double rkj = right(k, j);
+                        double *d = dest.col(j);
+                        const double *l = left.col(k);
+                        for (int r = 0; r < dest.rows; ++r) d[r] -= rkj * l[r];
+                    }
+            }
+        }
+        diagonal_[i].compute(blocks_[diagonal_slots_[i]]);
+    }
+}
+// block_ilu.cpp:62-84
+void BlockILU0::apply(const Vec &rhs, Vec &result) const {
+    const size_t w = width_;
+    for (int i = 0; i < rows_; ++i)
+        std::memcpy(work_.data() + i * w, rhs.data() + order_[i] * w, sizeof(double) * w);
+    auto subtract = [&](int slot, int row) {
+        const Mat &b = blocks_[slot];
+        const double *x = work_.data() + col_idx_[slot] * w;
+        double *y = work_.data() + row * w;
+        for (size_t j = 0; j < w; ++j) {
+            const double xj = x[j];
+            const double *c = b.col(static_cast<int>(j));
+            for (size_t r = 0; r < w; ++r) y[r] -= c[r] * xj;
+        }
+    };
+    for (int i = 0; i < rows_; ++i)
+        for (int p = row_ptr_[i]; p < diagonal_slots_[i]; ++p) subtract(p, i);
+    for (int i = rows_ - 1; i >= 0; --i) {
+        for (int p = diagonal_slots_[i] + 1; p < row_ptr_[i + 1]; ++p) subtract(p, i);
+        diagonal_[i].solve_inplace(work_.data() + i * w);
+    }
+    result.resize(rhs.size());
+    for (int i = 0; i < rows_; ++i)
+        std::memcpy(result.data() + order_[i] * w, work_.data() + i * w, sizeof(double) * w);
+}
+
 // ------------------------------------------------------------------- GMRES
 // gmres.cpp:5-12
 GMRES::GMRES(int n, int restart, const Vec &weights)
@@ -1511,11 +1638,14 @@ RunStatistics TimeIntegrator::advance(Vec &solution) {
     const Vec &linear_weights = method_ == Method::DG ? p_.inverse_mass : p_.inverse_trace_mass;
     const int rhs_size = method_ == Method::DG ? p_.layout.size() : p_.trace_size();
     std::unique_ptr<BlockJacobi> preconditioner;
+    std::unique_ptr<BlockILU0> ilu;
     std::unique_ptr<GMRES> gmres;
     std::unique_ptr<DirectSolver> direct;
     if (c.direct) direct.reset(new DirectSolver(blocks, width));
     else {
-        preconditioner.reset(new BlockJacobi(blocks, width));
+        if (c.ilu_red_black && method_ == Method::DG)
+            ilu.reset(new BlockILU0(graph, width, red_black_order(p_.layout.nx, p_.layout.ny)));
+        else preconditioner.reset(new BlockJacobi(blocks, width));
         gmres.reset(new GMRES(rhs_size, c.restart, linear_weights));
     }
     Vec state(N), trial(N), residual(N), trial_residual(N), direction(N), rhs(rhs_size);
@@ -1571,6 +1701,12 @@ RunStatistics TimeIntegrator::advance(Vec &solution) {
                         linear = direct->solve(rhs, direction, linear_weights);
                         dg->apply(direction, equation_error);
                         for (size_t i = 0; i < N; ++i) equation_error[i] += residual[i];
+                    } else if (ilu) {
+                        ilu->refresh(provider);
+                        linear = gmres->solve([&](const Vec &x, Vec &y) { dg->apply(x, y); },
+                                              [&](const Vec &x, Vec &y) { ilu->apply(x, y); }, rhs, direction,
+                                              c.max_linear, c.linear_tolerance);
+                        for (size_t i = 0; i < N; ++i) equation_error[i] = -gmres->residual()[i];
                     } else {
                         preconditioner->refresh(provider);
                         linear = gmres->solve([&](const Vec &x, Vec &y) { dg->apply(x, y); },

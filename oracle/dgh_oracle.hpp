@@ -48,6 +48,10 @@ struct Config {
     double cfl = 2., final_time = .2, safety = 1.1;
     double nonlinear_tolerance = 1e-12, linear_tolerance = 1e-12;
     bool direct = false, colamd = false, timing_only = false, check_update = false, save_solution = true;
+    // Extension beside the reference's options: --precond block-ilu0 with
+    // --ordering redblack runs the DG block ILU(0) of block_ilu.cpp in
+    // red-black order (the reference's 2D parser rejects block-ilu0).
+    bool ilu_red_black = false;
     std::string integrator = "ssp-dirk3", method = "both", testcase = "diagonal", prefix = "output/run_2d";
 };
 Config parse(int argc, const char *const *argv);  // config.cpp:23-122
@@ -222,6 +226,45 @@ class BlockJacobi {  // block_jacobi.hpp:12-39
     int blocks_, width_;
     std::vector<DenseLU> lus_;
 };
+
+// block_ilu.cpp:3-84 with block_graph.cpp:7-44: block ILU(0) on the permuted
+// block graph (no fill). order[new] = old row; the graph rows hold the
+// diagonal. Refresh asks the provider for every pattern block in the original
+// numbering, factorizes in the permuted order and keeps L (unit lower, strictly
+// lower slots), U (upper slots) and the LU of each updated diagonal block.
+class BlockILU0 {
+  public:
+    BlockILU0(const std::vector<std::vector<int>> &adjacency, int width, std::vector<int> order);
+    template <class Provider> void refresh(Provider &&provider) {
+        for (int row = 0; row < rows_; ++row)
+            for (int p = row_ptr_[row]; p < row_ptr_[row + 1]; ++p) {
+                blocks_[p] = Mat(width_, width_);
+                provider(order_[row], order_[col_idx_[p]], blocks_[p]);
+            }
+        factorize();
+    }
+    void apply(const Vec &rhs, Vec &result) const;
+    int rows() const { return rows_; }
+    const std::vector<int> &row_ptr() const { return row_ptr_; }
+    const std::vector<int> &col_idx() const { return col_idx_; }
+    const std::vector<int> &order() const { return order_; }
+    const Mat &block(int slot) const { return blocks_[slot]; }
+    const DenseLU &diagonal(int row) const { return diagonal_[row]; }
+
+  private:
+    int rows_, width_;
+    std::vector<int> order_, row_ptr_, col_idx_, diagonal_slots_, update_offsets_, update_targets_, update_upper_;
+    std::vector<Mat> blocks_;
+    std::vector<DenseLU> diagonal_;
+    mutable Vec work_;
+    int find(int row, int column) const;
+    void factorize();
+};
+// X <- X A^-1 with the LU of A, one row of X at a time (dense_lu.hpp:38-45).
+void right_solve(const DenseLU &lu, Mat &x);
+// Red-black order of an nx x ny grid: cells with x + y even, then odd, each in
+// natural order (a two-colour ordering; not one of the reference's orderings).
+std::vector<int> red_black_order(int nx, int ny);
 
 struct LinearResult { int iterations; double relative_residual; };
 using Action = std::function<void(const Vec &, Vec &)>;  // gmres.hpp:5-14

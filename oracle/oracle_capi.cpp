@@ -18,6 +18,7 @@ struct Handle {
     std::unique_ptr<BlockJacobi> bj;
     std::unique_ptr<HybridOperator> hdg;
     std::unique_ptr<BlockJacobi> face_bj;
+    std::unique_ptr<BlockILU0> ilu;
     explicit Handle(const Config &c) : problem(c) {}
     DGOperator &op() {
         if (!dg) dg.reset(new DGOperator(problem));
@@ -237,6 +238,65 @@ int orc_bj_apply(void *h, const double *r, double *z) {
         Vec out;
         H->bj->apply(copy_in(r, H->problem.layout.size()), out);
         copy_out(out, z);
+    });
+}
+// DG block ILU(0) over the cartesian element graph; ordering 0 natural, 1 red-black
+int orc_ilu_refresh(void *h, int ordering) {
+    return guarded([&] {
+        auto *H = static_cast<Handle *>(h);
+        auto &p = H->problem;
+        std::vector<int> order(p.layout.cells);
+        for (int i = 0; i < p.layout.cells; ++i) order[i] = i;
+        if (ordering == 1) order = red_black_order(p.layout.nx, p.layout.ny);
+        H->ilu.reset(new BlockILU0(cartesian_graph(p.layout.nx, p.layout.ny), p.layout.block_size(), order));
+        auto &op = H->op();
+        H->ilu->refresh([&](int r, int c, Mat &b) { op.block(r, c, b); });
+    });
+}
+int orc_ilu_apply(void *h, const double *r, double *z) {
+    return guarded([&] {
+        auto *H = static_cast<Handle *>(h);
+        require(H->ilu != nullptr, "Refresh the block ILU(0) preconditioner first");
+        Vec out;
+        H->ilu->apply(copy_in(r, H->problem.layout.size()), out);
+        copy_out(out, z);
+    });
+}
+int orc_ilu_gmres(void *h, const double *rhs, double *x, int restart, int limit, double tol, int *its, double *rel) {
+    return guarded([&] {
+        auto *H = static_cast<Handle *>(h);
+        auto &p = H->problem;
+        require(H->ilu != nullptr, "Refresh the block ILU(0) preconditioner first");
+        GMRES g(p.layout.size(), restart, p.inverse_mass);
+        Vec out;
+        auto &op = H->op();
+        auto &ilu = *H->ilu;
+        auto r = g.solve([&](const Vec &a, Vec &b) { op.apply(a, b); }, [&](const Vec &a, Vec &b) { ilu.apply(a, b); },
+                         copy_in(rhs, p.layout.size()), out, limit, tol);
+        copy_out(out, x);
+        *its = r.iterations;
+        *rel = r.relative_residual;
+    });
+}
+// Standalone block ILU(0) of a block matrix given in CSR over blocks (each row
+// holding its diagonal; blocks column-major per slot), in the order `order`;
+// solves with rhs into x.
+int orc_block_ilu_solve(int rows, const int *row_ptr, const int *col_idx, int width, const double *blocks,
+                        const int *order, const double *rhs, double *x) {
+    return guarded([&] {
+        std::vector<std::vector<int>> adjacency(rows);
+        for (int r = 0; r < rows; ++r) adjacency[r].assign(col_idx + row_ptr[r], col_idx + row_ptr[r + 1]);
+        auto slot = [&](int r, int c) {
+            for (int p = row_ptr[r]; p < row_ptr[r + 1]; ++p)
+                if (col_idx[p] == c) return p;
+            throw std::runtime_error("Requested structural zero block");
+        };
+        BlockILU0 ilu(adjacency, width, std::vector<int>(order, order + rows));
+        const size_t bb = static_cast<size_t>(width) * width;
+        ilu.refresh([&](int r, int c, Mat &b) { std::memcpy(b.a.data(), blocks + slot(r, c) * bb, bb * sizeof(double)); });
+        Vec out;
+        ilu.apply(Vec(rhs, rhs + static_cast<size_t>(rows) * width), out);
+        std::memcpy(x, out.data(), out.size() * sizeof(double));
     });
 }
 // LU factors of one refreshed block (column-major, m*m) and its row sources

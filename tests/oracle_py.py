@@ -185,6 +185,23 @@ class Oracle:
                    C.c_double(tol), C.byref(its), C.byref(rel), _p(res))
         return x, its.value, rel.value, res
 
+    # ---- block ILU(0) over the element graph (block_ilu.cpp); ordering "natural" or "redblack"
+    def ilu_refresh(self, ordering="natural"):
+        self._call("orc_ilu_refresh", C.c_int({"natural": 0, "redblack": 1}[ordering]))
+
+    def ilu_apply(self, r):
+        z = np.zeros(self.size)
+        self._call("orc_ilu_apply", _p(np.ascontiguousarray(r)), _p(z))
+        return z
+
+    def ilu_gmres(self, rhs, restart, limit, tol):
+        x = np.zeros(self.size)
+        its = C.c_int()
+        rel = C.c_double()
+        self._call("orc_ilu_gmres", _p(np.ascontiguousarray(rhs)), _p(x), C.c_int(restart), C.c_int(limit),
+                   C.c_double(tol), C.byref(its), C.byref(rel))
+        return x, its.value, rel.value
+
     def dense_jacobian(self):
         out = np.zeros(self.size * self.size)
         self._call("orc_dense_jacobian", _p(out))
@@ -261,6 +278,28 @@ def gauss(n):
     if rc:
         raise OracleError(lib().orc_error().decode())
     return x, w
+
+
+def block_ilu_solve(adjacency, blocks, order, rhs):
+    """Block ILU(0) (block_ilu.cpp) of the block matrix {(r, c): blocks[(r, c)]} on
+    the graph `adjacency` (rows include their diagonal), rows in `order`; returns
+    the preconditioner applied to rhs."""
+    rows = len(adjacency)
+    width = next(iter(blocks.values())).shape[0]
+    cols = [sorted(set(a) | {r}) for r, a in enumerate(adjacency)]
+    row_ptr = np.zeros(rows + 1, dtype=np.int32)
+    for r in range(rows):
+        row_ptr[r + 1] = row_ptr[r] + len(cols[r])
+    col_idx = np.array([c for row in cols for c in row], dtype=np.int32)
+    data = np.concatenate([np.asarray(blocks[(r, c)], dtype=np.float64).ravel(order="F")
+                           for r in range(rows) for c in cols[r]])
+    x = np.zeros(rows * width)
+    rc = lib().orc_block_ilu_solve(C.c_int(rows), row_ptr.ctypes.data_as(_I), col_idx.ctypes.data_as(_I),
+                                   C.c_int(width), _p(data), np.asarray(order, dtype=np.int32).ctypes.data_as(_I),
+                                   _p(np.ascontiguousarray(rhs, dtype=np.float64)), _p(x))
+    if rc:
+        raise OracleError(lib().orc_error().decode())
+    return x
 
 
 def lu_solve(a, b):

@@ -41,7 +41,8 @@ class Cfg(C.Structure):
                 ("timing_only", C.c_int), ("check_update", C.c_int), ("save_solution", C.c_int),
                 ("integrator", C.c_int), ("method", C.c_int), ("testcase", C.c_int), ("precision", C.c_int),
                 ("inner_restart", C.c_int), ("inner_max", C.c_int), ("inner_tolerance", C.c_double),
-                ("seed", C.c_ulonglong), ("graph", C.c_int), ("prefix", C.c_char * 512)]
+                ("seed", C.c_ulonglong), ("graph", C.c_int), ("precond", C.c_int), ("ordering", C.c_int),
+                ("prefix", C.c_char * 512)]
 
 
 def test_config_struct_layout_and_defaults():
@@ -58,12 +59,25 @@ def test_config_struct_layout_and_defaults():
                                     "--precision", "mixed", "--device", "gpu", "--restart", "20"]))
     assert (c.nx, c.degree, c.integrator, c.steps, c.precision, c.restart) == (256, 8, 0, 1, 1, 20)
     assert c.graph == 0 and Cfg.from_buffer_copy(parse(["--graph"])).graph == 1
+    # device block ILU(0): DG, FP64, red-black order on an even grid
+    c = Cfg.from_buffer_copy(parse(["--precond", "block-ilu0", "--ordering", "redblack", "--method", "dg"]))
+    assert (c.precond, c.ordering, c.colamd) == (1, 2, 0)
+    c = Cfg.from_buffer_copy(parse(["--ordering", "colamd"]))
+    assert (c.precond, c.ordering, c.colamd) == (0, 1, 1)
 
 
 @pytest.mark.parametrize("args,message", [
     (["--k", "9"], "Invalid mesh or degree"),
     (["--T", "0.1", "--steps", "2"], "Choose either --T or --steps"),
-    (["--precond", "block-ilu0"], "2D currently supports block-jacobi only"),
+    (["--precond", "ilu"], "Invalid block preconditioner"),
+    (["--precond", "block-ilu0", "--method", "dg"],
+     "Block ILU(0) on the device runs in red-black order: use --ordering redblack"),
+    (["--precond", "block-ilu0", "--ordering", "redblack", "--method", "dg", "--nx", "5"],
+     "Block ILU(0) on the device runs in red-black order: nx and ny must be even and >= 4"),
+    (["--precond", "block-ilu0", "--ordering", "redblack"], "Block ILU(0) on the device is built for --method dg"),
+    (["--precond", "block-ilu0", "--ordering", "redblack", "--method", "dg", "--precision", "mixed"],
+     "Block ILU(0) on the device is built for --precision fp64"),
+    (["--ordering", "redblack"], "Red-black ordering applies to --precond block-ilu0"),
     (["--linear", "lu"], "Invalid linear solver"),
     (["--integrator", "rk4"], "Invalid integrator"),
     (["--method", "fem"], "Method must be dg, hdg, or both"),
