@@ -61,6 +61,36 @@ def run_hdg(args, nx, k, init):
             "rms": [x["residual_physical_rms"] for x in upd]}
 
 
+def run_ilu(args, nx, k, init):
+    """Block ILU(0) in red-black order (FP64) at the same workload: Newton
+    updates/s, Krylov iterations, and the preconditioner's refresh and apply
+    times (HIP events on the library's stream)."""
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    ilu = ["--precond", "block-ilu0", "--ordering", "redblack"]
+    with DeviceContext(solver_args(nx, k, init, "fp64", args.warmup + args.steps, args.warmup) + ilu) as dev:
+        u = dev.initial()
+        dev.set_timing(True)
+        dev.reset_kernel_times()
+        stats = dev.advance(u)
+        times = {name: dev.kernel_time(name) for name in ["assemble", "lu", "ilu_coupling", "ilu_update",
+                                                          "ilu_apply", "jv"]}
+        dev.set_timing(False)
+        u2 = dev.initial()
+        stats_untimed = dev.advance(u2)
+    measured = stats_untimed["measured_newton_seconds"]
+    upd = stats_untimed["updates"]
+    log(f"[bench] ilu0 fp64 {init}: updates={len(upd)} mean_s={measured / max(1, stats_untimed['measured_updates']):.4f}")
+    for x in upd:
+        log(f"   update {x['iteration']}: rms={x['residual_physical_rms']:.6e} krylov={x['krylov_iterations']} "
+            f"t={x['iteration_s']:.4f}s")
+    per = {name: {"ms_total": ms, "launches": n, "avg_ms": ms / max(1, n)} for name, (ms, n) in times.items()}
+    return {"updates_per_s": stats_untimed["measured_updates"] / measured if measured > 0 else None,
+            "ms_per_update": 1e3 * measured / max(1, stats_untimed["measured_updates"]),
+            "krylov_per_update": sum(x["krylov_iterations"] for x in upd[args.warmup:]) / max(1, len(upd) - args.warmup),
+            "rms": [x["residual_physical_rms"] for x in upd], "timed_regions": per}
+
+
 def run_precision_check(args, nx, k, init, precision):
     """Untimed: every update of the mixed form also solves its Newton system
     through the FP64 path and reports ||du_mixed - du_fp64||_M / ||du_fp64||_M."""
@@ -232,6 +262,7 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the untimed mixed-vs-FP64 update comparison")
     ap.add_argument("--graph", action="store_true", help="GMRES Arnoldi steps as captured hipGraphs")
     ap.add_argument("--only-hdg", action="store_true", help="run and report the condensed HDG form alone")
+    ap.add_argument("--no-ilu", action="store_true", help="skip the block ILU(0) form")
     ap.add_argument("--probe", action="store_true", help="run each hot kernel 3x for counter profiles")
     args = ap.parse_args()
     global GRAPH
@@ -289,6 +320,7 @@ def main():
                  "max_rel_diff_first_two_updates": max(abs(x - y) / abs(x) for x, y in zip(a[:2], b[:2])),
                  "speedup_mixed_over_fp64": forms_out["mixed"]["updates_per_s"] / forms_out["fp64"]["updates_per_s"]}
     hdg = None if args.no_hdg or args.precision != "both" else run_hdg(args, args.nx, args.k, args.init)
+    ilu = None if args.no_ilu or args.precision != "both" else run_ilu(args, args.nx, args.k, args.init)
     check = None if args.no_check or head == "fp64" else run_precision_check(args, args.nx, args.k, args.init, head)
     cpu = None
     if not args.no_cpu_baseline:
@@ -317,6 +349,7 @@ def main():
         "equal_residual": equal,
         "update_error_vs_fp64": check,
         "hdg_fp64": hdg,
+        "ilu0_redblack_fp64": ilu,
         "jv": {"dof_per_s": g["size"] / dev["jv_avg_s"], "achieved_GBs": jv_bytes(g) / dev["jv_avg_s"] / 1e9,
                "avg_launch_s": dev["jv_avg_s"]},
         "cpu_baseline": None if cpu is None else {k: v for k, v in cpu.items() if k != "seconds_per_update"},

@@ -214,7 +214,7 @@ struct dgh_ctx {
     // Block ILU(0) in red-black order (cfg.precond == 1): compact red factors
     // in factors[0, cells/2), black ones in factors[cells/2, cells); the red
     // couplings S [red cell][side][4n x 4n]; apply work vectors
-    DevBuf<double> ilu_S, ilu_z, ilu_w;
+    DevBuf<double> ilu_Y, ilu_S, ilu_z, ilu_w;
     bool ilu_ready = false;
     DevBuf<double> gH, gcs, gsn, gg, gnext;
     double *gout_host = nullptr, *gout_dev = nullptr;
@@ -472,6 +472,7 @@ struct dgh_ctx {
         factors.alloc(static_cast<size_t>(cells) * mm, bytes);
         rowsrc.alloc(static_cast<size_t>(cells) * m, bytes);
         ilu_S.alloc(static_cast<size_t>(half) * 4 * 16 * n * n, bytes);
+        ilu_Y.alloc(static_cast<size_t>(half) * 16 * n * m, bytes);
         if (!ilu_z.p) {
             ilu_z.alloc(D, bytes);
             ilu_w.alloc(D, bytes);
@@ -487,7 +488,7 @@ struct dgh_ctx {
                                             stream));
         });
         timed("lu", [&] { HIPCHECK(launch_lu(m, factors.p, rowsrc.p, half, flags.p + 2, stream)); });
-        timed("ilu_coupling", [&] { HIPCHECK(launch_ilu_coupling(n, factors.p, rowsrc.p, ilu_S.p, basis.p, geo, stream)); });
+        timed("ilu_coupling", [&] { HIPCHECK(launch_ilu_coupling(n, factors.p, rowsrc.p, ilu_Y.p, ilu_S.p, basis.p, geo, stream)); });
         timed("ilu_update", [&] {
             HIPCHECK(launch_ilu_update(n, factors.p + half * mm, ilu_S.p, fcache.p, basis.p, geo, stream));
         });
@@ -498,8 +499,9 @@ struct dgh_ctx {
         require(read_flag(2) == 0, "Singular or nonfinite LU factors");
         ilu_ready = true;
     }
-    // block_ilu.cpp:62-84 in red-black order: z_r = A_rr^-1 r_r;
-    // w_b = r_b - sum A_br z_r; z_b = A_bb'^-1 w_b; z_r = A_rr^-1 (r_r - sum A_rb z_b)
+    // block_ilu.cpp:62-84 in red-black order: z0_r = A_rr^-1 r_r;
+    // w_b = r_b - sum A_br z0_r; z_b = A_bb'^-1 w_b;
+    // z_r = A_rr^-1 (r_r - sum A_rb z_b) = z0_r - sum Y_r J R_b z_b
     void ilu_apply(const double *r, double *z) {
         require(ilu_ready, "Refresh the block ILU(0) preconditioner first");
         require(r != z, "The block ILU(0) apply needs distinct input and output vectors");
@@ -512,8 +514,7 @@ struct dgh_ctx {
             HIPCHECK(launch_bj_apply_colour(m, factors.p + half * mm, rowsrc.p + static_cast<size_t>(half) * m,
                                             ilu_w.p, z, half, 1, cfg.nx, stream));
             HIPCHECK(launch_restrict(n, z, tr_x.p, basis.p, geo, stream));
-            HIPCHECK(launch_ilu_offdiag(n, 0, tr_x.p, fcache.p, r, ilu_w.p, basis.p, geo, stream));
-            HIPCHECK(launch_bj_apply_colour(m, factors.p, rowsrc.p, ilu_w.p, z, half, 0, cfg.nx, stream));
+            HIPCHECK(launch_ilu_red_back(n, ilu_Y.p, tr_x.p, fcache.p, ilu_z.p, z, geo, stream));
         });
     }
     void precond_refresh() {

@@ -190,11 +190,15 @@ hipError_t launch_mass_combine(int n, const double *lambda, const double *k, dou
                                const Basis *b, hipStream_t s);
 
 // Block ILU(0) of the DG element graph in red-black order (block_ilu.cpp:48-84
-// with the symmetric permutation red cells first). S holds, per red cell and
-// side s, the 4n x 4n coupling R_s A^-1 E_s of its factorised block (trace of
-// side s of the block solve against the side's lift columns).
-hipError_t launch_ilu_coupling(int n, const double *factors_red, const int *rowsrc_red, double *S, const Basis *b,
-                               const Geo &g, hipStream_t s);
+// with the symmetric permutation red cells first). Y holds, per red cell, the
+// block solve A^-1 E against its 16n lift columns (m x 16n) and S, per red
+// cell and side s, the 4n x 4n coupling R_s A^-1 E_s.
+hipError_t launch_ilu_coupling(int n, const double *factors_red, const int *rowsrc_red, double *Y, double *S,
+                               const Basis *b, const Geo &g, hipStream_t s);
+// z_r = z0_r - A_rr^-1 sum_b A_rb x_b for the red cells through Y = A_rr^-1 E
+// (x given by its side traces); black entries of z untouched
+hipError_t launch_ilu_red_back(int n, const double *Y, const double *traces, const double *fcache, const double *z0,
+                               double *z, const Geo &g, hipStream_t s);
 // A_bb -= sum over the four red neighbours of L_bj U_jb (rank-4n per side)
 hipError_t launch_ilu_update(int n, double *blocks_black, const double *S, const double *fcache, const Basis *b,
                              const Geo &g, hipStream_t s);

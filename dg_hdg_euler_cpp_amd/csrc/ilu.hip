@@ -78,16 +78,17 @@ __device__ __forceinline__ void trace_derivative(const double *__restrict__ fc, 
     for (int k = 0; k < 4; ++k) a[k * 5] += left ? pen : -pen;
 }
 
-// S[k][s] = R_s A^-1 E_s for the k-th red cell (A factorised in place by k_lu,
-// compact red storage). Column c = s*4n + v*n + q of E_s is the lift of a unit
-// face value (v, q) on side s: +right(a) (odd s) or -left(a) (even s) at rows
-// (v, face_node(s, q, a)). S[k][s] is column-major 4n x 4n with row w*n + q'
-// (trace of variable w at node q') and column v*n + q.
+// Y[k] = A^-1 E and S[k][s] = R_s A^-1 E_s for the k-th red cell (A factorised
+// in place by k_lu, compact red storage). Column c = s*4n + v*n + q of E is the
+// lift of a unit face value (v, q) on side s: +right(a) (odd s) or -left(a)
+// (even s) at rows (v, face_node(s, q, a)). Y[k] is column-major m x 16n;
+// S[k][s] is column-major 4n x 4n with row w*n + q' (trace of variable w at
+// node q') and column v*n + q.
 template <int N>
 __global__ __launch_bounds__(ILU_WAVES * 64) void k_ilu_coupling(const double *__restrict__ factors,
                                                                  const int *__restrict__ rowsrc,
-                                                                 double *__restrict__ S, int col0,
-                                                                 const Basis *__restrict__ B, Geo g) {
+                                                                 double *__restrict__ Y, double *__restrict__ S,
+                                                                 int col0, const Basis *__restrict__ B, Geo g) {
     constexpr int P = N * N, M = 4 * P, TS = 4 * N, T = 4 * TS, NA = (M + 63) / 64;
     __shared__ double colbuf[ILU_WAVES][M];
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
@@ -123,13 +124,16 @@ __global__ __launch_bounds__(ILU_WAVES * 64) void k_ilu_coupling(const double *_
     // through LDS so that lane r36 can sum the n values of its face line
 #pragma unroll
     for (int cc = 0; cc < ILU_COLS; ++cc) {
+        const int c = cols[cc];
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int row = a * 64 + lane;
-            if (row < M) colbuf[wave][row] = x[cc][a];
+            if (row < M) {
+                colbuf[wave][row] = x[cc][a];
+                if (c < T) Y[(k * T + c) * M + row] = x[cc][a];
+            }
         }
         __syncthreads();
-        const int c = cols[cc];
         if (c < T && lane < TS) {
             const int s = c / TS, w = lane / N, q = lane % N;
             const double *shape = s % 2 ? B->right : B->left;
@@ -204,6 +208,63 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_update(double *__restrict__ block
     }
 }
 
+// Back substitution of the red cells without a second block solve:
+// z_r = A_rr^-1 (r_r - sum_s A_{r,b_s} x_b) = z0_r - sum_s Y_r^s (J_s R_b x_b),
+// with z0_r = A_rr^-1 r_r from the forward sweep, Y_r = A_rr^-1 E_r and J_s the
+// derivative of the side-s face flux with respect to the black neighbour's
+// trace (given by tr). One wavefront per red cell; lane l owns rows l + 64a.
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_ilu_red_back(const double *__restrict__ Y, const double *__restrict__ tr,
+                                                        const double *__restrict__ fc, const double *__restrict__ z0,
+                                                        double *__restrict__ z, Geo g) {
+    constexpr int P = N * N, M = 4 * P, T = 16 * N, NA = (M + 63) / 64, NG = (T + 63) / 64;
+    const int lane = threadIdx.x % 64;
+    const long long k = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + threadIdx.x / 64;
+    if (k >= g.cells / 2) return;
+    const int cell = colour_cell(static_cast<int>(k), 0, g.nx);
+    const long long nodes = 2LL * g.cells * N;
+    // gv[b] holds column c = 64 b + lane of J t: side s, variable v, node q
+    double gv[NG];
+#pragma unroll
+    for (int b = 0; b < NG; ++b) {
+        const int c = 64 * b + lane;
+        double val = 0;
+        if (c < T) {
+            const int s = c / (4 * N), v = (c / N) % 4, q = c % N;
+            const int nb = neighbor(cell, s, g.nx, g.ny), face = cell_face(cell, s, g);
+            const bool nb_left = s % 2 == 0;
+            const long long fn = static_cast<long long>(face) * N + q;
+            double t[4], f[4];
+            for (int w = 0; w < 4; ++w) t[w] = tr[static_cast<long long>(nb) * 16 * N + (s ^ 1) * 4 * N + w * N + q];
+            flux_jacobian_action(fc[(nb_left ? F_UL : F_UR) * nodes + fn], fc[(nb_left ? F_VL : F_VR) * nodes + fn],
+                                 fc[(nb_left ? F_HL : F_HR) * nodes + fn], s / 2, g.gamma, t, f);
+            const double blend = fc[(nb_left ? F_BL : F_BR) * nodes + fn];
+            const double pen = nb_left ? fc[F_PEN * nodes + fn] : -fc[F_PEN * nodes + fn];
+            val = blend * f[v] + pen * t[v];
+        }
+        gv[b] = val;
+    }
+    double acc[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) acc[a] = 0;
+    const double *Yk = Y + k * T * M;
+#pragma unroll 4
+    for (int c = 0; c < T; ++c) {
+        const double gc = readlane_d(gv[c / 64], c % 64);
+        const double *col = Yk + static_cast<long long>(c) * M;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            const int row = a * 64 + lane;
+            if (row < M) acc[a] += __builtin_nontemporal_load(col + row) * gc;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        const int row = a * 64 + lane;
+        if (row < M) z[static_cast<long long>(cell) * M + row] = z0[static_cast<long long>(cell) * M + row] - acc[a];
+    }
+}
+
 // out = rhs - sum_s A_{cell, nb_s} x_nb for the cells of `colour`, with x given
 // by the restricted side traces tr[cell][side][v][q].
 template <int N>
@@ -257,12 +318,12 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_offdiag(int colour, const double 
     }                                                   \
     return hipGetLastError();
 
-hipError_t launch_ilu_coupling(int n, const double *factors_red, const int *rowsrc_red, double *S, const Basis *b,
-                               const Geo &g, hipStream_t s) {
+hipError_t launch_ilu_coupling(int n, const double *factors_red, const int *rowsrc_red, double *Y, double *S,
+                               const Basis *b, const Geo &g, hipStream_t s) {
     DGH_DISPATCH_N(n, {
         const int count = g.cells / 2, cols = 16 * N;
         for (int col0 = 0; col0 < cols; col0 += ILU_WAVES * ILU_COLS) {
-            k_ilu_coupling<N><<<count, ILU_WAVES * 64, 0, s>>>(factors_red, rowsrc_red, S, col0, b, g);
+            k_ilu_coupling<N><<<count, ILU_WAVES * 64, 0, s>>>(factors_red, rowsrc_red, Y, S, col0, b, g);
             const hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
         }
@@ -271,6 +332,12 @@ hipError_t launch_ilu_coupling(int n, const double *factors_red, const int *rows
 hipError_t launch_ilu_update(int n, double *blocks_black, const double *S, const double *fcache, const Basis *b,
                              const Geo &g, hipStream_t s) {
     DGH_DISPATCH_N(n, { k_ilu_update<N><<<g.cells / 2, BLOCK, 0, s>>>(blocks_black, S, fcache, b, g); })
+}
+hipError_t launch_ilu_red_back(int n, const double *Y, const double *traces, const double *fcache, const double *z0,
+                               double *z, const Geo &g, hipStream_t s) {
+    DGH_DISPATCH_N(n, {
+        k_ilu_red_back<N><<<(g.cells / 2 + BLOCK / 64 - 1) / (BLOCK / 64), BLOCK, 0, s>>>(Y, traces, fcache, z0, z, g);
+    })
 }
 hipError_t launch_ilu_offdiag(int n, int colour, const double *traces, const double *fcache, const double *rhs,
                               double *out, const Basis *b, const Geo &g, hipStream_t s) {
