@@ -216,6 +216,9 @@ struct dgh_ctx {
     // couplings S [red cell][side][4n x 4n]; apply work vectors
     DevBuf<double> ilu_Y, ilu_S, ilu_z, ilu_w;
     bool ilu_ready = false;
+    // mixed precision: FP32 factors in factors_f32 (same split), FP32 Y and work
+    DevBuf<float> ilu_Y32, ilu_z32, ilu_w32;
+    bool ilu32_ready = false;
     DevBuf<double> gH, gcs, gsn, gg, gnext;
     double *gout_host = nullptr, *gout_dev = nullptr;
     int ck_stages = 0;
@@ -288,8 +291,10 @@ struct dgh_ctx {
         basis.release(bytes);
         drop_step_graphs();
         if (gout_host) (void)hipHostFree(gout_host);
-        DevBuf<double> *gbufs[] = {&gH, &gcs, &gsn, &gg, &gnext};
+        DevBuf<double> *gbufs[] = {&gH, &gcs, &gsn, &gg, &gnext, &ilu_Y, &ilu_S, &ilu_z, &ilu_w};
         for (auto *b : gbufs) b->release(bytes);
+        DevBuf<float> *ibufs[] = {&ilu_Y32, &ilu_z32, &ilu_w32};
+        for (auto *b : ibufs) b->release(bytes);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -424,6 +429,7 @@ struct dgh_ctx {
         refreshed = true;
         bj_ready = false;
         ilu_ready = false;
+        ilu32_ready = false;
         adj_bj_ready = false;
         bj32_ready = false;
         hdg_ready = false;
@@ -517,9 +523,72 @@ struct dgh_ctx {
             HIPCHECK(launch_ilu_red_back(n, ilu_Y.p, tr_x.p, fcache.p, ilu_z.p, z, geo, stream));
         });
     }
+    // The same preconditioner with FP32 factors for the inner FP32 GMRES of the
+    // mixed-precision solve: blocks assembled from the FP64 linearisation into
+    // FP32, FP32 LU and block solves, couplings accumulated in FP64.
+    void ilu_refresh_mixed() {
+        require(refreshed, "Refresh the volume Jacobian before assembling a block");
+        require(cfg.nx % 2 == 0 && cfg.ny % 2 == 0 && cfg.nx >= 4 && cfg.ny >= 4,
+                "Block ILU(0) on the device runs in red-black order: nx and ny must be even and >= 4");
+        const int half = cells / 2;
+        const size_t mm = static_cast<size_t>(m) * m;
+        factors_f32.alloc(static_cast<size_t>(cells) * mm, bytes);
+        rowsrc32.alloc(static_cast<size_t>(cells) * m, bytes);
+        ilu_S.alloc(static_cast<size_t>(half) * 4 * 16 * n * n, bytes);
+        ilu_Y32.alloc(static_cast<size_t>(half) * 16 * n * m, bytes);
+        if (!ilu_z32.p) {
+            ilu_z32.alloc(D, bytes);
+            ilu_w32.alloc(D, bytes);
+            HIPCHECK(hipMemsetAsync(ilu_z32.p, 0, D * sizeof(float), stream));
+        }
+        bj32_ready = false;
+        ilu32_ready = false;
+        set_flag(2, 0);
+        timed("assemble", [&] {
+            HIPCHECK(launch_assemble_colour(n, vcache.p, fcache.p, factors_f32.p, inv_step, basis.p, geo, 0, stream));
+            HIPCHECK(launch_assemble_colour(n, vcache.p, fcache.p, factors_f32.p + half * mm, inv_step, basis.p, geo,
+                                            1, stream));
+        });
+        timed("lu", [&] { HIPCHECK(launch_lu(m, factors_f32.p, rowsrc32.p, half, flags.p + 2, stream)); });
+        timed("ilu_coupling", [&] {
+            HIPCHECK(launch_ilu_coupling(n, factors_f32.p, rowsrc32.p, ilu_Y32.p, ilu_S.p, basis.p, geo, stream));
+        });
+        timed("ilu_update", [&] {
+            HIPCHECK(launch_ilu_update(n, factors_f32.p + half * mm, ilu_S.p, fcache.p, basis.p, geo, stream));
+        });
+        timed("lu", [&] {
+            HIPCHECK(launch_lu(m, factors_f32.p + half * mm, rowsrc32.p + static_cast<size_t>(half) * m, half,
+                               flags.p + 2, stream));
+        });
+        require(read_flag(2) == 0, "Singular or nonfinite LU factors");
+        ilu32_ready = true;
+    }
+    void ilu_apply32(const float *r, float *z) {
+        require(ilu32_ready, "Refresh the block ILU(0) preconditioner first");
+        require(r != z, "The block ILU(0) apply needs distinct input and output vectors");
+        const int half = cells / 2;
+        const size_t mm = static_cast<size_t>(m) * m;
+        timed("ilu_apply", [&] {
+            HIPCHECK(launch_bj_apply_colour(m, factors_f32.p, rowsrc32.p, r, ilu_z32.p, half, 0, cfg.nx, stream));
+            HIPCHECK(launch_restrict(n, ilu_z32.p, tr_x.p, basis.p, geo, stream));
+            HIPCHECK(launch_ilu_offdiag(n, 1, tr_x.p, fcache.p, r, ilu_w32.p, basis.p, geo, stream));
+            HIPCHECK(launch_bj_apply_colour(m, factors_f32.p + half * mm, rowsrc32.p + static_cast<size_t>(half) * m,
+                                            ilu_w32.p, z, half, 1, cfg.nx, stream));
+            HIPCHECK(launch_restrict(n, z, tr_x.p, basis.p, geo, stream));
+            HIPCHECK(launch_ilu_red_back(n, ilu_Y32.p, tr_x.p, fcache.p, ilu_z32.p, z, geo, stream));
+        });
+    }
     void precond_refresh() {
         if (cfg.precond == 1) ilu_refresh();
         else bj_refresh();
+    }
+    void precond_refresh_mixed() {
+        if (cfg.precond == 1) ilu_refresh_mixed();
+        else bj_refresh_mixed();
+    }
+    void precond_apply32(const float *r, float *z) {
+        if (cfg.precond == 1) ilu_apply32(r, z);
+        else bj_apply32(r, z);
     }
     void precond_apply(const double *r, double *z) {
         if (cfg.precond == 1) ilu_apply(r, z);
@@ -808,6 +877,7 @@ struct dgh_ctx {
         });
         timed("lu", [&] { HIPCHECK(launch_lu(m, factors_f32.p, rowsrc32.p, cells, flags.p + 2, stream)); });
         require(read_flag(2) == 0, "Singular block Jacobi factor");
+        ilu32_ready = false;
         if (cfg.precision == DGH_PRECISION_MIXED_BF16) {
             factors_bf16.alloc(static_cast<size_t>(cells) * m * m, bytes);
             HIPCHECK(launch_to_bf16(factors_f32.p, factors_bf16.p, static_cast<long long>(cells) * m * m, stream));
@@ -881,7 +951,7 @@ struct dgh_ctx {
             int count = 0;
             bool done = false;
             for (int j = 0; j < restart && iterations < max_its; ++j) {
-                bj_apply32(Vk(j), it.p);
+                precond_apply32(Vk(j), it.p);
                 dg_apply32(it.p, iw.p);
                 timed("gmres_ortho", [&] {
                     HIPCHECK(launch_multi_dot(n, V.data(), j + 1, iw.p, nullptr, D, partials.p, basis.p, stream));
@@ -927,7 +997,7 @@ struct dgh_ctx {
             }
             HIPCHECK(hipMemcpyAsync(ky.p, y.data(), count * sizeof(double), hipMemcpyHostToDevice, stream));
             HIPCHECK(launch_combine(V.data(), count, ky.p, it.p, D, stream));
-            bj_apply32(it.p, is.p);
+            precond_apply32(it.p, is.p);
             HIPCHECK(launch_axpby(1.0, z, 1.0, is.p, z, D, stream));
             if (done || iterations >= max_its) break;
             dg_apply32(z, iw.p);
@@ -1147,7 +1217,7 @@ struct dgh_ctx {
                         precond_refresh();
                         linear = gmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance);
                     } else {
-                        bj_refresh_mixed();
+                        precond_refresh_mixed();
                         linear = fgmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance, outer);
                     }
                     axpby(-1.0, kres.p, 0.0, nullptr, equation_error.p, D);
@@ -1159,14 +1229,18 @@ struct dgh_ctx {
                         sync();
                         factors_f32.release(bytes);
                         factors_bf16.release(bytes);
+                        ilu_Y32.release(bytes);
                         bj32_ready = false;
+                        ilu32_ready = false;
                         check_dir.alloc(D, bytes);
-                        bj_refresh();
+                        precond_refresh();
                         gmres(rhs.p, check_dir.p, c.max_linear, c.linear_tolerance);
                         axpby(1.0, direction.p, -1.0, check_dir.p, kwork.p, D);
                         precision_error = state_norm(kwork.p) / state_norm(check_dir.p);
                         factors.release(bytes);
+                        ilu_Y.release(bytes);
                         bj_ready = false;
+                        ilu_ready = false;
                     }
                     }
                     const double error_norm = dual_norm(equation_error.p);

@@ -73,7 +73,8 @@ void validate_config(const dgh_config &c) {
         require(c.nx % 2 == 0 && c.ny % 2 == 0 && c.nx >= 4 && c.ny >= 4,
                 "Block ILU(0) on the device runs in red-black order: nx and ny must be even and >= 4");
         require(c.method == DGH_METHOD_DG, "Block ILU(0) on the device is built for --method dg");
-        require(c.precision == DGH_PRECISION_FP64, "Block ILU(0) on the device is built for --precision fp64");
+        require(c.precision != DGH_PRECISION_MIXED_BF16,
+                "Block ILU(0) on the device is built for --precision fp64 and mixed");
         require(!c.direct, "Block ILU(0) preconditions GMRES; --linear direct takes no preconditioner");
     } else {
         require(c.ordering != 2, "Red-black ordering applies to --precond block-ilu0");

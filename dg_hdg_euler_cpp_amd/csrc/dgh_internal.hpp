@@ -125,6 +125,8 @@ hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fca
 // colour's k-th cell.
 hipError_t launch_assemble_colour(int n, const double *vcache, const double *fcache, double *blocks,
                                   double inv_step, const Basis *b, const Geo &g, int colour, hipStream_t s);
+hipError_t launch_assemble_colour(int n, const double *vcache, const double *fcache, float *blocks,
+                                  double inv_step, const Basis *b, const Geo &g, int colour, hipStream_t s);
 hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s);
 hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s);
 hipError_t launch_bj_apply(int m, const double *factors, const int *rowsrc, const double *r, double *z,
@@ -134,6 +136,8 @@ hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const
 // Compact factors of one colour's cells; r and z in the full vector layout,
 // only that colour's cells read and written.
 hipError_t launch_bj_apply_colour(int m, const double *factors, const int *rowsrc, const double *r, double *z,
+                                  int nblocks, int colour, int nx, hipStream_t s);
+hipError_t launch_bj_apply_colour(int m, const float *factors, const int *rowsrc, const float *r, float *z,
                                   int nblocks, int colour, int nx, hipStream_t s);
 // factors: bfloat16 storage (FP32 arithmetic)
 hipError_t launch_bj_apply_bf16(int m, const void *factors, const int *rowsrc, const float *r, float *z,
@@ -195,17 +199,25 @@ hipError_t launch_mass_combine(int n, const double *lambda, const double *k, dou
 // cell and side s, the 4n x 4n coupling R_s A^-1 E_s.
 hipError_t launch_ilu_coupling(int n, const double *factors_red, const int *rowsrc_red, double *Y, double *S,
                                const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_ilu_coupling(int n, const float *factors_red, const int *rowsrc_red, float *Y, double *S,
+                               const Basis *b, const Geo &g, hipStream_t s);
 // z_r = z0_r - A_rr^-1 sum_b A_rb x_b for the red cells through Y = A_rr^-1 E
 // (x given by its side traces); black entries of z untouched
 hipError_t launch_ilu_red_back(int n, const double *Y, const double *traces, const double *fcache, const double *z0,
                                double *z, const Geo &g, hipStream_t s);
+hipError_t launch_ilu_red_back(int n, const float *Y, const double *traces, const double *fcache, const float *z0,
+                               float *z, const Geo &g, hipStream_t s);
 // A_bb -= sum over the four red neighbours of L_bj U_jb (rank-4n per side)
 hipError_t launch_ilu_update(int n, double *blocks_black, const double *S, const double *fcache, const Basis *b,
+                             const Geo &g, hipStream_t s);
+hipError_t launch_ilu_update(int n, float *blocks_black, const double *S, const double *fcache, const Basis *b,
                              const Geo &g, hipStream_t s);
 // out_c = rhs_c - sum over the four neighbours of A_{c,nb} x_nb for the cells of
 // one colour, with x given by its side traces
 hipError_t launch_ilu_offdiag(int n, int colour, const double *traces, const double *fcache, const double *rhs,
                               double *out, const Basis *b, const Geo &g, hipStream_t s);
+hipError_t launch_ilu_offdiag(int n, int colour, const double *traces, const double *fcache, const float *rhs,
+                              float *out, const Basis *b, const Geo &g, hipStream_t s);
 
 // Statically condensed HDG (hybrid.cpp:31-121).
 hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache, double *blocks, double inv_step,

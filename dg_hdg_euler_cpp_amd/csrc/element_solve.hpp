@@ -28,35 +28,38 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
     return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ float readlane_d(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
 __device__ __forceinline__ double face_weight(int face, int q, const Basis *__restrict__ B, const Geo &g) {
     return .5 * (face < g.cells ? g.hy : g.hx) * B->w[q];
 }
 
 // x <- U^-1 L^-1 x for COLS right-hand sides held in registers (lane l owns
 // rows l + 64a; the rows are already permuted by the pivots), with the M x M
-// column-major LU factors F of one element block (dense_lu.hpp:33-37).
-template <int M, int COLS, int RING>
-__device__ __forceinline__ void lu_solve_columns(const double *__restrict__ F, double (&x)[COLS][(M + 63) / 64],
-                                                 int lane) {
+// column-major LU factors F of one element block (dense_lu.hpp:33-37), in the
+// factors' precision T.
+template <int M, int COLS, int RING, typename T>
+__device__ __forceinline__ void lu_solve_columns(const T *__restrict__ F, T (&x)[COLS][(M + 63) / 64], int lane) {
     constexpr int NA = (M + 63) / 64;
     // The factor columns stream through a ring of RING buffers so the
     // loads of the next columns are in flight while a column is applied; they
     // are read once per pass, hence non-temporal.
-    double buf[RING][NA];
-    auto load_lower = [&](double(&b)[NA], int c, int a0) {  // rows > c, chunks >= a0
-        const double *colp = F + static_cast<long long>(c) * M;
+    T buf[RING][NA];
+    auto load_lower = [&](T(&b)[NA], int c, int a0) {  // rows > c, chunks >= a0
+        const T *colp = F + static_cast<long long>(c) * M;
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int row = a * 64 + lane;
-            if (a >= a0) b[a] = (row > c && row < M) ? __builtin_nontemporal_load(colp + row) : 0.0;
+            if (a >= a0) b[a] = (row > c && row < M) ? __builtin_nontemporal_load(colp + row) : T(0);
         }
     };
-    auto load_upper = [&](double(&b)[NA], int c, int a1) {  // rows <= c, chunks <= a1
-        const double *colp = F + static_cast<long long>(c) * M;
+    auto load_upper = [&](T(&b)[NA], int c, int a1) {  // rows <= c, chunks <= a1
+        const T *colp = F + static_cast<long long>(c) * M;
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int row = a * 64 + lane;
-            if (a <= a1) b[a] = row <= c ? __builtin_nontemporal_load(colp + row) : 0.0;
+            if (a <= a1) b[a] = row <= c ? __builtin_nontemporal_load(colp + row) : T(0);
         }
     };
     // forward substitution with the unit lower L (column j in ring slot j % RING)
@@ -74,7 +77,7 @@ __device__ __forceinline__ void lu_solve_columns(const double *__restrict__ F, d
                     const int j = ja * 64 + jl + u;
 #pragma unroll
                     for (int cc = 0; cc < COLS; ++cc) {
-                        const double xj = readlane_d(x[cc][ja], jl + u);
+                        const T xj = readlane_d(x[cc][ja], jl + u);
 #pragma unroll
                         for (int a = ja; a < NA; ++a) x[cc][a] -= buf[u][a] * xj;
                     }
@@ -98,11 +101,11 @@ __device__ __forceinline__ void lu_solve_columns(const double *__restrict__ F, d
                 const int jl = jend - 1 - g0 - u;
                 if (jl >= 0) {
                     const int j = ja * 64 + jl;
-                    double(&cur)[NA] = buf[(t0 + u) % RING];
-                    const double diag = readlane_d(cur[ja], jl);
+                    T(&cur)[NA] = buf[(t0 + u) % RING];
+                    const T diag = readlane_d(cur[ja], jl);
 #pragma unroll
                     for (int cc = 0; cc < COLS; ++cc) {
-                        const double xj = readlane_d(x[cc][ja], jl) / diag;
+                        const T xj = readlane_d(x[cc][ja], jl) / diag;
                         if (lane == jl) x[cc][ja] = xj;
 #pragma unroll
                         for (int a = 0; a <= ja; ++a) {

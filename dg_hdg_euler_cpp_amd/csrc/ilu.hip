@@ -83,19 +83,20 @@ __device__ __forceinline__ void trace_derivative(const double *__restrict__ fc, 
 // lift of a unit face value (v, q) on side s: +right(a) (odd s) or -left(a)
 // (even s) at rows (v, face_node(s, q, a)). Y[k] is column-major m x 16n;
 // S[k][s] is column-major 4n x 4n with row w*n + q' (trace of variable w at
-// node q') and column v*n + q.
-template <int N>
-__global__ __launch_bounds__(ILU_WAVES * 64) void k_ilu_coupling(const double *__restrict__ factors,
+// node q') and column v*n + q. TF is the factors' precision (FP64, or FP32
+// for the mixed-precision preconditioner); Y is stored in it.
+template <int N, typename TF>
+__global__ __launch_bounds__(ILU_WAVES * 64) void k_ilu_coupling(const TF *__restrict__ factors,
                                                                  const int *__restrict__ rowsrc,
-                                                                 double *__restrict__ Y, double *__restrict__ S,
+                                                                 TF *__restrict__ Y, double *__restrict__ S,
                                                                  int col0, const Basis *__restrict__ B, Geo g) {
     constexpr int P = N * N, M = 4 * P, TS = 4 * N, T = 4 * TS, NA = (M + 63) / 64;
-    __shared__ double colbuf[ILU_WAVES][M];
+    __shared__ TF colbuf[ILU_WAVES][M];
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const long long k = blockIdx.x;
-    const double *F = factors + k * M * M;
+    const TF *F = factors + k * M * M;
     const int *perm = rowsrc + k * M;
-    double x[ILU_COLS][NA];
+    TF x[ILU_COLS][NA];
     int cols[ILU_COLS];
 #pragma unroll
     for (int cc = 0; cc < ILU_COLS; ++cc) {
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(ILU_WAVES * 64) void k_ilu_coupling(const double *_
                     if (s >= 2 && in == q) val = sign * shape[jn];
                 }
             }
-            x[cc][a] = val;
+            x[cc][a] = static_cast<TF>(val);
         }
     }
     lu_solve_columns<M, ILU_COLS, ILU_RING>(F, x, lane);
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(ILU_WAVES * 64) void k_ilu_coupling(const double *_
             const double *shape = s % 2 ? B->right : B->left;
             double acc = 0;
 #pragma unroll
-            for (int a = 0; a < N; ++a) acc += shape[a] * colbuf[wave][w * P + face_node(s, q, a, N)];
+            for (int a = 0; a < N; ++a) acc += shape[a] * static_cast<double>(colbuf[wave][w * P + face_node(s, q, a, N)]);
             S[(k * 4 + s) * TS * TS + lane + static_cast<long long>(TS) * (c % TS)] = acc;
         }
         __syncthreads();
@@ -150,8 +151,8 @@ __global__ __launch_bounds__(ILU_WAVES * 64) void k_ilu_coupling(const double *_
 // (compact black storage; nb = the red neighbour on side s, its facing side
 // s ^ 1). J1 = derivative with respect to the neighbour's trace and J2 with
 // respect to the cell's own trace at the shared face.
-template <int N>
-__global__ __launch_bounds__(BLOCK) void k_ilu_update(double *__restrict__ blocks, const double *__restrict__ S,
+template <int N, typename T>
+__global__ __launch_bounds__(BLOCK) void k_ilu_update(T *__restrict__ blocks, const double *__restrict__ S,
                                                       const double *__restrict__ fc, const Basis *__restrict__ B,
                                                       Geo g) {
     constexpr int P = N * N, M = 4 * P, TS = 4 * N;
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_update(double *__restrict__ block
         }
         __syncthreads();
     }
-    double *A = blocks + k * M * M;
+    T *A = blocks + k * M * M;
     for (int col = 0; col < M; ++col) {
         const int w = col / P, cnode = col % P, ci = cnode % N, cj = cnode / N;
         for (int row = threadIdx.x; row < M; row += BLOCK) {
@@ -203,7 +204,8 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_update(double *__restrict__ block
                 const double rr = s % 2 ? B->right[ca] : B->left[ca];
                 delta += e * Tm[s][v * N + rq + TS * (w * N + cq)] * rr;
             }
-            A[static_cast<long long>(col) * M + row] -= delta;
+            T &entry = A[static_cast<long long>(col) * M + row];
+            entry = static_cast<T>(static_cast<double>(entry) - delta);
         }
     }
 }
@@ -213,10 +215,10 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_update(double *__restrict__ block
 // with z0_r = A_rr^-1 r_r from the forward sweep, Y_r = A_rr^-1 E_r and J_s the
 // derivative of the side-s face flux with respect to the black neighbour's
 // trace (given by tr). One wavefront per red cell; lane l owns rows l + 64a.
-template <int N>
-__global__ __launch_bounds__(BLOCK) void k_ilu_red_back(const double *__restrict__ Y, const double *__restrict__ tr,
-                                                        const double *__restrict__ fc, const double *__restrict__ z0,
-                                                        double *__restrict__ z, Geo g) {
+template <int N, typename TY, typename TV>
+__global__ __launch_bounds__(BLOCK) void k_ilu_red_back(const TY *__restrict__ Y, const double *__restrict__ tr,
+                                                        const double *__restrict__ fc, const TV *__restrict__ z0,
+                                                        TV *__restrict__ z, Geo g) {
     constexpr int P = N * N, M = 4 * P, T = 16 * N, NA = (M + 63) / 64, NG = (T + 63) / 64;
     const int lane = threadIdx.x % 64;
     const long long k = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + threadIdx.x / 64;
@@ -247,30 +249,32 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_red_back(const double *__restrict
     double acc[NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a) acc[a] = 0;
-    const double *Yk = Y + k * T * M;
+    const TY *Yk = Y + k * T * M;
 #pragma unroll 4
     for (int c = 0; c < T; ++c) {
         const double gc = readlane_d(gv[c / 64], c % 64);
-        const double *col = Yk + static_cast<long long>(c) * M;
+        const TY *col = Yk + static_cast<long long>(c) * M;
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int row = a * 64 + lane;
-            if (row < M) acc[a] += __builtin_nontemporal_load(col + row) * gc;
+            if (row < M) acc[a] += static_cast<double>(__builtin_nontemporal_load(col + row)) * gc;
         }
     }
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
         const int row = a * 64 + lane;
-        if (row < M) z[static_cast<long long>(cell) * M + row] = z0[static_cast<long long>(cell) * M + row] - acc[a];
+        if (row < M)
+            z[static_cast<long long>(cell) * M + row] =
+                static_cast<TV>(static_cast<double>(z0[static_cast<long long>(cell) * M + row]) - acc[a]);
     }
 }
 
 // out = rhs - sum_s A_{cell, nb_s} x_nb for the cells of `colour`, with x given
 // by the restricted side traces tr[cell][side][v][q].
-template <int N>
+template <int N, typename TV>
 __global__ __launch_bounds__(BLOCK) void k_ilu_offdiag(int colour, const double *__restrict__ tr,
-                                                       const double *__restrict__ fc, const double *__restrict__ rhs,
-                                                       double *__restrict__ out, const Basis *__restrict__ B,
+                                                       const double *__restrict__ fc, const TV *__restrict__ rhs,
+                                                       TV *__restrict__ out, const Basis *__restrict__ B,
                                                        Geo g) {
     constexpr int P = N * N, M = 4 * P, CPB = BLOCK / P > 0 ? BLOCK / P : 1;
     const int local = threadIdx.x / P, node = threadIdx.x % P;
@@ -280,7 +284,7 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_offdiag(int colour, const double 
     const int i = node % N, j = node / N;
     const long long nodes = 2LL * g.cells * N;
     double res[4];
-    for (int v = 0; v < 4; ++v) res[v] = rhs[static_cast<long long>(cell) * M + v * P + node];
+    for (int v = 0; v < 4; ++v) res[v] = static_cast<double>(rhs[static_cast<long long>(cell) * M + v * P + node]);
 #pragma unroll 1
     for (int s = 0; s < 4; ++s) {
         const int nb = neighbor(cell, s, g.nx, g.ny), face = cell_face(cell, s, g);
@@ -298,7 +302,7 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_offdiag(int colour, const double 
         const double e = nb_left ? -B->left[a] : B->right[a];
         for (int v = 0; v < 4; ++v) res[v] -= e * (blend * f[v] + pen * t[v]);
     }
-    for (int v = 0; v < 4; ++v) out[static_cast<long long>(cell) * M + v * P + node] = res[v];
+    for (int v = 0; v < 4; ++v) out[static_cast<long long>(cell) * M + v * P + node] = static_cast<TV>(res[v]);
 }
 
 }  // namespace
@@ -318,33 +322,70 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_offdiag(int colour, const double 
     }                                                   \
     return hipGetLastError();
 
-hipError_t launch_ilu_coupling(int n, const double *factors_red, const int *rowsrc_red, double *Y, double *S,
-                               const Basis *b, const Geo &g, hipStream_t s) {
+template <typename TF>
+static hipError_t ilu_coupling_any(int n, const TF *factors_red, const int *rowsrc_red, TF *Y, double *S,
+                                   const Basis *b, const Geo &g, hipStream_t s) {
     DGH_DISPATCH_N(n, {
         const int count = g.cells / 2, cols = 16 * N;
         for (int col0 = 0; col0 < cols; col0 += ILU_WAVES * ILU_COLS) {
-            k_ilu_coupling<N><<<count, ILU_WAVES * 64, 0, s>>>(factors_red, rowsrc_red, Y, S, col0, b, g);
+            k_ilu_coupling<N, TF><<<count, ILU_WAVES * 64, 0, s>>>(factors_red, rowsrc_red, Y, S, col0, b, g);
             const hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
         }
     })
 }
+hipError_t launch_ilu_coupling(int n, const double *factors_red, const int *rowsrc_red, double *Y, double *S,
+                               const Basis *b, const Geo &g, hipStream_t s) {
+    return ilu_coupling_any(n, factors_red, rowsrc_red, Y, S, b, g, s);
+}
+hipError_t launch_ilu_coupling(int n, const float *factors_red, const int *rowsrc_red, float *Y, double *S,
+                               const Basis *b, const Geo &g, hipStream_t s) {
+    return ilu_coupling_any(n, factors_red, rowsrc_red, Y, S, b, g, s);
+}
+template <typename T>
+static hipError_t ilu_update_any(int n, T *blocks_black, const double *S, const double *fcache, const Basis *b,
+                                 const Geo &g, hipStream_t s) {
+    DGH_DISPATCH_N(n, { k_ilu_update<N, T><<<g.cells / 2, BLOCK, 0, s>>>(blocks_black, S, fcache, b, g); })
+}
 hipError_t launch_ilu_update(int n, double *blocks_black, const double *S, const double *fcache, const Basis *b,
                              const Geo &g, hipStream_t s) {
-    DGH_DISPATCH_N(n, { k_ilu_update<N><<<g.cells / 2, BLOCK, 0, s>>>(blocks_black, S, fcache, b, g); })
+    return ilu_update_any(n, blocks_black, S, fcache, b, g, s);
+}
+hipError_t launch_ilu_update(int n, float *blocks_black, const double *S, const double *fcache, const Basis *b,
+                             const Geo &g, hipStream_t s) {
+    return ilu_update_any(n, blocks_black, S, fcache, b, g, s);
+}
+template <typename TY, typename TV>
+static hipError_t ilu_red_back_any(int n, const TY *Y, const double *traces, const double *fcache, const TV *z0,
+                                   TV *z, const Geo &g, hipStream_t s) {
+    DGH_DISPATCH_N(n, {
+        k_ilu_red_back<N, TY, TV><<<(g.cells / 2 + BLOCK / 64 - 1) / (BLOCK / 64), BLOCK, 0, s>>>(Y, traces, fcache,
+                                                                                              z0, z, g);
+    })
 }
 hipError_t launch_ilu_red_back(int n, const double *Y, const double *traces, const double *fcache, const double *z0,
                                double *z, const Geo &g, hipStream_t s) {
+    return ilu_red_back_any(n, Y, traces, fcache, z0, z, g, s);
+}
+hipError_t launch_ilu_red_back(int n, const float *Y, const double *traces, const double *fcache, const float *z0,
+                               float *z, const Geo &g, hipStream_t s) {
+    return ilu_red_back_any(n, Y, traces, fcache, z0, z, g, s);
+}
+template <typename TV>
+static hipError_t ilu_offdiag_any(int n, int colour, const double *traces, const double *fcache, const TV *rhs,
+                                  TV *out, const Basis *b, const Geo &g, hipStream_t s) {
     DGH_DISPATCH_N(n, {
-        k_ilu_red_back<N><<<(g.cells / 2 + BLOCK / 64 - 1) / (BLOCK / 64), BLOCK, 0, s>>>(Y, traces, fcache, z0, z, g);
+        constexpr int CPB = BLOCK / (N * N) > 0 ? BLOCK / (N * N) : 1;
+        k_ilu_offdiag<N, TV><<<(g.cells / 2 + CPB - 1) / CPB, BLOCK, 0, s>>>(colour, traces, fcache, rhs, out, b, g);
     })
 }
 hipError_t launch_ilu_offdiag(int n, int colour, const double *traces, const double *fcache, const double *rhs,
                               double *out, const Basis *b, const Geo &g, hipStream_t s) {
-    DGH_DISPATCH_N(n, {
-        constexpr int CPB = BLOCK / (N * N) > 0 ? BLOCK / (N * N) : 1;
-        k_ilu_offdiag<N><<<(g.cells / 2 + CPB - 1) / CPB, BLOCK, 0, s>>>(colour, traces, fcache, rhs, out, b, g);
-    })
+    return ilu_offdiag_any(n, colour, traces, fcache, rhs, out, b, g, s);
+}
+hipError_t launch_ilu_offdiag(int n, int colour, const double *traces, const double *fcache, const float *rhs,
+                              float *out, const Basis *b, const Geo &g, hipStream_t s) {
+    return ilu_offdiag_any(n, colour, traces, fcache, rhs, out, b, g, s);
 }
 
 }  // namespace dgh

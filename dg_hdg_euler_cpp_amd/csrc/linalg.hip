@@ -1027,13 +1027,22 @@ hipError_t launch_assemble_blocks(int n, const double *vcache, const double *fca
         else k_assemble<N, float, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
     })
 }
-hipError_t launch_assemble_colour(int n, const double *vcache, const double *fcache, double *blocks,
-                                  double inv_step, const Basis *b, const Geo &g, int colour, hipStream_t s) {
+template <typename T>
+static hipError_t assemble_colour_any(int n, const double *vcache, const double *fcache, T *blocks, double inv_step,
+                                      const Basis *b, const Geo &g, int colour, hipStream_t s) {
     const int count = g.cells / 2;
     DGH_DISPATCH_N(n, {
-        if (g.nq > N) k_assemble_oi<N, double, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, 0, colour);
-        else k_assemble<N, double, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, 0, colour);
+        if (g.nq > N) k_assemble_oi<N, T, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, 0, colour);
+        else k_assemble<N, T, false><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, 0, colour);
     })
+}
+hipError_t launch_assemble_colour(int n, const double *vcache, const double *fcache, double *blocks,
+                                  double inv_step, const Basis *b, const Geo &g, int colour, hipStream_t s) {
+    return assemble_colour_any(n, vcache, fcache, blocks, inv_step, b, g, colour, s);
+}
+hipError_t launch_assemble_colour(int n, const double *vcache, const double *fcache, float *blocks,
+                                  double inv_step, const Basis *b, const Geo &g, int colour, hipStream_t s) {
+    return assemble_colour_any(n, vcache, fcache, blocks, inv_step, b, g, colour, s);
 }
 hipError_t launch_assemble_blocks_transposed(int n, const double *vcache, const double *fcache, double *blocks,
                                              double inv_step, const Basis *b, const Geo &g, hipStream_t s) {
@@ -1087,6 +1096,12 @@ hipError_t launch_bj_apply_colour(int m, const double *factors, const int *rowsr
                                   int nblocks, int colour, int nx, hipStream_t s) {
     DGH_DISPATCH_M(m, {
         k_bj_apply<M, double, double, BJ_RING64><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks, colour, nx);
+    })
+}
+hipError_t launch_bj_apply_colour(int m, const float *factors, const int *rowsrc, const float *r, float *z,
+                                  int nblocks, int colour, int nx, hipStream_t s) {
+    DGH_DISPATCH_M(m, {
+        k_bj_apply<M, float, float, BJ_RING32><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks, colour, nx);
     })
 }
 hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const float *r, float *z, int nblocks,

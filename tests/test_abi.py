@@ -75,8 +75,8 @@ def test_config_struct_layout_and_defaults():
     (["--precond", "block-ilu0", "--ordering", "redblack", "--method", "dg", "--nx", "5"],
      "Block ILU(0) on the device runs in red-black order: nx and ny must be even and >= 4"),
     (["--precond", "block-ilu0", "--ordering", "redblack"], "Block ILU(0) on the device is built for --method dg"),
-    (["--precond", "block-ilu0", "--ordering", "redblack", "--method", "dg", "--precision", "mixed"],
-     "Block ILU(0) on the device is built for --precision fp64"),
+    (["--precond", "block-ilu0", "--ordering", "redblack", "--method", "dg", "--precision", "mixed-bf16"],
+     "Block ILU(0) on the device is built for --precision fp64 and mixed"),
     (["--ordering", "redblack"], "Red-black ordering applies to --precond block-ilu0"),
     (["--linear", "lu"], "Invalid linear solver"),
     (["--integrator", "rk4"], "Invalid integrator"),

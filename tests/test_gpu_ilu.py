@@ -135,3 +135,56 @@ def test_ilu_k8_256_first_update(torch_cuda):
         # the Newton iterates differ only by the linear solve tolerance
         assert abs(got["residual_physical_rms"] - w) <= [1e-13, 1e-6][it] * w
         assert got["krylov_iterations"] < int(ref[it]["krylov_iterations"])
+
+
+def test_mixed_ilu_runs_match_reference(torch_cuda):
+    """FP32 ILU(0) factors and inner FP32 GMRES inside the FP64 flexible GMRES:
+    the reference's archived 4x4, k=2 BE run (be_4x4_k2) update by update."""
+    import json
+    import os
+
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_runs.json")))
+    run = gold["runs"]["be_4x4_k2"]
+    # the later --method dg of ILU overrides the run's --method both
+    with DeviceContext(run["args"] + ILU + ["--precision", "mixed"]) as dev:
+        u = dev.initial()
+        st = dev.advance(u)
+        err = dev.error_l2(u, st["final_time"])
+    ref = [r for r in run["iterations"] if r["method"] == "dg"]
+    first = float(ref[0]["residual_physical_rms"])
+    tol = [1e-13, 1e-10, 1e-6]
+    assert len(st["updates"]) == len(ref)
+    for got, want in zip(st["updates"], ref):
+        it = int(want["iteration"])
+        w = float(want["residual_physical_rms"])
+        assert abs(got["residual_physical_rms"] - w) <= max(tol[it] * w, 1e-12 * first)
+        assert got["linear_relative_residual"] <= 1e-12 and got["outer_iterations"] > 0
+    summ = [s for s in run["summary"] if s["method"] == "dg"][0]
+    for v, k in enumerate(["rho_L2", "momentum_x_L2", "momentum_y_L2", "energy_L2"]):
+        assert abs(err[v] - float(summ[k])) <= 1e-9 * float(summ[k])
+
+
+def test_mixed_ilu_update_error_vs_fp64_256(torch_cuda):
+    """256^2, k=8: two BE updates with the FP32 ILU(0) inner solve; each
+    direction against the FP64 ILU(0) solution of the same system."""
+    import json
+    import os
+
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_runs.json")))
+    ref = gold["runs"]["k8_256_be"]["iterations"]
+    args = ["--nx", "256", "--ny", "256", "--k", "8", "--integrator", "be", "--steps", "1", "--newton-steps", "2",
+            "--cfl", "5", "--ntol", "1e-12", "--ltol", "1e-12", "--restart", "20", "--max-linear", "1500",
+            "--safety", "1.1", "--timing-only", "--precision", "mixed", "--check-update"] + ILU
+    with DeviceContext(args) as dev:
+        u = dev.initial()
+        st = dev.advance(u)
+    for it, got in enumerate(st["updates"]):
+        w = float(ref[it]["residual_physical_rms"])
+        print(f"update {it}: rms {got['residual_physical_rms']:.10e}, inner Krylov {got['krylov_iterations']}, "
+              f"outer {got['outer_iterations']}, error vs FP64 {got['precision_error']:.3e}", flush=True)
+        assert abs(got["residual_physical_rms"] - w) <= [1e-13, 1e-6][it] * w
+        assert 0 <= got["precision_error"] <= 1e-9, got["precision_error"]
