@@ -61,14 +61,15 @@ def run_hdg(args, nx, k, init):
             "rms": [x["residual_physical_rms"] for x in upd]}
 
 
-def run_ilu(args, nx, k, init):
-    """Block ILU(0) in red-black order (FP64) at the same workload: Newton
-    updates/s, Krylov iterations, and the preconditioner's refresh and apply
-    times (HIP events on the library's stream)."""
+def run_ilu(args, nx, k, init, precision="fp64"):
+    """Block ILU(0) in red-black order at the same workload (FP64, or FP32
+    factors inside the mixed-precision solve): Newton updates/s, Krylov
+    iterations, and the preconditioner's refresh and apply times (HIP events on
+    the library's stream)."""
     from dg_hdg_euler_cpp_amd import DeviceContext
 
     ilu = ["--precond", "block-ilu0", "--ordering", "redblack"]
-    with DeviceContext(solver_args(nx, k, init, "fp64", args.warmup + args.steps, args.warmup) + ilu) as dev:
+    with DeviceContext(solver_args(nx, k, init, precision, args.warmup + args.steps, args.warmup) + ilu) as dev:
         u = dev.initial()
         dev.set_timing(True)
         dev.reset_kernel_times()
@@ -80,7 +81,8 @@ def run_ilu(args, nx, k, init):
         stats_untimed = dev.advance(u2)
     measured = stats_untimed["measured_newton_seconds"]
     upd = stats_untimed["updates"]
-    log(f"[bench] ilu0 fp64 {init}: updates={len(upd)} mean_s={measured / max(1, stats_untimed['measured_updates']):.4f}")
+    log(f"[bench] ilu0 {precision} {init}: updates={len(upd)} "
+        f"mean_s={measured / max(1, stats_untimed['measured_updates']):.4f}")
     for x in upd:
         log(f"   update {x['iteration']}: rms={x['residual_physical_rms']:.6e} krylov={x['krylov_iterations']} "
             f"t={x['iteration_s']:.4f}s")
@@ -321,6 +323,8 @@ def main():
                  "speedup_mixed_over_fp64": forms_out["mixed"]["updates_per_s"] / forms_out["fp64"]["updates_per_s"]}
     hdg = None if args.no_hdg or args.precision != "both" else run_hdg(args, args.nx, args.k, args.init)
     ilu = None if args.no_ilu or args.precision != "both" else run_ilu(args, args.nx, args.k, args.init)
+    ilu_mixed = None if args.no_ilu or args.precision != "both" else run_ilu(args, args.nx, args.k, args.init,
+                                                                              "mixed")
     check = None if args.no_check or head == "fp64" else run_precision_check(args, args.nx, args.k, args.init, head)
     cpu = None
     if not args.no_cpu_baseline:
@@ -350,6 +354,7 @@ def main():
         "update_error_vs_fp64": check,
         "hdg_fp64": hdg,
         "ilu0_redblack_fp64": ilu,
+        "ilu0_redblack_mixed": ilu_mixed,
         "jv": {"dof_per_s": g["size"] / dev["jv_avg_s"], "achieved_GBs": jv_bytes(g) / dev["jv_avg_s"] / 1e9,
                "avg_launch_s": dev["jv_avg_s"]},
         "cpu_baseline": None if cpu is None else {k: v for k, v in cpu.items() if k != "seconds_per_update"},
