@@ -247,7 +247,92 @@ def probe(args):
             dev.bj_apply_f32(x32, y32)
         torch.cuda.synchronize()
         log("[probe] bj_apply fp32 x3")
+        dev.ilu_refresh()
+        for _ in range(3):
+            dev.ilu_apply(x, y)
+        torch.cuda.synchronize()
+        log("[probe] ilu_apply fp64 x3")
     print(json.dumps({"probe": "done", "nx": args.nx, "k": args.k}), flush=True)
+
+
+def summarize_counters(fetch_csv, write_csv, out_json):
+    """HBM bytes per launch of every kernel from two rocprofv3 counter runs of
+    `bench.py --probe` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), scaled by
+    what the same counters read for the probe's 1 GiB device-to-device copy.
+    Prints a table and writes the per-kernel figures to out_json."""
+    import collections
+    import csv
+    import re
+
+    def load(path, counter):
+        # per (kernel, grid size): the block ILU(0) launches the block kernels
+        # on one colour (half the grid) beside the full block-Jacobi launches
+        agg = collections.defaultdict(list)
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    agg[(r["Kernel_Name"], int(r["Grid_Size"]))].append(float(r["Counter_Value"]) * 1024.0)
+        return agg
+
+    fetch, write = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    gib = float(1 << 30)
+    copy_f = [v for (k, _), vs in fetch.items() if k == "__amd_rocclr_copyBuffer" for v in vs if v > 0.4 * gib]
+    copy_w = [v for (k, _), vs in write.items() if k == "__amd_rocclr_copyBuffer" for v in vs if v > 0.4 * gib]
+    sf, sw = gib / (sum(copy_f) / len(copy_f)), gib / (sum(copy_w) / len(copy_w))
+    cells, m = 256 * 256, 324
+    d = cells * m
+    alg = {"k_bj_apply<324, double": cells * m * m * 8 + cells * m * 4 + 2 * d * 8,
+           "k_bj_apply<324, float": cells * m * m * 4 + cells * m * 4 + 2 * d * 4}
+    out = {"fetch_scale": sf, "write_scale": sw, "kernels": {}}
+    print(f"# copy kernel (1 GiB read + 1 GiB write): FETCH_SIZE reads {1 / sf:.4f} GiB, WRITE_SIZE {1 / sw:.4f} GiB")
+    print(f"# scale factors: fetch x{sf:.4f}, write x{sw:.4f}")
+    print(f"# {'kernel [grid threads]':72s} {'fetch GB':>10s} {'write GB':>10s} {'total GB':>10s} "
+          f"{'algorithmic GB':>15s} {'ratio':>6s}")
+    full = {}
+    for (name, grid) in fetch:
+        if name.startswith("void dgh::"):
+            full[name] = max(full.get(name, 0), grid)
+    for (name, grid) in fetch:
+        if not name.startswith("void dgh::"):
+            continue
+        f = sum(fetch[(name, grid)]) / len(fetch[(name, grid)]) * sf
+        ws = write.get((name, grid), [0.0])
+        w = sum(ws) / len(ws) * sw
+        short = re.sub(r"^void dgh::\(anonymous namespace\)::", "", name)
+        short = short[:short.index("(")] if "(" in short else short
+        a = next((v for k, v in alg.items() if short.startswith(k)), None) if grid == full[name] else None
+        key = short if grid == full[name] else f"{short} [one colour]"
+        out["kernels"][key] = {"grid_threads": grid, "fetch_bytes": f, "write_bytes": w,
+                               "hbm_bytes_per_launch": f + w, "algorithmic_bytes": a}
+        extra = "" if a is None else f"{a / 1e9:15.3f} {(f + w) / a:6.3f}"
+        print(f"  {(key + f' [{grid}]')[:72]:72s} {f / 1e9:10.3f} {w / 1e9:10.3f} {(f + w) / 1e9:10.3f} {extra}")
+    with open(out_json, "w") as f:
+        json.dump(out, f, indent=1)
+
+
+def summarize_kernel_stats(stats_csv, top=14):
+    """Per-kernel time table from a rocprofv3 --stats kernel_stats.csv
+    (template instances of the reduction kernels summed)."""
+    import collections
+    import csv
+    import re
+
+    with open(stats_csv) as f:
+        rows = list(csv.DictReader(f))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    fam = collections.defaultdict(lambda: [0, 0.0])
+    for r in rows:
+        mt = re.match(r"(?:void )?(?:dgh::\(anonymous namespace\)::)?([A-Za-z_0-9:]+)(<[^(]*>)?", r["Name"])
+        key = mt.group(1) if mt else r["Name"][:40]
+        if mt and mt.group(2) and not key.startswith(("k_mdot", "k_mupdate")):
+            key += mt.group(2)
+        fam[key][0] += int(r["Calls"])
+        fam[key][1] += float(r["TotalDurationNs"])
+    print(f"# {'kernel (template instances of k_mdot / k_mupdate summed)':78s} {'calls':>6s} {'total ms':>10s} "
+          f"{'avg us':>10s} {'share':>7s}")
+    for k, (c, t) in sorted(fam.items(), key=lambda kv: -kv[1][1])[:top]:
+        print(f"  {k[:78]:78s} {c:6d} {t / 1e6:10.2f} {t / 1e3 / c:10.1f} {100 * t / tot:6.2f}%")
+    print(f"# total kernel time {tot / 1e6:.1f} ms")
 
 
 def main():
@@ -266,7 +351,16 @@ def main():
     ap.add_argument("--only-hdg", action="store_true", help="run and report the condensed HDG form alone")
     ap.add_argument("--no-ilu", action="store_true", help="skip the block ILU(0) form")
     ap.add_argument("--probe", action="store_true", help="run each hot kernel 3x for counter profiles")
+    ap.add_argument("--counters", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
+                    help="summarise two rocprofv3 counter runs of --probe into HBM bytes per launch")
+    ap.add_argument("--kernel-stats", metavar="STATS_CSV", help="summarise a rocprofv3 kernel_stats.csv")
     args = ap.parse_args()
+    if args.counters:
+        summarize_counters(*args.counters)
+        return
+    if args.kernel_stats:
+        summarize_kernel_stats(args.kernel_stats)
+        return
     global GRAPH
     GRAPH = args.graph
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
