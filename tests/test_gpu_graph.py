@@ -64,3 +64,28 @@ def test_graph_newton_updates_match(torch_cuda):
     print("rms", a, b)
     for k, (x, y) in enumerate(zip(a, b)):
         assert abs(x - y) <= [1e-13, 1e-10, 1e-6][k] * x + 1e-12 * a[0]
+
+
+def test_graph_newton_updates_with_ilu_match(torch_cuda):
+    """The captured Arnoldi step with the block ILU(0) apply (two one-colour
+    block solves, restrictions, the off-diagonal and red back-substitution
+    kernels) against the host-driven loop, whole Newton updates."""
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    args = ["--nx", "32", "--ny", "32", "--k", "8", "--method", "dg", "--integrator", "be", "--cfl", "5",
+            "--steps", "1", "--newton-steps", "3", "--restart", "20", "--precond", "block-ilu0",
+            "--ordering", "redblack"]
+    runs = []
+    for graph in (False, True):
+        print(f"\n[case] ilu graph={graph}", flush=True)
+        with DeviceContext(args + (["--graph"] if graph else [])) as dev:
+            u = dev.initial()
+            runs.append(dev.advance(u))
+    a = [x["residual_physical_rms"] for x in runs[0]["updates"]]
+    b = [x["residual_physical_rms"] for x in runs[1]["updates"]]
+    ka = [x["krylov_iterations"] for x in runs[0]["updates"]]
+    kb = [x["krylov_iterations"] for x in runs[1]["updates"]]
+    print("rms", a, b, "krylov", ka, kb)
+    for k, (x, y) in enumerate(zip(a, b)):
+        assert abs(x - y) <= [1e-13, 1e-10, 1e-6][k] * x + 1e-12 * a[0]
+    assert all(abs(x - y) <= 1 for x, y in zip(ka, kb))
