@@ -118,3 +118,30 @@ def test_mixed_update_error_vs_fp64_256(torch_cuda):
         w = float(run["iterations"][it]["residual_physical_rms"])
         assert abs(got["residual_physical_rms"] - w) <= [1e-13, 1e-10][it] * w
         assert 0 <= got["precision_error"] <= 1e-9, got["precision_error"]
+
+
+def test_mixed_512_config(torch_cuda):
+    """512^2, k=8 (the mixed-precision configuration): two BE Newton updates with
+    FP32 factors and the FP32 inner GMRES(30) inside FP64 FGMRES(20); the RMS
+    sequence of the reference's 512^2 run (k8_512_be) and, per update, the
+    direction's error against the FP64 path (FP32 factors released before the
+    FP64 factors are built: 111 GB, then 221 GB)."""
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    run = GOLD["runs"]["k8_512_be"]
+    # the reference's speed-run command line (speed_test/2d/benchmark.py:26-36) at 512^2
+    args = ["--nx", "512", "--ny", "512", "--k", "8", "--method", "dg", "--integrator", "be", "--steps", "1",
+            "--newton-steps", "2", "--cfl", "5", "--ntol", "1e-12", "--ltol", "1e-12", "--restart", "20",
+            "--max-linear", "1500", "--safety", "1.1", "--timing-only"]
+    print(f"\n[case] 512 mixed {args}", flush=True)
+    with DeviceContext(args + ["--precision", "mixed", "--check-update"]) as dev:
+        u = dev.initial()
+        st = dev.advance(u)
+    for it, got in enumerate(st["updates"]):
+        w = float(run["iterations"][it]["residual_physical_rms"])
+        print(f"update {it}: rms {got['residual_physical_rms']:.12e} (reference {w:.12e}), inner Krylov "
+              f"{got['krylov_iterations']}, error vs FP64 {got['precision_error']:.3e}", flush=True)
+        # RMS_0 sums 85M squared residual entries: the reference's vectorised
+        # sequential sum carries about sqrt(N / 8) eps ~ 3e-13 of rounding
+        assert abs(got["residual_physical_rms"] - w) <= [5e-13, 1e-10][it] * w
+        assert 0 <= got["precision_error"] <= 1e-9, got["precision_error"]
