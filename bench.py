@@ -406,7 +406,8 @@ def main():
     achieved = bytes_bj / dev["bj_avg_s"] / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"bj_apply_traffic_{head}.json")
-    if os.path.exists(tpath):
+    # the counter runs were taken at 256^2, k=8 (bench.py --probe)
+    if os.path.exists(tpath) and args.nx == 256 and args.k == 8:
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     equal = None
