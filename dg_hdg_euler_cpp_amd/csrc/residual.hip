@@ -231,6 +231,61 @@ __device__ __forceinline__ double lift_faces(const double *__restrict__ fq, cons
     return acc;
 }
 
+// The element kernels read the reference-element tables from LDS (staged once
+// per block) and load their 16 face values at the start, so that neither waits
+// on a dependent global load after the block's barrier (measured: the element
+// Jv kernel 160 -> 129 us at 256^2, k=8).
+template <int N> struct ElementTables {
+    double wd[N * N], w[N], left[N], right[N];
+};
+template <int N>
+__device__ __forceinline__ void stage_tables(const Basis *__restrict__ B, ElementTables<N> &t) {
+    for (int k = threadIdx.x; k < N * N + 3 * N; k += BLOCK) {
+        if (k < N * N) t.wd[k] = B->wd[k];
+        else if (k < N * N + N) t.w[k - N * N] = B->w[k - N * N];
+        else if (k < N * N + 2 * N) t.left[k - N * N - N] = B->left[k - N * N - N];
+        else t.right[k - N * N - 2 * N] = B->right[k - N * N - 2 * N];
+    }
+}
+// the four face values of node (i, j), variable v, for lift_staged
+template <int N>
+__device__ __forceinline__ void load_face_values(const double *__restrict__ fq, int cell, int i, int j, const Geo &g,
+                                                 double (&fv)[4][4]) {
+    const long long f0 = neighbor(cell, 0, g.nx, g.ny), f1 = cell;
+    const long long f2 = g.cells + neighbor(cell, 2, g.nx, g.ny), f3 = g.cells + cell;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        fv[v][0] = fq[f0 * 4 * N + v * N + j];
+        fv[v][1] = fq[f1 * 4 * N + v * N + j];
+        fv[v][2] = fq[f2 * 4 * N + v * N + i];
+        fv[v][3] = fq[f3 * 4 * N + v * N + i];
+    }
+}
+// lift_faces with preloaded face values and staged tables (same operation order)
+template <int N>
+__device__ __forceinline__ double lift_staged(const double (&f)[4], const ElementTables<N> &t, int i, int j,
+                                              double acc) {
+    acc += (-1 * t.left[i]) * f[0];
+    acc += (1 * t.right[i]) * f[1];
+    acc += (-1 * f[2]) * t.left[j];
+    acc += (1 * f[3]) * t.right[j];
+    return acc;
+}
+// - (hy/2) w_j sum_q d(q,i) X(q,j) - (hx/2) w_i sum_q Y(i,q) d(q,j) for one variable
+template <int N>
+__device__ __forceinline__ double weak_volume(const double *X, const double *Y, const ElementTables<N> &t, int i,
+                                              int j, const Geo &g, double acc) {
+    const double sx = .5 * g.hy, sy = .5 * g.hx;
+    double wx = 0, wy = 0;
+#pragma unroll
+    for (int q = 0; q < N; ++q) wx += t.wd[q + N * i] * X[q + N * j];
+#pragma unroll
+    for (int q = 0; q < N; ++q) wy += Y[i + N * q] * t.wd[q + N * j];
+    acc -= (sx * t.w[j]) * wx;
+    acc -= sy * t.w[i] * wy;
+    return acc;
+}
+
 // ------------------------------------------------------- element residual
 // discretization.cpp:111-140 + tensor_product.cpp:23-38:
 // r = M(u-u_b)/tau - (hy/2) w_j sum_q d(q,i) Fx(q,j) - (hx/2) w_i sum_q Fy(i,q) d(q,j) + lifts.
@@ -241,12 +296,15 @@ __global__ __launch_bounds__(BLOCK) void k_element_residual(const double *__rest
                                                             double tau, const Basis *__restrict__ B, Geo g) {
     using S = Shape<N>;
     __shared__ double sfx[S::CPB * S::M], sfy[S::CPB * S::M];
+    __shared__ ElementTables<N> tab;
+    stage_tables<N>(B, tab);
     const int local = threadIdx.x / S::P, node = threadIdx.x % S::P;
     const long long cell = static_cast<long long>(blockIdx.x) * S::CPB + local;
     const bool active = local < S::CPB && cell < g.cells;
     const int i = node % N, j = node / N;
-    double res[4];
+    double res[4], fv[4][4];
     if (active) {
+        load_face_values<N>(fq, static_cast<int>(cell), i, j, g, fv);
         double s[4];
         for (int v = 0; v < 4; ++v) {
             s[v] = u[cell * S::M + v * S::P + node];
@@ -263,24 +321,12 @@ __global__ __launch_bounds__(BLOCK) void k_element_residual(const double *__rest
             }
         }
     }
-    if (N > 1) __syncthreads();
+    __syncthreads();
     if (!active) return;
-    if (N > 1) {
-        const double sx = .5 * g.hy, sy = .5 * g.hx;
-        for (int v = 0; v < 4; ++v) {
-            const double *X = sfx + local * S::M + v * S::P, *Y = sfy + local * S::M + v * S::P;
-            double wx = 0, wy = 0;
-#pragma unroll
-            for (int q = 0; q < N; ++q) wx += B->wd[q + N * i] * X[q + N * j];
-#pragma unroll
-            for (int q = 0; q < N; ++q) wy += Y[i + N * q] * B->wd[q + N * j];
-            res[v] -= (sx * B->w[j]) * wx;
-            res[v] -= sy * B->w[i] * wy;
-        }
-    }
     for (int v = 0; v < 4; ++v) {
-        res[v] = lift_faces<N>(fq, B, static_cast<int>(cell), i, j, v, g, res[v]);
-        r[cell * S::M + v * S::P + node] = res[v];
+        if (N > 1) res[v] = weak_volume<N>(sfx + local * S::M + v * S::P, sfy + local * S::M + v * S::P, tab, i, j, g,
+                                           res[v]);
+        r[cell * S::M + v * S::P + node] = lift_staged<N>(fv[v], tab, i, j, res[v]);
     }
 }
 
@@ -401,12 +447,15 @@ __global__ __launch_bounds__(BLOCK) void k_element_jv(const TX *__restrict__ x, 
                                                       double inv_step, const Basis *__restrict__ B, Geo g) {
     using S = Shape<N>;
     __shared__ double sfx[S::CPB * S::M], sfy[S::CPB * S::M];
+    __shared__ ElementTables<N> tab;
+    stage_tables<N>(B, tab);
     const int local = threadIdx.x / S::P, node = threadIdx.x % S::P;
     const long long cell = static_cast<long long>(blockIdx.x) * S::CPB + local;
     const bool active = local < S::CPB && cell < g.cells;
     const int i = node % N, j = node / N;
-    double res[4];
+    double res[4], fv[4][4];
     if (active) {
+        load_face_values<N>(fq, static_cast<int>(cell), i, j, g, fv);
         double d[4];
         for (int v = 0; v < 4; ++v) {
             d[v] = static_cast<double>(x[cell * S::M + v * S::P + node]);
@@ -424,24 +473,12 @@ __global__ __launch_bounds__(BLOCK) void k_element_jv(const TX *__restrict__ x, 
             }
         }
     }
-    if (N > 1) __syncthreads();
+    __syncthreads();
     if (!active) return;
-    if (N > 1) {
-        const double sx = .5 * g.hy, sy = .5 * g.hx;
-        for (int v = 0; v < 4; ++v) {
-            const double *X = sfx + local * S::M + v * S::P, *Y = sfy + local * S::M + v * S::P;
-            double wx = 0, wy = 0;
-#pragma unroll
-            for (int q = 0; q < N; ++q) wx += B->wd[q + N * i] * X[q + N * j];
-#pragma unroll
-            for (int q = 0; q < N; ++q) wy += Y[i + N * q] * B->wd[q + N * j];
-            res[v] -= (sx * B->w[j]) * wx;
-            res[v] -= sy * B->w[i] * wy;
-        }
-    }
     for (int v = 0; v < 4; ++v) {
-        res[v] = lift_faces<N>(fq, B, static_cast<int>(cell), i, j, v, g, res[v]);
-        y[cell * S::M + v * S::P + node] = static_cast<TY>(res[v]);
+        if (N > 1) res[v] = weak_volume<N>(sfx + local * S::M + v * S::P, sfy + local * S::M + v * S::P, tab, i, j, g,
+                                           res[v]);
+        y[cell * S::M + v * S::P + node] = static_cast<TY>(lift_staged<N>(fv[v], tab, i, j, res[v]));
     }
 }
 
@@ -490,18 +527,34 @@ __global__ __launch_bounds__(BLOCK) void k_element_jtv(const double *__restrict_
                                                        double inv_step, const Basis *__restrict__ B, Geo g) {
     using S = Shape<N>;
     __shared__ double sw[S::CPB * S::M];
+    __shared__ ElementTables<N> tab;
+    stage_tables<N>(B, tab);
     const int local = threadIdx.x / S::P, node = threadIdx.x % S::P;
     const long long cell = static_cast<long long>(blockIdx.x) * S::CPB + local;
     const bool active = local < S::CPB && cell < g.cells;
     const int i = node % N, j = node / N;
-    double wv[4], res[4];
+    // Face adjoint lifts: the element is the left element of faces f1 (+x) and
+    // f3 (+y) and the right element of f0 (-x) and f2 (-y).
+    const int c = static_cast<int>(cell);
+    const long long f0 = neighbor(c, 0, g.nx, g.ny), f1 = c;
+    const long long f2 = g.cells + neighbor(c, 2, g.nx, g.ny), f3 = g.cells + c;
+    double wv[4], res[4], gv[4][4], uu = 0, vv = 0, H = 0;
+    if (active && N > 1) {
+        uu = vc[cell * 3 * S::P + node];
+        vv = vc[cell * 3 * S::P + S::P + node];
+        H = vc[cell * 3 * S::P + 2 * S::P + node];
+    }
     if (active)
         for (int v = 0; v < 4; ++v) {
+            gv[v][0] = gtr[f1 * 8 * N + v * N + j];
+            gv[v][1] = gtr[f0 * 8 * N + 4 * N + v * N + j];
+            gv[v][2] = gtr[f3 * 8 * N + v * N + i];
+            gv[v][3] = gtr[f2 * 8 * N + 4 * N + v * N + i];
             wv[v] = w[cell * S::M + v * S::P + node];
             sw[local * S::M + v * S::P + node] = wv[v];
             res[v] = B->mass[node] * wv[v] * inv_step;
         }
-    if (N > 1) __syncthreads();
+    __syncthreads();
     if (!active) return;
     if (N > 1) {
         const double sx = .5 * g.hy, sy = .5 * g.hx;
@@ -510,30 +563,23 @@ __global__ __launch_bounds__(BLOCK) void k_element_jtv(const double *__restrict_
             const double *W = sw + local * S::M + v * S::P;
             double ax = 0, ay = 0;
 #pragma unroll
-            for (int k = 0; k < N; ++k) ax += B->wd[i + N * k] * W[k + N * j];
+            for (int k = 0; k < N; ++k) ax += tab.wd[i + N * k] * W[k + N * j];
 #pragma unroll
-            for (int k = 0; k < N; ++k) ay += B->wd[j + N * k] * W[i + N * k];
-            gx[v] = sx * B->w[j] * ax;
-            gy[v] = sy * B->w[i] * ay;
+            for (int k = 0; k < N; ++k) ay += tab.wd[j + N * k] * W[i + N * k];
+            gx[v] = sx * tab.w[j] * ax;
+            gy[v] = sy * tab.w[i] * ay;
         }
-        const double uu = vc[cell * 3 * S::P + node], vv = vc[cell * 3 * S::P + S::P + node];
-        const double H = vc[cell * 3 * S::P + 2 * S::P + node];
         double tx[4], ty[4];
         normal_action_t(uu, vv, H, gx, 0, g.gamma - 1., g.gamma, tx);
         normal_action_t(uu, vv, H, gy, 1, g.gamma - 1., g.gamma, ty);
         for (int v = 0; v < 4; ++v) res[v] -= tx[v] + ty[v];
     }
-    // Face adjoint lifts: the element is the left element of faces f1 (+x) and
-    // f3 (+y) and the right element of f0 (-x) and f2 (-y).
-    const int c = static_cast<int>(cell);
-    const long long f0 = neighbor(c, 0, g.nx, g.ny), f1 = c;
-    const long long f2 = g.cells + neighbor(c, 2, g.nx, g.ny), f3 = g.cells + c;
     for (int v = 0; v < 4; ++v) {
         double acc = res[v];
-        acc += B->right[i] * gtr[f1 * 8 * N + v * N + j];
-        acc += B->left[i] * gtr[f0 * 8 * N + 4 * N + v * N + j];
-        acc += B->right[j] * gtr[f3 * 8 * N + v * N + i];
-        acc += B->left[j] * gtr[f2 * 8 * N + 4 * N + v * N + i];
+        acc += tab.right[i] * gv[v][0];
+        acc += tab.left[i] * gv[v][1];
+        acc += tab.right[j] * gv[v][2];
+        acc += tab.left[j] * gv[v][3];
         z[cell * S::M + v * S::P + node] = acc;
     }
 }
