@@ -170,14 +170,15 @@ def jv_bytes(g):
 
 
 def cpu_baseline(nx_full, k, init, krylov_per_update):
-    """The oracle (1 core, FP64) on a 32x32 sub-problem of the same k=8 workload,
-    scaled per element to 256^2: t_update = (C/1024) * (t_refresh + n_K * t_iter
-    + 2 * t_residual), with n_K the device run's mean Krylov count per update."""
+    """The oracle (1 core, FP64) on a 64x64 sub-problem of the same k=8 workload
+    (about 10 s of CPU work), scaled per element to 256^2:
+    t_update = (C/4096) * (t_refresh + n_K * t_iter + 2 * t_residual), with n_K
+    the device run's mean Krylov count per update."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import oracle_py as O
 
-    nx = 32
+    nx = 64
     o = O.Oracle(solver_args(nx, k, init, "fp64", 1, 0, device=False))
     u = o.seeded(2025) if init == "seeded" else o.initial()
     rate = o.freeze(u)
@@ -200,11 +201,11 @@ def cpu_baseline(nx_full, k, init, krylov_per_update):
     scale = (nx_full * nx_full) / (nx * nx)
     t_update = scale * (t_ref + krylov_per_update * t_iter + 2 * t_res)
     sample_s = t_res + t_ref + t_g
-    log(f"[cpu] 32x32 k={k}: residual {t_res:.3f}s refresh+LU {t_ref:.3f}s gmres/iter {t_iter:.4f}s "
+    log(f"[cpu] {nx}x{nx} k={k}: residual {t_res:.3f}s refresh+LU {t_ref:.3f}s gmres/iter {t_iter:.4f}s "
         f"-> {t_update:.1f} s/update at {nx_full}^2 (sample {sample_s:.1f}s)")
     return dict(value=1.0 / t_update, unit="Newton updates/s", cores=1, kind="port",
-                sample=(f"oracle FP64 on 1 host core: residual, Jacobian refresh + 1024 block LUs and {its} "
-                        f"GMRES(20) iterations on a 32x32 k={k} mesh ({sample_s:.1f}s), scaled by "
+                sample=(f"oracle FP64 on 1 host core: residual, Jacobian refresh + {nx * nx} block LUs and {its} "
+                        f"GMRES(20) iterations on a {nx}x{nx} k={k} mesh ({sample_s:.1f}s), scaled by "
                         f"{scale:.0f}x elements and the device run's {krylov_per_update:.0f} Krylov "
                         f"iterations per update"),
                 seconds_per_update=t_update)
