@@ -216,6 +216,8 @@ struct dgh_ctx {
     // couplings S [red cell][side][4n x 4n]; apply work vectors
     DevBuf<double> ilu_Y, ilu_S, ilu_z, ilu_w;
     bool ilu_ready = false;
+    // staging for the in-place packing of LU blocks (launch_pack)
+    DevBuf<double> pack_scratch;
     // mixed precision: FP32 factors in factors_f32 (same split), FP32 Y and work
     DevBuf<float> ilu_Y32, ilu_z32, ilu_w32;
     bool ilu32_ready = false;
@@ -291,7 +293,7 @@ struct dgh_ctx {
         basis.release(bytes);
         drop_step_graphs();
         if (gout_host) (void)hipHostFree(gout_host);
-        DevBuf<double> *gbufs[] = {&gH, &gcs, &gsn, &gg, &gnext, &ilu_Y, &ilu_S, &ilu_z, &ilu_w};
+        DevBuf<double> *gbufs[] = {&gH, &gcs, &gsn, &gg, &gnext, &ilu_Y, &ilu_S, &ilu_z, &ilu_w, &pack_scratch};
         for (auto *b : gbufs) b->release(bytes);
         DevBuf<float> *ibufs[] = {&ilu_Y32, &ilu_z32, &ilu_w32};
         for (auto *b : ibufs) b->release(bytes);
@@ -323,6 +325,14 @@ struct dgh_ctx {
         t.launches += 1;
     }
     void sync() { HIPCHECK(hipStreamSynchronize(stream)); }
+    // LU blocks of width w into the packed layout the block solve reads
+    template <typename T> void pack_blocks(int w, T *blocks, int count) {
+        const int chunk = std::min(count, 2048);
+        pack_scratch.alloc(static_cast<size_t>(chunk) * w * w, bytes);
+        timed("pack", [&] {
+            HIPCHECK(launch_pack(w, blocks, count, reinterpret_cast<T *>(pack_scratch.p), chunk, stream));
+        });
+    }
     bool oi() const { return geo.nq > n; }  // over-integrated volume rule (--quad Q > k+1)
 
     // ---- reductions
@@ -463,6 +473,7 @@ struct dgh_ctx {
         });
         timed("lu", [&] { HIPCHECK(launch_lu(m, factors.p, rowsrc.p, cells, flags.p + 2, stream)); });
         require(read_flag(2) == 0, "Singular block Jacobi factor");
+        pack_blocks(m, factors.p, cells);
         bj_ready = true;
         ilu_ready = false;
     }
@@ -495,6 +506,7 @@ struct dgh_ctx {
         });
         timed("lu", [&] { HIPCHECK(launch_lu(m, factors.p, rowsrc.p, half, flags.p + 2, stream)); });
         timed("ilu_coupling", [&] { HIPCHECK(launch_ilu_coupling(n, factors.p, rowsrc.p, ilu_Y.p, ilu_S.p, basis.p, geo, stream)); });
+        pack_blocks(m, factors.p, half);
         timed("ilu_update", [&] {
             HIPCHECK(launch_ilu_update(n, factors.p + half * mm, ilu_S.p, fcache.p, basis.p, geo, stream));
         });
@@ -503,6 +515,7 @@ struct dgh_ctx {
                                stream));
         });
         require(read_flag(2) == 0, "Singular or nonfinite LU factors");
+        pack_blocks(m, factors.p + half * mm, half);
         ilu_ready = true;
     }
     // block_ilu.cpp:62-84 in red-black order: z0_r = A_rr^-1 r_r;
@@ -553,6 +566,7 @@ struct dgh_ctx {
         timed("ilu_coupling", [&] {
             HIPCHECK(launch_ilu_coupling(n, factors_f32.p, rowsrc32.p, ilu_Y32.p, ilu_S.p, basis.p, geo, stream));
         });
+        pack_blocks(m, factors_f32.p, half);
         timed("ilu_update", [&] {
             HIPCHECK(launch_ilu_update(n, factors_f32.p + half * mm, ilu_S.p, fcache.p, basis.p, geo, stream));
         });
@@ -561,6 +575,7 @@ struct dgh_ctx {
                                flags.p + 2, stream));
         });
         require(read_flag(2) == 0, "Singular or nonfinite LU factors");
+        pack_blocks(m, factors_f32.p + half * mm, half);
         ilu32_ready = true;
     }
     void ilu_apply32(const float *r, float *z) {
@@ -613,6 +628,7 @@ struct dgh_ctx {
         });
         timed("lu", [&] { HIPCHECK(launch_lu(m, factors.p, rowsrc.p, cells, flags.p + 2, stream)); });
         require(read_flag(2) == 0, "Singular block Jacobi factor");
+        pack_blocks(m, factors.p, cells);
         adj_bj_ready = true;
         ilu_ready = false;
     }
@@ -837,6 +853,7 @@ struct dgh_ctx {
             HIPCHECK(launch_lu(fw, face_factors.p, face_rowsrc.p, faces, flags.p + 2, stream));
         });
         require(read_flag(2) == 0, "Singular block Jacobi factor");
+        pack_blocks(fw, face_factors.p, faces);
         face_bj_ready = true;
     }
     void hdg_bj_apply(const double *r, double *z) {
@@ -877,6 +894,7 @@ struct dgh_ctx {
         });
         timed("lu", [&] { HIPCHECK(launch_lu(m, factors_f32.p, rowsrc32.p, cells, flags.p + 2, stream)); });
         require(read_flag(2) == 0, "Singular block Jacobi factor");
+        pack_blocks(m, factors_f32.p, cells);
         ilu32_ready = false;
         if (cfg.precision == DGH_PRECISION_MIXED_BF16) {
             factors_bf16.alloc(static_cast<size_t>(cells) * m * m, bytes);

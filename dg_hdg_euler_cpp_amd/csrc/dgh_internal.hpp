@@ -128,6 +128,11 @@ hipError_t launch_assemble_colour(int n, const double *vcache, const double *fca
 hipError_t launch_assemble_colour(int n, const double *vcache, const double *fcache, float *blocks,
                                   double inv_step, const Basis *b, const Geo &g, int colour, hipStream_t s);
 hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s);
+// Square LU blocks into the packed layout that k_bj_apply reads (every block
+// solved by launch_bj_apply* must be packed), in place, `chunk` blocks at a
+// time through `scratch` (chunk * m*m entries).
+hipError_t launch_pack(int m, double *blocks, int nblocks, double *scratch, int chunk, hipStream_t s);
+hipError_t launch_pack(int m, float *blocks, int nblocks, float *scratch, int chunk, hipStream_t s);
 hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s);
 hipError_t launch_bj_apply(int m, const double *factors, const int *rowsrc, const double *r, double *z,
                            int nblocks, hipStream_t s);

@@ -547,6 +547,30 @@ template <typename T> __device__ __forceinline__ T factor_load(const T *p) {
     }
 }
 
+// Offsets of the packed LU layout of one M x M block: column c's strict lower
+// part (rows c+1..M-1) starts at packed_lower(M, c); column c's upper part
+// (rows 0..c) at packed_upper(M, c), after all lower parts, columns in
+// descending order.
+__device__ __forceinline__ long long packed_lower(int M, int c) {
+    return static_cast<long long>(c) * (M - 1) - static_cast<long long>(c) * (c - 1) / 2;
+}
+__device__ __forceinline__ long long packed_upper(int M, int c) {
+    return static_cast<long long>(M) * (M - 1) / 2 + static_cast<long long>(M) * (M + 1) / 2 -
+           static_cast<long long>(c + 1) * (c + 2) / 2;
+}
+// Square column-major LU blocks (from `square`) into the packed layout (into
+// `packed`, same block slots).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_pack(const T *__restrict__ square, T *__restrict__ packed, int M) {
+    const long long mm = static_cast<long long>(M) * M;
+    const T *src = square + blockIdx.x * mm;
+    T *dst = packed + blockIdx.x * mm;
+    for (long long t = threadIdx.x; t < mm; t += BLOCK) {
+        const int c = static_cast<int>(t / M), row = static_cast<int>(t % M);
+        dst[row > c ? packed_lower(M, c) + (row - c - 1) : packed_upper(M, c) + row] = src[t];
+    }
+}
+
 // TF: stored factor type (double, float or bfloat16); TV: vector and
 // arithmetic type (double for the FP64 path, float for the inner FP32 solve).
 template <int M, typename TF, typename TV, int U>
@@ -569,18 +593,21 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
         x[a] = row < M ? r[vb * M + rowsrc[blk * M + row]] : TV(0);
     }
     T buf[U][NA];
+    // Packed factors (k_pack): the strict lower parts of columns 0..M-2 back
+    // to back, then the upper parts (diagonal included) of columns M-1..0, so
+    // that both sweeps stream their columns contiguously.
     // strictly lower part of column c, chunks a >= a0
     auto load_lower = [&](T(&b)[NA], int c, int a0) {
-        const T *colp = F + static_cast<long long>(c) * M;
+        const T *colp = F + packed_lower(M, c);
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int row = a * 64 + lane;
-            if (a >= a0) b[a] = (row > c && row < M) ? factor_load(colp + row) : T(0);
+            if (a >= a0) b[a] = (row > c && row < M) ? factor_load(colp + (row - c - 1)) : T(0);
         }
     };
     // upper part (diagonal included) of column c, chunks a <= a1
     auto load_upper = [&](T(&b)[NA], int c, int a1) {
-        const T *colp = F + static_cast<long long>(c) * M;
+        const T *colp = F + packed_upper(M, c);
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int row = a * 64 + lane;
@@ -1116,6 +1143,27 @@ hipError_t launch_bj_apply_bf16(int m, const void *factors, const int *rowsrc, c
     DGH_DISPATCH_M(m, {
         k_bj_apply<M, __hip_bfloat16, float, BJ_RING16><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, rowsrc, r, z, nblocks, -1, 0);
     })
+}
+// In place, `chunk` blocks at a time through `scratch` (chunk * m*m entries):
+// copy the square blocks aside, then write them back packed.
+template <typename T>
+static hipError_t pack_any(int m, T *blocks, int nblocks, T *scratch, int chunk, hipStream_t s) {
+    const size_t mm = static_cast<size_t>(m) * m;
+    for (int b0 = 0; b0 < nblocks; b0 += chunk) {
+        const int count = min(chunk, nblocks - b0);
+        hipError_t e = hipMemcpyAsync(scratch, blocks + b0 * mm, count * mm * sizeof(T), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+        k_pack<T><<<count, BLOCK, 0, s>>>(scratch, blocks + b0 * mm, m);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+hipError_t launch_pack(int m, double *blocks, int nblocks, double *scratch, int chunk, hipStream_t s) {
+    return pack_any(m, blocks, nblocks, scratch, chunk, s);
+}
+hipError_t launch_pack(int m, float *blocks, int nblocks, float *scratch, int chunk, hipStream_t s) {
+    return pack_any(m, blocks, nblocks, scratch, chunk, s);
 }
 hipError_t launch_convert(const double *src, float *dst, long long count, hipStream_t s) {
     k_convert<double, float><<<4096, BLOCK, 0, s>>>(src, dst, count);
