@@ -40,7 +40,7 @@ dgh_config default_config() {
     c.precision = DGH_PRECISION_FP64;
     c.inner_restart = 30;
     c.inner_max = 300;
-    c.inner_tolerance = 1e-4;
+    c.inner_tolerance = 1e-3;
     c.seed = 2025;
     std::strncpy(c.prefix, "output/run_2d", sizeof(c.prefix) - 1);
     return c;
