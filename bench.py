@@ -366,7 +366,7 @@ def main():
     GRAPH = args.graph
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X; multi-GPU runs are not part of this build "
-              "(the path would shard by 2D element tiles, see DESIGN.md)", file=sys.stderr)
+              "(the path would shard by 2D element tiles with halo exchanges of side traces)", file=sys.stderr)
         sys.exit(2)
 
     if args.probe:

@@ -1,6 +1,6 @@
 """HIP path vs the CPU oracle on identical seeded / projected states.
 
-Tolerances: the north-star bar is 1e-12 relative for the residual, Jv and the
+Tolerances: the parity bar is 1e-12 relative for the residual, Jv and the
 preconditioner apply. Reductions and FMA contraction differ between the GPU
 and the host oracle, so nothing here is bitwise; each tolerance is written at
 its assert.
