@@ -165,6 +165,17 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ v
     constexpr int P = N * N, M = 4 * P;
     __shared__ double ax[P * 16], ay[P * 16];  // [node][v*4+w]
     __shared__ double hs[4 * N * 16];          // [side][q][v*4+w]
+    // reference-element tables, read once per entry below
+    __shared__ double twd[P], tmass[P], tw[N], tl[N], tr[N];
+    for (int k = threadIdx.x; k < P; k += BLOCK) {
+        twd[k] = B->wd[k];
+        tmass[k] = B->mass[k];
+    }
+    for (int k = threadIdx.x; k < N; k += BLOCK) {
+        tw[k] = B->w[k];
+        tl[k] = B->left[k];
+        tr[k] = B->right[k];
+    }
     const int cell = colour < 0 ? first + static_cast<int>(blockIdx.x) : colour_cell(blockIdx.x, colour, g.nx);
     const double gamma = g.gamma, gm = gamma - 1.;
     for (int node = threadIdx.x; node < P; node += BLOCK) {
@@ -194,17 +205,17 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ v
             const int vw = v * 4 + w;
             double val = 0;
             if (N > 1) {
-                if (j == jy) val -= (sx * B->w[jy] * ax[cnode * 16 + vw]) * B->wd[jx + N * i];
-                if (i == jx) val += (sy * B->w[jx] * ay[cnode * 16 + vw]) * B->wd[jy + N * j];
+                if (j == jy) val -= (sx * tw[jy] * ax[cnode * 16 + vw]) * twd[jx + N * i];
+                if (i == jx) val += (sy * tw[jx] * ay[cnode * 16 + vw]) * twd[jy + N * j];
             }
-            if (row == col) val += B->mass[rnode] * inv_step;
+            if (row == col) val += tmass[rnode] * inv_step;
             if (j == jy) {
-                val += (B->left[jx] * hs[(0 * N + j) * 16 + vw]) * B->left[i];
-                val += (B->right[jx] * hs[(1 * N + j) * 16 + vw]) * B->right[i];
+                val += (tl[jx] * hs[(0 * N + j) * 16 + vw]) * tl[i];
+                val += (tr[jx] * hs[(1 * N + j) * 16 + vw]) * tr[i];
             }
             if (i == jx) {
-                val += (B->left[jy] * hs[(2 * N + i) * 16 + vw]) * B->left[j];
-                val += (B->right[jy] * hs[(3 * N + i) * 16 + vw]) * B->right[j];
+                val += (tl[jy] * hs[(2 * N + i) * 16 + vw]) * tl[j];
+                val += (tr[jy] * hs[(3 * N + i) * 16 + vw]) * tr[j];
             }
             if (TRANSPOSE) out[static_cast<long long>(row) * M + col] = static_cast<T>(val);
             else out[static_cast<long long>(col) * M + row] = static_cast<T>(val);
