@@ -159,9 +159,14 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_update(T *__restrict__ blocks, co
     __shared__ double Tm[4][TS * TS];
     __shared__ double tmp[TS * TS];
     __shared__ double J1[4][N][16], J2[4][N][16];
+    __shared__ double tl[N], tr[N];
     const long long k = blockIdx.x;
     const int cell = colour_cell(static_cast<int>(k), 1, g.nx);
     const long long nodes = 2LL * g.cells * N;
+    for (int t = threadIdx.x; t < N; t += BLOCK) {
+        tl[t] = B->left[t];
+        tr[t] = B->right[t];
+    }
     for (int t = threadIdx.x; t < 4 * N; t += BLOCK) {
         const int s = t / N, q = t % N, face = cell_face(cell, s, g);
         const bool left = s % 2;  // the cell is the left element of its +x / +y face
@@ -200,8 +205,8 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_update(T *__restrict__ blocks, co
                 // x sides: line q = j, position a = i; y sides: q = i, a = j
                 const int rq = s < 2 ? rj : ri, ra = s < 2 ? ri : rj;
                 const int cq = s < 2 ? cj : ci, ca = s < 2 ? ci : cj;
-                const double e = s % 2 ? B->right[ra] : -B->left[ra];
-                const double rr = s % 2 ? B->right[ca] : B->left[ca];
+                const double e = s % 2 ? tr[ra] : -tl[ra];
+                const double rr = s % 2 ? tr[ca] : tl[ca];
                 delta += e * Tm[s][v * N + rq + TS * (w * N + cq)] * rr;
             }
             T &entry = A[static_cast<long long>(col) * M + row];
