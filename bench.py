@@ -265,20 +265,25 @@ def summarize_counters(fetch_csv, write_csv, out_json):
     import csv
     import re
 
+    gib = float(1 << 30)
+
     def load(path, counter):
         # per (kernel, grid size): the block ILU(0) launches the block kernels
-        # on one colour (half the grid) beside the full block-Jacobi launches
-        agg = collections.defaultdict(list)
+        # on one colour (half the grid) beside the full block-Jacobi launches.
+        # The calibration copies are the probe's first three large copies (the
+        # factor packing stages larger ones later).
+        agg, copies = collections.defaultdict(list), []
         with open(path) as f:
             for r in csv.DictReader(f):
-                if r["Counter_Name"] == counter:
-                    agg[(r["Kernel_Name"], int(r["Grid_Size"]))].append(float(r["Counter_Value"]) * 1024.0)
-        return agg
+                if r["Counter_Name"] != counter:
+                    continue
+                v = float(r["Counter_Value"]) * 1024.0
+                agg[(r["Kernel_Name"], int(r["Grid_Size"]))].append(v)
+                if r["Kernel_Name"] == "__amd_rocclr_copyBuffer" and v > 0.4 * gib:
+                    copies.append((int(r["Dispatch_Id"]), v))
+        return agg, [v for _, v in sorted(copies)[:3]]
 
-    fetch, write = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
-    gib = float(1 << 30)
-    copy_f = [v for (k, _), vs in fetch.items() if k == "__amd_rocclr_copyBuffer" for v in vs if v > 0.4 * gib]
-    copy_w = [v for (k, _), vs in write.items() if k == "__amd_rocclr_copyBuffer" for v in vs if v > 0.4 * gib]
+    (fetch, copy_f), (write, copy_w) = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
     sf, sw = gib / (sum(copy_f) / len(copy_f)), gib / (sum(copy_w) / len(copy_w))
     cells, m = 256 * 256, 324
     d = cells * m
