@@ -1096,18 +1096,20 @@ hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache
         else k_assemble<N, double, true><<<count, BLOCK, 0, s>>>(vcache, fcache, blocks, inv_step, b, g, first);
     })
 }
-// Panel width and workgroup size of k_lu. A wider panel halves the passes
-// over the trailing matrix but its LDS panel caps the resident workgroups
-// per CU; measured at 256^2, k=8 the 16-column panel wins (NB = 32: FP64
-// 1066 ms, FP32 338 ms; NB = 8: 376 / 211 ms; NB = 16: see the thread counts).
+// Panel width and workgroup size of k_lu. A wider panel saves passes over
+// the trailing matrix but its LDS panel and register tile cap the resident
+// workgroups per CU. Measured at 256^2, k=8 (m = 324 = 27 x 12), FP64 / FP32
+// ms per refresh: NB = 8: 306 / 211; 10: 343 / 196; 12: 224 / 149;
+// 14: 252 / 192; 16: 245 / 179; 20: 322 / 204; 24: 286 / 219; 32: 1066 / 338.
 #ifndef DGH_LU_NB64
-#define DGH_LU_NB64 16
+#define DGH_LU_NB64 12
 #endif
 #ifndef DGH_LU_NB32
-#define DGH_LU_NB32 16
+#define DGH_LU_NB32 12
 #endif
 // FP64 blocks factor faster with 8 waves per workgroup, FP32 blocks with 4
-// (measured, 256^2 k=8: 245 vs 353 ms and 179 vs 206 ms)
+// (measured, 256^2 k=8, NB = 12: 224 vs 248 ms (256 threads) and 149 vs
+// 171 ms (512 threads))
 #ifndef DGH_LU_THREADS64
 #define DGH_LU_THREADS64 512
 #endif
