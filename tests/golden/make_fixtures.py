@@ -33,6 +33,8 @@ RUNS = {
     "k8_512_be": ("speed_test/results/2d_n512_restart30_20261020T220633_rt2c3wx6/job_7718103/data", "k8_dg", None),
 }
 CONVERGENCE = "accuracy_test/results/2d_submission_20261019T234902_AtKkgL/job_7640912/data/convergence.csv"
+# 1D, k=2, DIRK3, CFL 2, T 0.2, GMRES(20) + block Jacobi (accuracy_test/1d/convergence.py:19-25)
+CONVERGENCE_1D = "accuracy_test/results/1d_slurm_7622710/convergence_slurm_7622710.csv"
 
 
 def read_csv(path):
@@ -67,6 +69,17 @@ def main(ref):
     out["convergence"] = {"source": CONVERGENCE, "rows": conv,
                           "args": ["--k", "2", "--cfl", "2", "--integrator", "ssp-dirk3", "--T", "0.2",
                                    "--ntol", "1e-12", "--ltol", "1e-12", "--method", "both"]}
+    conv1 = []
+    for row in read_csv(os.path.join(ref, CONVERGENCE_1D)):
+        conv1.append({k: row[k] for k in ["method", "cells", "k", "steps", "newton_iterations", "krylov_iterations",
+                                           "max_linear_relative_residual", "min_step", "max_step", "rho_L2",
+                                           "momentum_L2", "energy_L2", "max_conservation_drift", "rho_order",
+                                           "mean_newton_s", "difference_max_nodal"]})
+    out["convergence_1d"] = {"source": CONVERGENCE_1D, "rows": conv1,
+                             "args": ["--k", "2", "--integrator", "ssp-dirk3", "--cfl", "2", "--T", "0.2",
+                                      "--method", "both", "--precond", "block-jacobi", "--ordering", "natural",
+                                      "--ntol", "1e-12", "--ltol", "1e-12", "--restart", "20", "--max-linear",
+                                      "1500", "--max-newton", "30", "--safety", "1.1"]}
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_runs.json")
     with open(dst, "w") as f:
         json.dump(out, f, indent=1)

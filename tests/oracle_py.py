@@ -271,6 +271,122 @@ class Oracle:
         return u, stats
 
 
+class Oracle1D:
+    """The 1D DG restatement (dg_hdg_1d/src): one ``Problem`` + DG operator +
+    block Jacobi, vectors in the reference layout cell * 3n + v * n + i."""
+
+    def __init__(self, args):
+        L = lib()
+        L.orc1_create.restype = C.c_void_p
+        L.orc1_create.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
+        L.orc1_destroy.argtypes = [C.c_void_p]
+        L.orc1_last_error.restype = C.c_char_p
+        L.orc1_h.restype = C.c_double
+        argv = [str(a) for a in args]
+        arr = (C.c_char_p * max(1, len(argv)))(*[a.encode() for a in argv])
+        self.h = L.orc1_create(len(argv), arr)
+        if not self.h:
+            raise OracleError(L.orc1_last_error().decode())
+        self.size = L.orc1_size(C.c_void_p(self.h))
+        self.n = L.orc1_nodes(C.c_void_p(self.h))
+        self.m = 3 * self.n
+        self.cells = self.size // self.m
+        self.dx = L.orc1_h(C.c_void_p(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc1_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def _call(self, name, *args):
+        L = lib()
+        rc = getattr(L, name)(C.c_void_p(self.h), *args)
+        if rc != 0:
+            raise OracleError(L.orc1_last_error().decode())
+
+    def _vec(self, name, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.size)
+        self._call(name, _p(x), _p(y))
+        return y
+
+    def initial(self):
+        u = np.zeros(self.size)
+        self._call("orc1_initial", _p(u))
+        return u
+
+    def freeze(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        rate = C.c_double()
+        speeds = np.zeros(2 * self.cells)
+        self._call("orc1_freeze", _p(u), C.byref(rate), _p(speeds))
+        self.speeds = speeds.reshape(-1, 2)
+        return rate.value
+
+    def set_step(self, tau, old):
+        old = np.ascontiguousarray(old, dtype=np.float64)
+        self._call("orc1_set_step", C.c_double(tau), _p(old))
+
+    def residual(self, u):
+        return self._vec("orc1_residual", u)
+
+    def admissible(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return bool(lib().orc1_admissible(C.c_void_p(self.h), _p(u)))
+
+    def refresh(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self._call("orc1_refresh", _p(u))
+
+    def apply(self, x):
+        return self._vec("orc1_apply", x)
+
+    def block(self, row, col):
+        out = np.zeros(self.m * self.m)
+        self._call("orc1_block", C.c_int(row), C.c_int(col), _p(out))
+        return out.reshape(self.m, self.m, order="F")
+
+    def bj_refresh(self):
+        self._call("orc1_bj_refresh")
+
+    def bj_apply(self, r):
+        return self._vec("orc1_bj_apply", r)
+
+    def gmres(self, rhs, restart, limit, tol):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros(self.size)
+        its, rel = C.c_int(), C.c_double()
+        self._call("orc1_gmres", _p(rhs), _p(x), C.c_int(restart), C.c_int(limit), C.c_double(tol),
+                   C.byref(its), C.byref(rel))
+        return x, its.value, rel.value
+
+    def error_l2(self, u, t):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        e = np.zeros(3)
+        self._call("orc1_error", _p(u), C.c_double(t), _p(e))
+        return e
+
+    def integral(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        e = np.zeros(3)
+        self._call("orc1_integral", _p(u), _p(e))
+        return e
+
+    def advance(self, u, cap=100000):
+        u = np.array(u, dtype=np.float64, copy=True)
+        summary = np.zeros(9)
+        upd = np.zeros((cap, 12))
+        count = C.c_int()
+        self._call("orc1_advance", _p(u), _p(summary), _p(upd), C.c_int(cap), C.byref(count))
+        keys = ["final_time", "steps", "newton_iterations", "krylov_iterations", "conservation_drift",
+                "max_linear_residual", "max_dg_linear_residual", "minimum_step", "maximum_step"]
+        stats = dict(zip(keys, summary.tolist()))
+        ukeys = ["step", "stage", "iteration", "krylov_iterations", "time", "stage_time", "dt", "cfl",
+                 "residual_physical_rms", "damping", "linear_relative_residual", "dg_linear_relative_residual"]
+        stats["updates"] = [dict(zip(ukeys, row.tolist())) for row in upd[:min(count.value, cap)]]
+        return u, stats
+
+
 def gauss(n):
     x = np.zeros(n)
     w = np.zeros(n)
