@@ -27,7 +27,7 @@ ARCH = os.environ.get("DGH_OFFLOAD_ARCH", "gfx950")
 # Packed FP32 instructions are disabled for every HIP translation unit.
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
-SOURCES = ["residual.hip", "linalg.hip", "hdg.hip", "ilu.hip", "api.hip", "config.cpp"]
+SOURCES = ["residual.hip", "linalg.hip", "hdg.hip", "ilu.hip", "api.hip", "config.cpp", "basis.cpp", "oned.hip"]
 
 
 def _run(cmd):

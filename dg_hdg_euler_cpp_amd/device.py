@@ -381,3 +381,158 @@ class DeviceContext:
         launches = C.c_longlong()
         self._call("dgh_kernel_time", name.encode(), C.byref(ms), C.byref(launches))
         return ms.value, launches.value
+
+
+class Dgh1Config(C.Structure):
+    """dgh1_config of include/dgh/dgh1d_device.h (1D euler::Config)."""
+    _fields_ = [("cells", C.c_int), ("degree", C.c_int), ("quadrature", C.c_int), ("restart", C.c_int),
+                ("max_linear", C.c_int), ("max_newton", C.c_int), ("steps", C.c_int), ("newton_steps", C.c_int),
+                ("warmup_updates", C.c_int), ("cfl", C.c_double), ("final_time", C.c_double),
+                ("safety", C.c_double), ("nonlinear_tolerance", C.c_double), ("linear_tolerance", C.c_double),
+                ("integrator", C.c_int), ("method", C.c_int), ("timing_only", C.c_int), ("prefix", C.c_char * 512)]
+
+
+def _lib1():
+    L = load_library()
+    if not getattr(L, "_dgh1_ready", False):
+        L.dgh1_create.restype = C.c_void_p
+        L.dgh1_create.argtypes = [C.POINTER(Dgh1Config)]
+        L.dgh1_create_error.restype = C.c_char_p
+        L.dgh1_last_error.restype = C.c_char_p
+        L.dgh1_last_error.argtypes = [C.c_void_p]
+        L.dgh1_destroy.argtypes = [C.c_void_p]
+        L.dgh1_size.restype = C.c_longlong
+        L.dgh1_size.argtypes = [C.c_void_p]
+        L.dgh1_parse_config.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(Dgh1Config)]
+        L._dgh1_ready = True
+    return L
+
+
+def parse_config_1d(args):
+    """Reference 1D CLI flags (dg_hdg_1d/src/config.cpp) -> Dgh1Config."""
+    L = _lib1()
+    argv = [str(a) for a in args]
+    arr = (C.c_char_p * max(1, len(argv)))(*[a.encode() for a in argv])
+    cfg = Dgh1Config()
+    if L.dgh1_parse_config(len(argv), arr, C.byref(cfg)) != 0:
+        raise DeviceError(L.dgh1_create_error().decode())
+    return cfg
+
+
+class DeviceContext1D:
+    """The 1D periodic Euler DG path (dg_hdg_1d) behind include/dgh/dgh1d_device.h:
+    Problem, DG operator, block Jacobi, GMRES and TimeIntegrator on the device."""
+
+    def __init__(self, args):
+        import torch  # memory plumbing only
+
+        self._torch = torch
+        L = _lib1()
+        self.config = parse_config_1d(args)
+        self.h = L.dgh1_create(C.byref(self.config))
+        if not self.h:
+            raise DeviceError(L.dgh1_create_error().decode())
+        self.h = C.c_void_p(self.h)
+        self.size = int(L.dgh1_size(self.h))
+        self.n = self.config.degree + 1
+        self.m = 3 * self.n
+        self.cells = self.config.cells
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib1().dgh1_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _call(self, name, *args):
+        L = _lib1()
+        if getattr(L, name)(self.h, *args) != 0:
+            raise DeviceError(L.dgh1_last_error(self.h).decode())
+
+    def zeros(self):
+        return self._torch.zeros(self.size, dtype=self._torch.float64, device="cuda")
+
+    def initial(self):
+        u = self.zeros()
+        self._call("dgh1_initial", _ptr(u))
+        return u
+
+    def freeze(self, u):
+        rate = C.c_double()
+        speeds = (C.c_double * (2 * self.cells))()
+        self._call("dgh1_freeze", _ptr(u), C.byref(rate), speeds)
+        self.speeds = list(speeds)
+        return rate.value
+
+    def set_step(self, tau, old):
+        self._call("dgh1_set_step", C.c_double(tau), _ptr(old))
+
+    def residual(self, u):
+        r = self.zeros()
+        self._call("dgh1_residual", _ptr(u), _ptr(r))
+        return r
+
+    def admissible(self, u):
+        ok = C.c_int()
+        self._call("dgh1_admissible", _ptr(u), C.byref(ok))
+        return bool(ok.value)
+
+    def refresh(self, u):
+        self._call("dgh1_refresh", _ptr(u))
+
+    def apply(self, x):
+        y = self.zeros()
+        self._call("dgh1_apply", _ptr(x), _ptr(y))
+        return y
+
+    def block(self, row):
+        out = (C.c_double * (self.m * self.m))()
+        self._call("dgh1_block", C.c_int(row), out)
+        import numpy as np
+
+        return np.array(out).reshape(self.m, self.m, order="F")
+
+    def bj_refresh(self):
+        self._call("dgh1_bj_refresh")
+
+    def bj_apply(self, r):
+        z = self.zeros()
+        self._call("dgh1_bj_apply", _ptr(r), _ptr(z))
+        return z
+
+    def gmres(self, rhs, limit, tol):
+        x = self.zeros()
+        its, rel = C.c_int(), C.c_double()
+        self._call("dgh1_gmres", _ptr(rhs), _ptr(x), C.c_int(limit), C.c_double(tol), C.byref(its), C.byref(rel))
+        return x, its.value, rel.value
+
+    def error_l2(self, u, t):
+        e = (C.c_double * 3)()
+        self._call("dgh1_error", _ptr(u), C.c_double(t), e)
+        return list(e)
+
+    def integral(self, u):
+        e = (C.c_double * 3)()
+        self._call("dgh1_integral", _ptr(u), e)
+        return list(e)
+
+    def advance(self, u, capacity=100000):
+        run = RunStats()
+        ups = (UpdateStats * capacity)()
+        count = C.c_int()
+        self._call("dgh1_advance", _ptr(u), C.byref(run), ups, C.c_int(capacity), C.byref(count))
+        out = {k: getattr(run, k) for k, _ in RunStats._fields_}
+        out["updates"] = [{k: getattr(ups[i], k) for k, _ in UpdateStats._fields_}
+                          for i in range(min(count.value, capacity))]
+        return out

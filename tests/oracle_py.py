@@ -295,7 +295,10 @@ class Oracle1D:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc1_destroy(C.c_void_p(self.h))
+            try:
+                lib().orc1_destroy(C.c_void_p(self.h))
+            except TypeError:  # interpreter shutdown: the module globals are gone
+                pass
             self.h = None
 
     def _call(self, name, *args):

@@ -26,6 +26,39 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_library_exports_every_1d_symbol():
+    from dg_hdg_euler_cpp_amd import load_library
+
+    text = open(os.path.join(ROOT, "include", "dgh", "dgh1d_device.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    syms = sorted(set(re.findall(r"\b(dgh1_[a-z0-9_]+)\s*\(", text)))
+    assert len(syms) >= 18
+    L = load_library()
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("args,message", [
+    (["--cells", "2"], "Invalid mesh or degree"),
+    (["--k", "9"], "Invalid mesh or degree"),
+    (["--k", "3", "--quad", "3"], "Quadrature must be >= degree+1"),
+    (["--T", "0.1", "--steps", "3"], "Choose either --T or --steps"),
+    (["--integrator", "rk4"], "Invalid integrator"),
+    (["--method", "both2"], "Method must be dg, hdg, or both"),
+    (["--ltol", "1"], "Invalid solver tolerance or wave-speed safety"),
+    (["--bogus"], "Unknown option: --bogus"),
+    (["--precond", "block-ilu0"], "The 1D device path preconditions with block Jacobi"),
+])
+def test_1d_config_errors_use_reference_messages(args, message):
+    """dg_hdg_1d/src/config.cpp:6-116 messages and defaults (config.hpp:7-18)."""
+    from dg_hdg_euler_cpp_amd.device import DeviceError, parse_config_1d
+
+    c = parse_config_1d([])
+    assert (c.cells, c.degree, c.restart, c.integrator, c.cfl, c.final_time) == (32, 3, 30, 1, 2.0, 0.2)
+    with pytest.raises(DeviceError, match=re.escape(message)):
+        parse_config_1d(args)
+
+
 def parse(args):
     from dg_hdg_euler_cpp_amd.device import parse_config
 
