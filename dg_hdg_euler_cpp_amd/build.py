@@ -5,6 +5,7 @@
 Outputs (all in-tree, git-ignored):
     dg_hdg_euler_cpp_amd/libdgh_device.so   HIP kernels + C ABI (include/dgh/dgh_device.h)
     dg_hdg_euler_cpp_amd/bin/dg_hdg_2d       reference-compatible CLI on the device path
+    dg_hdg_euler_cpp_amd/bin/dg_hdg_1d       the 1D CLI (method dg) on the device path
     oracle/libdgh_oracle.so                 CPU FP64 oracle (test infrastructure)
 """
 from __future__ import annotations
@@ -21,6 +22,7 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libdgh_device.so")
 CLI = os.path.join(PKG, "bin", "dg_hdg_2d")
+CLI1D = os.path.join(PKG, "bin", "dg_hdg_1d")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = os.environ.get("DGH_OFFLOAD_ARCH", "gfx950")
@@ -56,6 +58,10 @@ def build_device(jobs: int = 8) -> str:
     _run([HIPCC, "-O2", "-std=c++17", "-o", tmp, os.path.join(PKG, "cli", "main.cpp"), "-L" + PKG, "-ldgh_device",
           "-Wl,-rpath,$ORIGIN/.."])
     os.replace(tmp, CLI)
+    tmp = CLI1D + ".tmp"
+    _run([HIPCC, "-O2", "-std=c++17", "-o", tmp, os.path.join(PKG, "cli", "main1d.cpp"), "-L" + PKG, "-ldgh_device",
+          "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, CLI1D)
     return LIB
 
 

@@ -1312,6 +1312,13 @@ const char *dgh1_create_error(void) { return g_create_error1.c_str(); }
 void dgh1_destroy(dgh1_ctx *ctx) { delete ctx; }
 const char *dgh1_last_error(const dgh1_ctx *ctx) { return ctx ? ctx->error.c_str() : "Null context"; }
 long long dgh1_size(const dgh1_ctx *ctx) { return ctx ? ctx->D : 0; }
+int dgh1_nodes(const dgh1_ctx *ctx, double *x) {
+    if (!ctx) return DGH_ERROR;
+    std::vector<double> nodes, weights;
+    gauss_rule(ctx->n, nodes, weights);
+    for (int i = 0; i < ctx->n; ++i) x[i] = nodes[i];
+    return DGH_OK;
+}
 int dgh1_initial(dgh1_ctx *ctx, double *u) {
     return run1(ctx, [&] {
         ctx->initial(u);

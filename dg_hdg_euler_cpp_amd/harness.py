@@ -3,6 +3,7 @@
     python -m dg_hdg_euler_cpp_amd.harness check    --output DIR
     python -m dg_hdg_euler_cpp_amd.harness accuracy --output DIR [--grids 32 64 128 256]
     python -m dg_hdg_euler_cpp_amd.harness speed    --output DIR [--cells 256] [--degrees 8 7 ...]
+    python -m dg_hdg_euler_cpp_amd.harness accuracy1d --output DIR [--grids 32 64 128 256 512]
 
 Each runs `bin/dg_hdg_2d --device gpu` with the command lines of the
 reference's drivers and applies their acceptance rules:
@@ -17,6 +18,10 @@ reference's drivers and applies their acceptance rules:
   speedup.csv, with the checks of scripts/postprocess.py:190-195 (four
   updates, warm-up flags 0,1,1,1, the summary mean equals the mean of the
   three measured iteration times).
+* accuracy1d: accuracy_test/1d/convergence.py:19-47 with `bin/dg_hdg_1d` --
+  k=2, DIRK3, CFL 2, T 0.2, GMRES(20) + block Jacobi per grid, log2 orders per
+  component, written to convergence.csv; the device's 1D path runs method dg,
+  so the rows are the reference table's dg rows (no DG/HDG difference columns).
 A SUCCESS file is written when every run passed; any failure raises.
 """
 from __future__ import annotations
@@ -31,6 +36,7 @@ import time
 from pathlib import Path
 
 EXE = Path(__file__).resolve().parent / "bin" / "dg_hdg_2d"
+EXE1D = Path(__file__).resolve().parent / "bin" / "dg_hdg_1d"
 
 
 def rows(path):
@@ -112,6 +118,32 @@ def accuracy(output: Path, exe: Path, grids):
     return records
 
 
+def accuracy1d(output: Path, exe: Path, grids):
+    require(all(n >= 3 for n in grids) and grids == sorted(set(grids)), "Grids must be distinct and increasing, N>=3")
+    records, previous = [], {}
+    for cells in grids:
+        prefix = output / f"n{cells}"
+        print(f"Start N={cells}, k=2, DIRK3, CFL=2, T=0.2", flush=True)
+        run([exe, "--cells", cells, "--k", 2, "--integrator", "ssp-dirk3", "--cfl", 2, "--T", .2, "--method", "dg",
+             "--precond", "block-jacobi", "--ordering", "natural", "--ntol", "1e-12", "--ltol", "1e-12",
+             "--restart", 20, "--max-linear", 1500, "--max-newton", 30, "--safety", 1.1, "--prefix", prefix], prefix)
+        summaries = rows(f"{prefix}_summary.csv")
+        require([r["method"] for r in summaries] == ["dg"], "Missing solver results")
+        data = summaries[0]
+        require(float(data["final_time"]) == .2 and data["integrator"] == "ssp-dirk3"
+                and int(data["fixed_newton_steps"]) == 0, "Unexpected convergence protocol")
+        for component in ("rho", "momentum", "energy"):
+            error = float(data[f"{component}_L2"])
+            require(math.isfinite(error) and error > 0, "Nonfinite/zero convergence error")
+            data[f"{component}_order"] = math.log2(previous[component] / error) if component in previous else ""
+            previous[component] = error
+        records.append(data)
+        write_csv(output / "convergence.csv", records)
+        print(f"N={cells} completed", flush=True)
+    (output / "SUCCESS").write_text("All requested 1D DG grids completed.\n")
+    return records
+
+
 def speed(output: Path, exe: Path, cells, degrees):
     require(cells >= 3 and len(set(degrees)) == len(degrees) and all(k in range(9) for k in degrees),
             "Invalid mesh or degree list")
@@ -153,14 +185,15 @@ def speed(output: Path, exe: Path, cells, degrees):
 
 def main(argv=None):
     parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    parser.add_argument("mode", choices=["check", "accuracy", "speed"])
+    parser.add_argument("mode", choices=["check", "accuracy", "speed", "accuracy1d"])
     parser.add_argument("--output", required=True)
-    parser.add_argument("--exe", default=str(EXE))
-    parser.add_argument("--grids", type=int, nargs="+", default=[32, 64, 128, 256])
+    parser.add_argument("--exe", default=None)
+    parser.add_argument("--grids", type=int, nargs="+", default=None)
     parser.add_argument("--cells", type=int, default=256)
     parser.add_argument("--degrees", type=int, nargs="+", default=list(range(8, -1, -1)))
     args = parser.parse_args(argv)
-    exe = Path(args.exe)
+    exe = Path(args.exe or (EXE1D if args.mode == "accuracy1d" else EXE))
+    grids = args.grids or ([32, 64, 128, 256, 512] if args.mode == "accuracy1d" else [32, 64, 128, 256])
     if not exe.is_file():
         raise SystemExit(f"{exe} is missing: build it with `python -m dg_hdg_euler_cpp_amd.build`")
     output = Path(args.output)
@@ -168,7 +201,9 @@ def main(argv=None):
     if args.mode == "check":
         check(output, exe)
     elif args.mode == "accuracy":
-        accuracy(output, exe, args.grids)
+        accuracy(output, exe, grids)
+    elif args.mode == "accuracy1d":
+        accuracy1d(output, exe, grids)
     else:
         speed(output, exe, args.cells, args.degrees)
     return 0

@@ -56,6 +56,9 @@ const char *dgh1_last_error(const dgh1_ctx *ctx);
 /* number of unknowns cells*3*(k+1) */
 long long dgh1_size(const dgh1_ctx *ctx);
 
+/* reference-element Gauss nodes (nodal_basis.hpp:34-45), n = k+1 values into
+ * HOST memory, for the node coordinates of the solution output (main.cpp:91-94) */
+int dgh1_nodes(const dgh1_ctx *ctx, double *x);
 /* Problem::initial (discretization.cpp:31-73): projected simple wave with
  * the cell-mean scaling. */
 int dgh1_initial(dgh1_ctx *ctx, double *u);

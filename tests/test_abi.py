@@ -134,3 +134,16 @@ def test_cli_reports_errors_like_the_reference():
     exe = os.path.join(ROOT, "dg_hdg_euler_cpp_amd", "bin", "dg_hdg_2d")
     p = subprocess.run([exe, "--k", "9"], capture_output=True, text=True)
     assert p.returncode == 1 and p.stderr.strip() == "ERROR: Invalid mesh or degree"
+
+
+def test_1d_cli_reports_errors_like_the_reference():
+    """dg_hdg_1d/src/main.cpp:103-106: "ERROR: <message>" and exit status 1."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "dg_hdg_euler_cpp_amd", "bin", "dg_hdg_1d")
+    p = subprocess.run([exe, "--cells", "2"], capture_output=True, text=True)
+    assert p.returncode == 1 and p.stderr.strip() == "ERROR: Invalid mesh or degree"
+    p = subprocess.run([exe, "--cells", "8"], capture_output=True, text=True)
+    assert p.returncode == 1 and p.stderr.strip() == "ERROR: The 1D device path runs the DG method: use --method dg"
+    p = subprocess.run([exe, "--help"], capture_output=True, text=True)
+    assert p.returncode == 0 and "--cells" in p.stdout
