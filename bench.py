@@ -93,6 +93,31 @@ def run_ilu(args, nx, k, init, precision="fp64"):
             "rms": [x["residual_physical_rms"] for x in upd], "timed_regions": per}
 
 
+def run_1d():
+    """The reference's 1D accuracy protocol at 128 cells (accuracy_test/1d/
+    convergence.py:19-25: k=2, DIRK3, CFL 2, T 0.2, GMRES(20) + block Jacobi)
+    on the device 1D path, one warm-up run then one timed run; beside it the
+    reference's archived single-core time for the same run
+    (convergence_slurm_7622710.csv:6, a different CPU, not a same-run figure)."""
+    from dg_hdg_euler_cpp_amd import DeviceContext1D
+
+    args = ["--cells", "128", "--k", "2", "--integrator", "ssp-dirk3", "--cfl", "2", "--T", "0.2", "--method", "dg",
+            "--ntol", "1e-12", "--ltol", "1e-12", "--restart", "20", "--max-linear", "1500", "--max-newton", "30",
+            "--safety", "1.1"]
+    with DeviceContext1D(args) as dev:
+        dev.advance(dev.initial())
+        u = dev.initial()
+        st = dev.advance(u)
+        err = dev.error_l2(u, st["final_time"])
+    log(f"[bench] 1d dg 128 cells k=2 DIRK3: {st['newton_iterations']} updates, {st['krylov_iterations']} Krylov, "
+        f"{st['seconds']:.4f} s")
+    return {"workload": "1D DG, 128 cells, k=2, DIRK3, CFL 2, T 0.2, GMRES(20) + block Jacobi",
+            "time_loop_s": st["seconds"], "newton_updates": st["newton_iterations"],
+            "krylov_iterations": st["krylov_iterations"], "ms_per_update": 1e3 * st["seconds"] / st["newton_iterations"],
+            "rho_L2": err[0], "reference_time_loop_s_published": 0.45550448400000004,
+            "reference_mean_newton_s_published": 0.0028767540316455686}
+
+
 def run_precision_check(args, nx, k, init, precision):
     """Untimed: every update of the mixed form also solves its Newton system
     through the FP64 path and reports ||du_mixed - du_fp64||_M / ||du_fp64||_M."""
@@ -356,6 +381,7 @@ def main():
     ap.add_argument("--graph", action="store_true", help="GMRES Arnoldi steps as captured hipGraphs")
     ap.add_argument("--only-hdg", action="store_true", help="run and report the condensed HDG form alone")
     ap.add_argument("--no-ilu", action="store_true", help="skip the block ILU(0) form")
+    ap.add_argument("--no-1d", action="store_true", help="skip the 1D DG protocol run")
     ap.add_argument("--probe", action="store_true", help="run each hot kernel 3x for counter profiles")
     ap.add_argument("--counters", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
                     help="summarise two rocprofv3 counter runs of --probe into HBM bytes per launch")
@@ -423,6 +449,7 @@ def main():
                  "max_rel_diff_first_two_updates": max(abs(x - y) / abs(x) for x, y in zip(a[:2], b[:2])),
                  "speedup_mixed_over_fp64": forms_out["mixed"]["updates_per_s"] / forms_out["fp64"]["updates_per_s"]}
     hdg = None if args.no_hdg or args.precision != "both" else run_hdg(args, args.nx, args.k, args.init)
+    oned = None if args.no_1d or args.precision != "both" else run_1d()
     ilu = None if args.no_ilu or args.precision != "both" else run_ilu(args, args.nx, args.k, args.init)
     ilu_mixed = None if args.no_ilu or args.precision != "both" else run_ilu(args, args.nx, args.k, args.init,
                                                                               "mixed")
@@ -456,6 +483,7 @@ def main():
         "hdg_fp64": hdg,
         "ilu0_redblack_fp64": ilu,
         "ilu0_redblack_mixed": ilu_mixed,
+        "oned_dg": oned,
         "jv": {"dof_per_s": g["size"] / dev["jv_avg_s"], "achieved_GBs": jv_bytes(g) / dev["jv_avg_s"] / 1e9,
                "avg_launch_s": dev["jv_avg_s"]},
         "cpu_baseline": None if cpu is None else {k: v for k, v in cpu.items() if k != "seconds_per_update"},
