@@ -408,6 +408,29 @@ class DGOperator {
                 }
         }
     }
+    // volume_jacobian.cpp:105-132: the volume part of block(row, row) plus M / tau
+    void volume_block(int row, Mat &out) const {
+        const int n = p_.nodes, Q = p_.basis.points, m = p_.m;
+        out = Mat(m, m);
+        if (n > 1)
+            for (int w = 0; w < NV; ++w)
+                for (int j = 0; j < n; ++j)
+                    for (int v = 0; v < NV; ++v)
+                        for (int i = 0; i < n; ++i) {
+                            double value;
+                            if (Q == n) {
+                                value = -coef(row, j)[v + NV * w] * p_.basis.weighted_derivatives(j, i);
+                            } else {
+                                double s = 0;
+                                for (int q = 0; q < Q; ++q)
+                                    s += p_.basis.weighted_derivatives(q, i) *
+                                         (coef(row, q)[v + NV * w] * p_.basis.values(q, j));
+                                value = -s;
+                            }
+                            out(v * n + i, w * n + j) = value;
+                        }
+        for (int k = 0; k < m; ++k) out(k, k) += p_.mass[static_cast<size_t>(row) * m + k] * inverse_step_;
+    }
     // dg_operator.cpp:50-79 with volume_jacobian.cpp:105-132; column-major m x m,
     // row/column index v * n + i
     void block(int row, int col, Mat &out) const {
@@ -463,6 +486,135 @@ class DGOperator {
     }
 };
 
+// hybrid.cpp:3-139 (1D): static condensation of each cell onto its two face
+// traces (3 values per face). Per cell A = volume block, B (m x 6), C (6 x m);
+// X = A^-1 B, y = A^-1 r, S = -C X; the Schur operator is -(a+b) I per face
+// plus the cells' S; recover du = -y - X dl.
+class Hybrid {
+  public:
+    explicit Hybrid(const Problem &p)
+        : p_(p), dg_(p), m_(p.m), X_(p.cells), C_(p.cells), S_(p.cells), B_(p.cells), y_(p.cells),
+          rhs_(static_cast<size_t>(NV) * p.cells) {}
+    int face(int cell, int side) const { return side == 0 ? (cell + p_.cells - 1) % p_.cells : cell; }
+    void refresh_projected(const Vec &u, const Vec &r) {
+        dg_.refresh(u);  // the volume Jacobian; the face part below replaces the DG faces
+        std::fill(rhs_.begin(), rhs_.end(), 0.0);
+        const int n = p_.nodes, m = m_;
+        for (int cell = 0; cell < p_.cells; ++cell) {
+            Mat A;
+            dg_.volume_block(cell, A);
+            Mat B(m, 2 * NV), C(2 * NV, m);
+            for (int side = 0; side < 2; ++side) {
+                const Speed sp = p_.speeds[face(cell, side)];
+                const Vec &shape = side == 0 ? p_.basis.left : p_.basis.right;
+                const double sign = side == 0 ? -1.0 : 1.0, st = side == 0 ? sp.b : sp.a;
+                const Jacobian J = p_.example.gas.jacobian(p_.sample(u, cell, shape));
+                Jacobian d;
+                for (int k = 0; k < NV * NV; ++k) d[k] = sign * J[k] + st * ((k % 4 == 0) ? 1.0 : 0.0);
+                for (int v = 0; v < NV; ++v)
+                    for (int i = 0; i < n; ++i) {
+                        const int row = v * n + i;
+                        B(row, side * NV + v) = -shape[i] * st;
+                        for (int w = 0; w < NV; ++w) {
+                            C(side * NV + w, row) = shape[i] * d[w + NV * v];
+                            for (int j = 0; j < n; ++j) A(row, w * n + j) += shape[i] * shape[j] * d[v + NV * w];
+                        }
+                    }
+            }
+            oracle::DenseLU lu;
+            lu.compute(A);
+            lu.solve_multiple(B, 2 * NV, X_[cell]);
+            Vec y(r.begin() + static_cast<long>(cell) * m, r.begin() + static_cast<long>(cell + 1) * m);
+            lu.solve_inplace(y.data());
+            y_[cell] = y;
+            Mat S(2 * NV, 2 * NV);
+            for (int j = 0; j < 2 * NV; ++j)
+                for (int a = 0; a < 2 * NV; ++a) {
+                    double acc = 0;
+                    for (int k = 0; k < m; ++k) acc += C(a, k) * X_[cell](k, j);
+                    S(a, j) = -acc;
+                }
+            S_[cell] = S;
+            C_[cell] = C;
+            B_[cell] = B;
+            for (int side = 0; side < 2; ++side)
+                for (int w = 0; w < NV; ++w) {
+                    double acc = 0;
+                    for (int k = 0; k < m; ++k) acc += C(side * NV + w, k) * y[k];
+                    rhs_[static_cast<size_t>(face(cell, side)) * NV + w] += acc;
+                }
+        }
+        ready_ = true;
+    }
+    const Vec &rhs() const { return rhs_; }
+    void apply(const Vec &x, Vec &y) const {  // hybrid.cpp:82-101
+        require(ready_, "Refresh the hybrid operator first");
+        y.assign(x.size(), 0.0);
+        for (int f = 0; f < p_.cells; ++f)
+            for (int v = 0; v < NV; ++v) y[f * NV + v] = -(p_.speeds[f].a + p_.speeds[f].b) * x[f * NV + v];
+        for (int cell = 0; cell < p_.cells; ++cell) {
+            double g[2 * NV];
+            for (int side = 0; side < 2; ++side)
+                for (int v = 0; v < NV; ++v) g[side * NV + v] = x[face(cell, side) * NV + v];
+            for (int side = 0; side < 2; ++side)
+                for (int w = 0; w < NV; ++w) {
+                    double acc = 0;
+                    for (int k = 0; k < 2 * NV; ++k) acc += S_[cell](side * NV + w, k) * g[k];
+                    y[face(cell, side) * NV + w] += acc;
+                }
+        }
+    }
+    void block(int row, int col, Mat &out) const {  // hybrid.cpp:102-115
+        out = Mat(NV, NV);
+        if (row == col)
+            for (int v = 0; v < NV; ++v) out(v, v) = -(p_.speeds[row].a + p_.speeds[row].b);
+        for (int adjacent = 0; adjacent < 2; ++adjacent) {
+            const int cell = (row + adjacent) % p_.cells, row_side = adjacent == 0 ? 1 : 0;
+            for (int col_side = 0; col_side < 2; ++col_side)
+                if (face(cell, col_side) == col)
+                    for (int a = 0; a < NV; ++a)
+                        for (int b = 0; b < NV; ++b) out(a, b) += S_[cell](row_side * NV + a, col_side * NV + b);
+        }
+    }
+    void recover(const Vec &dl, Vec &du) const {  // hybrid.cpp:128-137
+        du.assign(static_cast<size_t>(p_.size()), 0.0);
+        for (int cell = 0; cell < p_.cells; ++cell)
+            for (int k = 0; k < m_; ++k) {
+                double acc = 0;
+                for (int side = 0; side < 2; ++side)
+                    for (int v = 0; v < NV; ++v) acc += X_[cell](k, side * NV + v) * dl[face(cell, side) * NV + v];
+                du[static_cast<size_t>(cell) * m_ + k] = -y_[cell][k] - acc;
+            }
+    }
+    void recovered_residual(const Vec &dl, Vec &out) const {  // hybrid.cpp:116-127
+        Vec e;
+        apply(dl, e);
+        for (size_t k = 0; k < e.size(); ++k) e[k] -= rhs_[k];
+        out.assign(static_cast<size_t>(p_.size()), 0.0);
+        for (int cell = 0; cell < p_.cells; ++cell) {
+            double g[2 * NV];
+            for (int side = 0; side < 2; ++side) {
+                const Speed sp = p_.speeds[face(cell, side)];
+                for (int v = 0; v < NV; ++v) g[side * NV + v] = e[face(cell, side) * NV + v] / -(sp.a + sp.b);
+            }
+            for (int k = 0; k < m_; ++k) {
+                double acc = 0;
+                for (int j = 0; j < 2 * NV; ++j) acc += B_[cell](k, j) * g[j];
+                out[static_cast<size_t>(cell) * m_ + k] = -acc;
+            }
+        }
+    }
+
+  private:
+    const Problem &p_;
+    DGOperator dg_;
+    int m_;
+    std::vector<Mat> X_, C_, S_, B_;
+    std::vector<Vec> y_;
+    Vec rhs_;
+    bool ready_ = false;
+};
+
 struct Update {
     int step, stage, iteration, krylov;
     double time, stage_time, dt, cfl, rms, damping, linear_relative, dg_linear_relative;
@@ -473,8 +625,11 @@ struct Run {
     std::vector<Update> updates;
 };
 
-// time_integrator.cpp:45-207, DG with GMRES(restart) + block Jacobi
-Run advance(Problem &p, Vec &solution) {
+// time_integrator.cpp:45-207 with GMRES(restart) + block Jacobi: DG on the
+// element blocks (metric M^-1), or condensed HDG on the 3x3 face blocks
+// (unweighted metric, time_integrator.cpp:36) with the 4-attempt Schur
+// refinement of :137-153
+Run advance(Problem &p, Vec &solution, bool hdg = false) {
     const Config &c = p.c;
     require(static_cast<int>(solution.size()) == p.size() && p.admissible(solution), "Invalid initial solution");
     require(c.steps > 0 || c.final_time < p.example.shock_time(), "Final time reaches simple-wave shock");
@@ -488,6 +643,10 @@ Run advance(Problem &p, Vec &solution) {
     DGOperator dg(p);
     BlockJacobi pc(p.cells, p.m);
     GMRES gmres(D, c.restart, p.inverse_mass);
+    Hybrid hybrid(p);
+    BlockJacobi face_pc(p.cells, NV);
+    GMRES trace_gmres(NV * p.cells, c.restart, Vec(static_cast<size_t>(NV) * p.cells, 1.0));
+    Vec trace_direction(static_cast<size_t>(NV) * p.cells);
     p.freeze(solution);
     const State initial_integral = p.integral(solution);
     Vec state = solution, trial(D), residual(D), trial_residual(D), direction(D), rhs(D), step_state(D),
@@ -530,14 +689,36 @@ Run advance(Problem &p, Vec &solution) {
                 const double allowance = 128 * std::numeric_limits<double>::epsilon() *
                                          ((p.state_norm(state) + old_norm) / tau + norm);
                 const double target = std::max(10 * c.linear_tolerance * norm, allowance);
-                dg.refresh(state);
-                for (int k = 0; k < D; ++k) rhs[k] = -residual[k];
-                pc.refresh([&](int r, int cl, Mat &blk) { dg.block(r, cl, blk); });
-                const LinearResult linear = gmres.solve([&](const Vec &a, Vec &b) { dg.apply(a, b); },
-                                                        [&](const Vec &a, Vec &b) { pc.apply(a, b); }, rhs,
-                                                        direction, c.max_linear, c.linear_tolerance);
-                const Vec &gr = gmres.residual();
-                for (int k = 0; k < D; ++k) equation_error[k] = -gr[k];
+                LinearResult linear{0, 0};
+                if (!hdg) {
+                    dg.refresh(state);
+                    for (int k = 0; k < D; ++k) rhs[k] = -residual[k];
+                    pc.refresh([&](int r, int cl, Mat &blk) { dg.block(r, cl, blk); });
+                    linear = gmres.solve([&](const Vec &a, Vec &b) { dg.apply(a, b); },
+                                         [&](const Vec &a, Vec &b) { pc.apply(a, b); }, rhs, direction,
+                                         c.max_linear, c.linear_tolerance);
+                    const Vec &gr = gmres.residual();
+                    for (int k = 0; k < D; ++k) equation_error[k] = -gr[k];
+                } else {
+                    hybrid.refresh_projected(state, residual);
+                    face_pc.refresh([&](int r, int cl, Mat &blk) { hybrid.block(r, cl, blk); });
+                    double tolerance = c.linear_tolerance;
+                    bool valid = false;
+                    for (int attempt = 0; attempt < 4; ++attempt) {
+                        const LinearResult cur =
+                            trace_gmres.solve([&](const Vec &a, Vec &b) { hybrid.apply(a, b); },
+                                              [&](const Vec &a, Vec &b) { face_pc.apply(a, b); }, hybrid.rhs(),
+                                              trace_direction, c.max_linear, tolerance);
+                        linear.iterations += cur.iterations;
+                        linear.relative_residual = cur.relative_residual;
+                        hybrid.recover(trace_direction, direction);
+                        hybrid.recovered_residual(trace_direction, equation_error);
+                        valid = p.dual_norm(equation_error) <= target;
+                        if (valid) break;
+                        tolerance *= .01;
+                    }
+                    require(valid, "Recovered DG residual criterion failed after Schur refinement");
+                }
                 const double error_norm = p.dual_norm(equation_error);
                 require(error_norm <= target, "DG Newton linear residual too large");
                 double damping = 1.;
@@ -593,9 +774,12 @@ struct Context {
     Problem problem;
     DGOperator dg;
     BlockJacobi pc;
+    Hybrid hybrid;
+    BlockJacobi face_pc;
     Run last;
     std::string error;
-    explicit Context(const Config &c) : problem(c), dg(problem), pc(c.cells, NV * (c.degree + 1)) {}
+    explicit Context(const Config &c)
+        : problem(c), dg(problem), pc(c.cells, NV * (c.degree + 1)), hybrid(problem), face_pc(c.cells, NV) {}
 };
 }  // namespace oracle1d
 
@@ -732,11 +916,63 @@ int orc1_integral(void *h, const double *u, double *out3) {
 // steps, newton, krylov, drift, max_linear, max_dg_linear, min_step, max_step.
 // updates: up to cap rows of (step, stage, iteration, krylov, time, stage_time,
 // dt, cfl, rms, damping, linear_relative, dg_linear_relative).
-int orc1_advance(void *h, double *u, double *summary, double *updates, int cap, int *count) {
+// condensed HDG (hybrid.cpp, 1D): trace vectors are face * 3 + v
+int orc1_hdg_refresh(void *h, const double *u, const double *r) {
+    auto *c = static_cast<Context *>(h);
+    return guarded(c, [&] {
+        c->hybrid.refresh_projected(in(c, u), in(c, r));
+        c->face_pc.refresh([&](int a, int b, oracle::Mat &blk) { c->hybrid.block(a, b, blk); });
+    });
+}
+int orc1_hdg_rhs(void *h, double *out3c) {
+    auto *c = static_cast<Context *>(h);
+    return guarded(c, [&] { out(c->hybrid.rhs(), out3c); });
+}
+int orc1_hdg_apply(void *h, const double *x, double *y) {
+    auto *c = static_cast<Context *>(h);
+    return guarded(c, [&] {
+        Vec v;
+        c->hybrid.apply(Vec(x, x + 3 * c->problem.cells), v);
+        out(v, y);
+    });
+}
+int orc1_hdg_block(void *h, int row, int col, double *out9) {
+    auto *c = static_cast<Context *>(h);
+    return guarded(c, [&] {
+        oracle::Mat b;
+        c->hybrid.block(row, col, b);
+        out(b.a, out9);
+    });
+}
+int orc1_hdg_bj_apply(void *h, const double *r, double *z) {
+    auto *c = static_cast<Context *>(h);
+    return guarded(c, [&] {
+        Vec v;
+        c->face_pc.apply(Vec(r, r + 3 * c->problem.cells), v);
+        out(v, z);
+    });
+}
+int orc1_hdg_recover(void *h, const double *dl, double *du) {
+    auto *c = static_cast<Context *>(h);
+    return guarded(c, [&] {
+        Vec v;
+        c->hybrid.recover(Vec(dl, dl + 3 * c->problem.cells), v);
+        out(v, du);
+    });
+}
+int orc1_hdg_recovered_residual(void *h, const double *dl, double *res) {
+    auto *c = static_cast<Context *>(h);
+    return guarded(c, [&] {
+        Vec v;
+        c->hybrid.recovered_residual(Vec(dl, dl + 3 * c->problem.cells), v);
+        out(v, res);
+    });
+}
+int orc1_advance(void *h, int hdg, double *u, double *summary, double *updates, int cap, int *count) {
     auto *c = static_cast<Context *>(h);
     return guarded(c, [&] {
         Vec v = in(c, u);
-        c->last = oracle1d::advance(c->problem, v);
+        c->last = oracle1d::advance(c->problem, v, hdg != 0);
         out(v, u);
         const auto &s = c->last;
         const double sm[9] = {s.final_time, double(s.steps), double(s.newton), double(s.krylov), s.conservation_drift,

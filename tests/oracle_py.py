@@ -375,12 +375,46 @@ class Oracle1D:
         self._call("orc1_integral", _p(u), _p(e))
         return e
 
-    def advance(self, u, cap=100000):
+    def _trace(self, name, x, size):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(size)
+        self._call(name, _p(x), _p(y))
+        return y
+
+    def hdg_refresh(self, u, r):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        self._call("orc1_hdg_refresh", _p(u), _p(r))
+
+    def hdg_rhs(self):
+        y = np.zeros(3 * self.cells)
+        self._call("orc1_hdg_rhs", _p(y))
+        return y
+
+    def hdg_apply(self, x):
+        return self._trace("orc1_hdg_apply", x, 3 * self.cells)
+
+    def hdg_block(self, row, col):
+        out = np.zeros(9)
+        self._call("orc1_hdg_block", C.c_int(row), C.c_int(col), _p(out))
+        return out.reshape(3, 3, order="F")
+
+    def hdg_bj_apply(self, r):
+        return self._trace("orc1_hdg_bj_apply", r, 3 * self.cells)
+
+    def hdg_recover(self, dl):
+        return self._trace("orc1_hdg_recover", dl, self.size)
+
+    def hdg_recovered_residual(self, dl):
+        return self._trace("orc1_hdg_recovered_residual", dl, self.size)
+
+    def advance(self, u, cap=100000, method="dg"):
         u = np.array(u, dtype=np.float64, copy=True)
         summary = np.zeros(9)
         upd = np.zeros((cap, 12))
         count = C.c_int()
-        self._call("orc1_advance", _p(u), _p(summary), _p(upd), C.c_int(cap), C.byref(count))
+        self._call("orc1_advance", C.c_int(1 if method == "hdg" else 0), _p(u), _p(summary), _p(upd),
+                   C.c_int(cap), C.byref(count))
         keys = ["final_time", "steps", "newton_iterations", "krylov_iterations", "conservation_drift",
                 "max_linear_residual", "max_dg_linear_residual", "minimum_step", "maximum_step"]
         stats = dict(zip(keys, summary.tolist()))

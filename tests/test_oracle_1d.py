@@ -8,7 +8,12 @@
   make the residual smooth in u), collocated, over-integrated and k = 0;
 * the element blocks against the matrix-free Jacobian on unit vectors, and
   the block-Jacobi apply against a dense solve of each diagonal block;
-* GMRES's true residual against its tolerance (gmres.cpp:78-83).
+* GMRES's true residual against its tolerance (gmres.cpp:78-83);
+* the condensed HDG operator (hybrid.cpp, 1D): the reference's HDG rows of the
+  same table (Krylov counts included), the DG/HDG agreement of the final
+  states, the recovered-residual identity J recover(dl) + r =
+  recovered_residual(dl) against the pinned DG Jacobian, and the face blocks
+  against the Schur apply on unit vectors.
 """
 import json
 import os
@@ -42,6 +47,42 @@ def test_convergence_table(cells):
     err = o.error_l2(uN, st["final_time"])
     for e, key in zip(err, ["rho_L2", "momentum_L2", "energy_L2"]):
         assert abs(e - float(ref[key])) <= 2e-9 * float(ref[key]), (key, e, ref[key])
+
+
+@pytest.mark.parametrize("cells", [32, 64, 128])
+def test_hdg_convergence_table(cells):
+    ref = [r for r in GOLD["rows"] if r["method"] == "hdg" and int(r["cells"]) == cells][0]
+    o = O.Oracle1D(protocol(cells))
+    uN, st = o.advance(o.initial(), method="hdg")
+    assert int(st["steps"]) == int(ref["steps"])
+    assert int(st["newton_iterations"]) == int(ref["newton_iterations"])
+    assert abs(int(st["krylov_iterations"]) - int(ref["krylov_iterations"])) <= 2
+    assert st["conservation_drift"] <= 1e-14
+    err = o.error_l2(uN, st["final_time"])
+    for e, key in zip(err, ["rho_L2", "momentum_L2", "energy_L2"]):
+        assert abs(e - float(ref[key])) <= 2e-9 * float(ref[key]), (key, e, ref[key])
+    uD, _ = O.Oracle1D(protocol(cells)).advance(o.initial())
+    assert np.abs(uD - uN).max() <= 1e-13 and float(ref["difference_max_nodal"]) <= 1e-13
+
+
+@pytest.mark.parametrize("args", [["--cells", "6", "--k", "2"], ["--cells", "5", "--k", "3", "--quad", "5"],
+                                  ["--cells", "4", "--k", "0"]], ids=["k2", "k3-q5", "k0"])
+def test_hdg_identities(args):
+    o, u = setup(args)
+    r = o.residual(u)
+    o.refresh(u)
+    o.hdg_refresh(u, r)
+    dl = np.random.default_rng(1).standard_normal(3 * o.cells)
+    rr = o.hdg_recovered_residual(dl)
+    assert np.linalg.norm(o.apply(o.hdg_recover(dl)) + r - rr) <= 1e-13 * np.linalg.norm(rr)
+    C = o.cells
+    blocks = np.zeros((3 * C, 3 * C))
+    for a in range(C):
+        for b in range(C):
+            blocks[3 * a:3 * a + 3, 3 * b:3 * b + 3] = o.hdg_block(a, b)
+    E = np.eye(3 * C)
+    applied = np.stack([o.hdg_apply(E[:, j]) for j in range(3 * C)], axis=1)
+    assert np.abs(blocks - applied).max() <= 1e-14 * np.abs(applied).max()
 
 
 def setup(args, cfl=2.0):
