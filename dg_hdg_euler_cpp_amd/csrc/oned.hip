@@ -553,6 +553,271 @@ __global__ __launch_bounds__(CELL_BLOCK) void k1_bj_apply(const double *r, doubl
     if (e < cells) bj_cell<N>(e, r, z, factors, rowsrc);
 }
 
+// ------------------------------------------------------ condensed HDG
+// hybrid.cpp:42-81 (1D) for cell e: A = volume block + M / tau + the HDG face
+// terms shape_i shape_j d_s(v, w) with d_s = sign_s F'(U_s) + stab_s I (side 0:
+// sign -1, stab b of face e-1; side 1: sign +1, stab a of face e); B(v n + i,
+// 3 s + v) = -shape_i stab_s; C(3 s + w, v n + i) = shape_i d_s(w, v). The
+// partial-pivot LU of A (kept in factors) gives X = A^-1 B (m x 6), y = A^-1 r;
+// S = -C X (6 x 6) and C y (the cell's part of the Schur right-hand side).
+template <int N>
+__device__ __forceinline__ void hdg_side(const double *u, int e, int side, const double *speeds, const Line &L,
+                                         double d[9], double &stab, const double *&shape) {
+    const int face = side == 0 ? prev_cell(e, L.cells) : e;
+    shape = side == 0 ? L.left : L.right;
+    stab = side == 0 ? speeds[2 * face + 1] : speeds[2 * face];
+    const double sign = side == 0 ? -1.0 : 1.0;
+    double z[3], J[9];
+    sample1<N>(u, e, shape, z);
+    jacobian1(z, L.gamma, J);
+    for (int k = 0; k < 9; ++k) d[k] = sign * J[k] + stab * (k % 4 == 0 ? 1.0 : 0.0);
+}
+// permuted unit-lower + upper solve with one cell's packed-in-place LU
+template <int M>
+__device__ __forceinline__ void lu_solve_cell(const double *A, const int *perm, const double *rhs, double *out) {
+    double y[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) y[i] = rhs[perm[i]];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        const double yj = y[j];
+#pragma unroll
+        for (int i = j + 1; i < M; ++i) y[i] -= A[i + M * j] * yj;
+    }
+#pragma unroll
+    for (int j = M - 1; j >= 0; --j) {
+        y[j] /= A[j + M * j];
+        const double yj = y[j];
+#pragma unroll
+        for (int i = 0; i < j; ++i) y[i] -= A[i + M * j] * yj;
+    }
+#pragma unroll
+    for (int i = 0; i < M; ++i) out[i] = y[i];
+}
+// in-place partial-pivot LU of an m x m column-major block (dense_lu.hpp:10-46);
+// false when singular or not finite
+__device__ bool lu_in_place(double *A, int *perm, int m) {
+    auto at = [&](int i, int j) -> double & { return A[i + m * j]; };
+    for (int i = 0; i < m; ++i) perm[i] = i;
+    for (int k = 0; k < m; ++k) {
+        int piv = k;
+        double best = fabs(at(k, k));
+        for (int i = k + 1; i < m; ++i)
+            if (fabs(at(i, k)) > best) {
+                best = fabs(at(i, k));
+                piv = i;
+            }
+        if (piv != k) {
+            for (int j = 0; j < m; ++j) {
+                const double t = at(k, j);
+                at(k, j) = at(piv, j);
+                at(piv, j) = t;
+            }
+            const int t = perm[k];
+            perm[k] = perm[piv];
+            perm[piv] = t;
+        }
+        const double pivot = at(k, k);
+        if (pivot != 0)
+            for (int i = k + 1; i < m; ++i) at(i, k) /= pivot;
+        for (int j = k + 1; j < m; ++j) {
+            const double ukj = at(k, j);
+            for (int i = k + 1; i < m; ++i) at(i, j) -= at(i, k) * ukj;
+        }
+    }
+    bool ok = true;
+    for (int k = 0; k < m * m; ++k) ok = ok && isfinite(A[k]);
+    for (int k = 0; k < m; ++k) ok = ok && fabs(at(k, k)) > 2.2250738585072014e-308;
+    return ok;
+}
+template <int N>
+__global__ __launch_bounds__(CELL_BLOCK) void k1_condense(const double *u, const double *r, const double *speeds,
+                                                          const double *coef, double inv_step, double *factors,
+                                                          int *rowsrc, double *X, double *Y, double *S, double *CY,
+                                                          int *flag, const Line *__restrict__ LP) {
+    constexpr int m = 3 * N;
+    const Line &L = *LP;
+    const int e = blockIdx.x * CELL_BLOCK + threadIdx.x;
+    if (e >= L.cells) return;
+    double *A = factors + static_cast<long long>(e) * m * m;
+    auto at = [&](int i, int j) -> double & { return A[i + m * j]; };
+    // volume block (volume_jacobian.cpp:105-132), as k1_block without its faces
+    for (int w = 0; w < 3; ++w)
+        for (int j = 0; j < N; ++j)
+            for (int v = 0; v < 3; ++v)
+                for (int i = 0; i < N; ++i) {
+                    double value = 0;
+                    if (N > 1) {
+                        if (L.Q == N) {
+                            value = -coef[(static_cast<long long>(e) * N + j) * 9 + v + 3 * w] * L.wd[j + N * i];
+                        } else {
+                            double s = 0;
+                            for (int q = 0; q < L.Q; ++q)
+                                s += L.wd[q + L.Q * i] *
+                                     (coef[(static_cast<long long>(e) * L.Q + q) * 9 + v + 3 * w] * L.vq[q + L.Q * j]);
+                            value = -s;
+                        }
+                    }
+                    at(v * N + i, w * N + j) = value;
+                }
+    for (int k = 0; k < m; ++k) at(k, k) += L.mass[k % N] * inv_step;
+    double d[2][9], stab[2];
+    const double *shape[2];
+    for (int side = 0; side < 2; ++side) {
+        hdg_side<N>(u, e, side, speeds, L, d[side], stab[side], shape[side]);
+        for (int v = 0; v < 3; ++v)
+            for (int i = 0; i < N; ++i)
+                for (int w = 0; w < 3; ++w)
+                    for (int j = 0; j < N; ++j)
+                        at(v * N + i, w * N + j) += shape[side][i] * shape[side][j] * d[side][v + 3 * w];
+    }
+    int *perm = rowsrc + static_cast<long long>(e) * m;
+    bool ok = true;
+    for (int k = 0; k < m * m; ++k) ok = ok && isfinite(A[k]);
+    if (!ok) {
+        atomicOr(flag, 16);
+        return;
+    }
+    if (!lu_in_place(A, perm, m)) {
+        atomicOr(flag, 32);
+        return;
+    }
+    double *Xe = X + static_cast<long long>(e) * m * 6, *Ye = Y + static_cast<long long>(e) * m;
+    double col[m];
+    for (int side = 0; side < 2; ++side)
+        for (int v = 0; v < 3; ++v) {
+            for (int k = 0; k < m; ++k) col[k] = 0;
+            for (int i = 0; i < N; ++i) col[v * N + i] = -shape[side][i] * stab[side];
+            lu_solve_cell<m>(A, perm, col, Xe + m * (side * 3 + v));
+        }
+    lu_solve_cell<m>(A, perm, r + static_cast<long long>(e) * m, Ye);
+    // C(3 s + w, v N + i) = shape_s[i] d_s(w, v)
+    auto Cm = [&](int a, int k) {
+        const int side = a / 3, w = a % 3, v = k / N, i = k % N;
+        return shape[side][i] * d[side][w + 3 * v];
+    };
+    double *Se = S + static_cast<long long>(e) * 36, *CYe = CY + static_cast<long long>(e) * 6;
+    for (int j = 0; j < 6; ++j)
+        for (int a = 0; a < 6; ++a) {
+            double acc = 0;
+            for (int k = 0; k < m; ++k) acc += Cm(a, k) * Xe[k + m * j];
+            Se[a + 6 * j] = -acc;
+        }
+    for (int a = 0; a < 6; ++a) {
+        double acc = 0;
+        for (int k = 0; k < m; ++k) acc += Cm(a, k) * Ye[k];
+        CYe[a] = acc;
+    }
+}
+// Schur right-hand side of face f: the C y of its two cells in the cell order
+// of hybrid.cpp:76-80 (face cells-1 gets cell 0's part first)
+__global__ void k1_hdg_rhs(const double *CY, double *rhs, int cells) {
+    const int f = blockIdx.x * CELL_BLOCK + threadIdx.x;
+    if (f >= cells) return;
+    const int nx = f + 1 == cells ? 0 : f + 1;
+    for (int w = 0; w < 3; ++w) {
+        const double own = CY[6 * f + 3 + w], next = CY[6 * nx + w];
+        rhs[3 * f + w] = f + 1 == cells ? (0.0 + next) + own : (0.0 + own) + next;
+    }
+}
+// HybridOperator::apply (hybrid.cpp:82-101) for face f: -(a + b) x_f plus the
+// S of its two cells on their [x_{c-1}; x_c], in the reference's cell order
+__device__ __forceinline__ void schur_face(int f, const double *x, double *y, const double *S, const double *speeds,
+                                           int cells) {
+    const int nx = f + 1 == cells ? 0 : f + 1;
+    double acc[3];
+    for (int w = 0; w < 3; ++w) acc[w] = -(speeds[2 * f] + speeds[2 * f + 1]) * x[3 * f + w];
+    auto add = [&](int cell, int side) {
+        const int p = cell == 0 ? cells - 1 : cell - 1;
+        const double g[6] = {x[3 * p], x[3 * p + 1], x[3 * p + 2], x[3 * cell], x[3 * cell + 1], x[3 * cell + 2]};
+        const double *Sc = S + static_cast<long long>(cell) * 36;
+        for (int w = 0; w < 3; ++w) {
+            double s = 0;
+            for (int k = 0; k < 6; ++k) s += Sc[side * 3 + w + 6 * k] * g[k];
+            acc[w] += s;
+        }
+    };
+    if (f + 1 == cells) {
+        add(nx, 0);
+        add(f, 1);
+    } else {
+        add(f, 1);
+        add(nx, 0);
+    }
+    for (int w = 0; w < 3; ++w) y[3 * f + w] = acc[w];
+}
+__global__ void k1_schur_apply(const double *x, double *y, const double *S, const double *speeds, int cells) {
+    const int f = blockIdx.x * CELL_BLOCK + threadIdx.x;
+    if (f < cells) schur_face(f, x, y, S, speeds, cells);
+}
+// HybridOperator::block(f, f) (hybrid.cpp:102-115): -(a + b) I + S_f(side 1,
+// side 1) + S_{f+1}(side 0, side 0); written to blocks (3x3 column-major) and,
+// when factorize, replaced by its partial-pivot LU
+__global__ void k1_face_blocks(const double *S, const double *speeds, double *blocks, int *perm, int *flag,
+                               int factorize, int cells) {
+    const int f = blockIdx.x * CELL_BLOCK + threadIdx.x;
+    if (f >= cells) return;
+    const int nx = f + 1 == cells ? 0 : f + 1;
+    double *B = blocks + 9 * f;
+    for (int b = 0; b < 3; ++b)
+        for (int a = 0; a < 3; ++a) B[a + 3 * b] = a == b ? -(speeds[2 * f] + speeds[2 * f + 1]) : 0.0;
+    for (int b = 0; b < 3; ++b)
+        for (int a = 0; a < 3; ++a) B[a + 3 * b] += S[36LL * f + 3 + a + 6 * (3 + b)];
+    for (int b = 0; b < 3; ++b)
+        for (int a = 0; a < 3; ++a) B[a + 3 * b] += S[36LL * nx + a + 6 * b];
+    if (factorize && !lu_in_place(B, perm + 3 * f, 3)) atomicOr(flag, 8);
+}
+__device__ __forceinline__ void face_bj(int f, const double *r, double *z, const double *fac, const int *perm) {
+    lu_solve_cell<3>(fac + 9 * f, perm + 3 * f, r + 3 * f, z + 3 * f);
+}
+__global__ void k1_face_bj_apply(const double *r, double *z, const double *fac, const int *perm, int cells) {
+    const int f = blockIdx.x * CELL_BLOCK + threadIdx.x;
+    if (f < cells) face_bj(f, r, z, fac, perm);
+}
+// HybridOperator::recover (hybrid.cpp:128-137): du = -y - X [dl_{e-1}; dl_e]
+template <int N>
+__global__ __launch_bounds__(CELL_BLOCK) void k1_recover(const double *dl, double *du, const double *X,
+                                                         const double *Y, int cells) {
+    constexpr int m = 3 * N;
+    const int e = blockIdx.x * CELL_BLOCK + threadIdx.x;
+    if (e >= cells) return;
+    const int p = e == 0 ? cells - 1 : e - 1;
+    const double g[6] = {dl[3 * p], dl[3 * p + 1], dl[3 * p + 2], dl[3 * e], dl[3 * e + 1], dl[3 * e + 2]};
+    const double *Xe = X + static_cast<long long>(e) * m * 6, *Ye = Y + static_cast<long long>(e) * m;
+    for (int k = 0; k < m; ++k) {
+        double acc = 0;
+        for (int j = 0; j < 6; ++j) acc += Xe[k + m * j] * g[j];
+        du[static_cast<long long>(e) * m + k] = -Ye[k] - acc;
+    }
+}
+// HybridOperator::recovered_residual (hybrid.cpp:116-127): with e = S dl - rhs
+// (in err), each side's e divided by -(a + b) of its face, lifted by -B
+template <int N>
+__global__ __launch_bounds__(CELL_BLOCK) void k1_recovered(const double *err, const double *rhs, double *out,
+                                                           const double *speeds, const Line *__restrict__ LP) {
+    const Line &L = *LP;
+    const int e = blockIdx.x * CELL_BLOCK + threadIdx.x;
+    if (e >= L.cells) return;
+    const int p = prev_cell(e, L.cells);
+    double g[6];
+    for (int side = 0; side < 2; ++side) {
+        const int f = side == 0 ? p : e;
+        for (int v = 0; v < 3; ++v)
+            g[side * 3 + v] = (err[3 * f + v] - rhs[3 * f + v]) / -(speeds[2 * f] + speeds[2 * f + 1]);
+    }
+    const double st[2] = {speeds[2 * p + 1], speeds[2 * e]};
+    for (int v = 0; v < 3; ++v)
+        for (int i = 0; i < N; ++i) {
+            double acc = 0;
+            for (int j = 0; j < 6; ++j) {
+                const int side = j / 3;
+                const double b = j % 3 == v ? -(side == 0 ? L.left[i] : L.right[i]) * st[side] : 0.0;
+                acc += b * g[j];
+            }
+            out[static_cast<long long>(e) * 3 * N + v * N + i] = -acc;
+        }
+}
+
 // ------------------------------------------------------------ reductions
 // Sum over the workgroup's threads of count (<= 32) per-thread values: a butterfly
 // within each wave, then the 16 wave sums in wave order. Deterministic.
@@ -638,8 +903,11 @@ struct SolveArgs {
     double inv_step, tol;
     long long D;
     int restart, limit;
+    // condensed HDG: the trace system with the cells' S, the face speeds and
+    // the face-block LUs (weights inv_mass = 1)
+    const double *S, *speeds;
 };
-template <int N>
+template <int N, bool HDG>
 __global__ __launch_bounds__(SOLVE_THREADS) void k1_gmres(SolveArgs a, const Line *__restrict__ LP) {
     const Line &L = *LP;
     __shared__ double red[SOLVE_WAVES][32], dots[32];
@@ -656,11 +924,17 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k1_gmres(SolveArgs a, const Lin
         return sqrt(fmax(0.0, dots[0]));
     };
     auto precondition = [&](const double *in, double *out) {
-        for (int e = t; e < cells; e += SOLVE_THREADS) bj_cell<N>(e, in, out, a.factors, a.rowsrc);
+        for (int e = t; e < cells; e += SOLVE_THREADS) {
+            if (HDG) face_bj(e, in, out, a.factors, a.rowsrc);
+            else bj_cell<N>(e, in, out, a.factors, a.rowsrc);
+        }
         __syncthreads();
     };
     auto jacobian = [&](const double *in, double *out) {
-        for (int e = t; e < cells; e += SOLVE_THREADS) apply_cell<N>(e, in, out, a.coef, a.fl, a.fr, a.inv_step, L);
+        for (int e = t; e < cells; e += SOLVE_THREADS) {
+            if (HDG) schur_face(e, in, out, a.S, a.speeds, cells);
+            else apply_cell<N>(e, in, out, a.coef, a.fl, a.fr, a.inv_step, L);
+        }
         __syncthreads();
     };
     for (long long i = t; i < D; i += SOLVE_THREADS) {
@@ -830,9 +1104,11 @@ struct dgh1_ctx {
     Buf<Line> line;
     Buf<double> mass, inv_mass, old, speeds, coef, fl, fr, factors, scratch, V, w, z, res, result, sums;
     Buf<double> state, trial, residual, trial_residual, direction, rhs, step_state, stage_base, slopes;
-    Buf<int> rowsrc, flag;
+    Buf<int> rowsrc, flag, face_perm;
     Buf<unsigned long long> rate_bits;
-    bool frozen = false, refreshed = false, factored = false;
+    // condensed HDG: X, y, S, C y per cell; face-block LUs; trace vectors
+    Buf<double> hX, hY, hS, hCY, face_lu, trhs, tdir, terr, ones;
+    bool frozen = false, refreshed = false, factored = false, hdg_ready = false;
     std::string error;
 
     explicit dgh1_ctx(const dgh1_config &c) : cfg(c) {
@@ -897,15 +1173,20 @@ struct dgh1_ctx {
         slopes.alloc(4 * D);
         result.alloc(4);
         sums.alloc(4);
+        face_perm.alloc(3 * cells);
+        for (auto *b2 : {&trhs, &tdir, &terr, &ones}) b2->alloc(3 * cells);
+        const std::vector<double> one(3 * cells, 1.0);
+        HIPCHECK1(hipMemcpy(ones.p, one.data(), 3 * cells * sizeof(double), hipMemcpyHostToDevice));
         flag.alloc(1);
         rate_bits.alloc(1);
     }
     ~dgh1_ctx() {
         for (auto *b : {&mass, &inv_mass, &old, &speeds, &coef, &fl, &fr, &factors, &scratch, &V, &w, &z, &res,
                         &result, &sums, &state, &trial, &residual, &trial_residual, &direction, &rhs, &step_state,
-                        &stage_base, &slopes})
+                        &stage_base, &slopes, &hX, &hY, &hS, &hCY, &face_lu, &trhs, &tdir, &terr, &ones})
             b->release();
         rowsrc.release();
+        face_perm.release();
         flag.release();
         rate_bits.release();
         line.release();
@@ -933,6 +1214,7 @@ struct dgh1_ctx {
         require1(!(read_flag() & 1), "Nonphysical cell average");
         require1(admissible(u), "Invalid projected initial state");
     }
+    long long trace_size() const { return 3LL * cells; }
     double freeze(const double *u, double *host_speeds = nullptr) {
         copy(old.p, u);
         clear_flag();
@@ -952,13 +1234,13 @@ struct dgh1_ctx {
         if (host_speeds)
             HIPCHECK1(hipMemcpy(host_speeds, speeds.p, 2 * cells * sizeof(double), hipMemcpyDeviceToHost));
         frozen = true;
-        refreshed = factored = false;
+        refreshed = factored = hdg_ready = false;
         return rate;
     }
     void set_step(double t, const double *base) {
         tau = t;
         copy(old.p, base);
-        refreshed = factored = false;
+        refreshed = factored = hdg_ready = false;
     }
     void residual_of(const double *u, double *r) {
         require1(frozen && tau > 0, "Freeze the state and set the step before the residual");
@@ -1037,8 +1319,8 @@ struct dgh1_ctx {
         require1(factored, "Refresh the block Jacobi preconditioner first");
         const int restart = static_cast<int>(std::min<long long>(D, cfg.restart));
         SolveArgs a{coef.p, fl.p, fr.p, factors.p, inv_mass.p, rowsrc.p, b, x, V.p, w.p, z.p, res.p, result.p,
-                    1 / tau, tol, D, restart, limit};
-        DGH1_DISPATCH(n, { k1_gmres<N><<<1, SOLVE_THREADS, 0, stream>>>(a, line.p); })
+                    1 / tau, tol, D, restart, limit, nullptr, nullptr};
+        DGH1_DISPATCH(n, { k1_gmres<N, false><<<1, SOLVE_THREADS, 0, stream>>>(a, line.p); })
         HIPCHECK1(hipGetLastError());
         double r[3];
         HIPCHECK1(hipMemcpyAsync(r, result.p, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -1051,9 +1333,85 @@ struct dgh1_ctx {
         require1(status != 3, "GMRES iteration limit reached");
     }
 
-    // ---- TimeIntegrator::advance (time_integrator.cpp:45-207), method dg
+    // ---- condensed HDG (hybrid.cpp, 1D)
+    void hdg_refresh(const double *u, const double *r) {
+        require1(frozen && tau > 0, "Set the time step before refreshing the volume Jacobian");
+        refresh(u);  // the volume Jacobian F'(U) at the points
+        hX.alloc(static_cast<size_t>(cells) * m * 6);
+        hY.alloc(static_cast<size_t>(cells) * m);
+        hS.alloc(36 * static_cast<size_t>(cells));
+        hCY.alloc(6 * static_cast<size_t>(cells));
+        face_lu.alloc(9 * static_cast<size_t>(cells));
+        clear_flag();
+        DGH1_DISPATCH(n, {
+            k1_condense<N><<<cell_grid(), CELL_BLOCK, 0, stream>>>(u, r, speeds.p, coef.p, 1 / tau, factors.p,
+                                                                   rowsrc.p, hX.p, hY.p, hS.p, hCY.p, flag.p, line.p);
+        })
+        k1_hdg_rhs<<<cell_grid(), CELL_BLOCK, 0, stream>>>(hCY.p, trhs.p, cells);
+        k1_face_blocks<<<cell_grid(), CELL_BLOCK, 0, stream>>>(hS.p, speeds.p, face_lu.p, face_perm.p, flag.p, 1, cells);
+        HIPCHECK1(hipGetLastError());
+        const int f = read_flag();
+        require1(!(f & 16), "Nonfinite block LU input");
+        require1(!(f & 32), "Singular or nonfinite LU factors");
+        require1(!(f & 8), "Singular block Jacobi factor");
+        factored = false;  // the cell LUs of the condensation reuse the block-Jacobi storage
+        hdg_ready = true;
+    }
+    void hdg_apply(const double *x, double *y) {
+        require1(hdg_ready, "Refresh the hybrid operator first");
+        k1_schur_apply<<<cell_grid(), CELL_BLOCK, 0, stream>>>(x, y, hS.p, speeds.p, cells);
+        HIPCHECK1(hipGetLastError());
+    }
+    void hdg_block(int face, double *host_out) {
+        require1(hdg_ready, "Refresh the hybrid operator first");
+        require1(face >= 0 && face < cells, "Block row out of range");
+        Buf<double> tmp;
+        tmp.alloc(9 * static_cast<size_t>(cells));
+        k1_face_blocks<<<cell_grid(), CELL_BLOCK, 0, stream>>>(hS.p, speeds.p, tmp.p, face_perm.p, flag.p, 0, cells);
+        HIPCHECK1(hipGetLastError());
+        HIPCHECK1(hipMemcpyAsync(host_out, tmp.p + 9 * face, 9 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        sync();
+        tmp.release();
+    }
+    void hdg_bj_apply(const double *r, double *zz) {
+        require1(hdg_ready, "Refresh the hybrid operator first");
+        k1_face_bj_apply<<<cell_grid(), CELL_BLOCK, 0, stream>>>(r, zz, face_lu.p, face_perm.p, cells);
+        HIPCHECK1(hipGetLastError());
+    }
+    void hdg_recover(const double *dl, double *du) {
+        require1(hdg_ready, "Refresh the hybrid operator first");
+        DGH1_DISPATCH(n, { k1_recover<N><<<cell_grid(), CELL_BLOCK, 0, stream>>>(dl, du, hX.p, hY.p, cells); })
+        HIPCHECK1(hipGetLastError());
+    }
+    void hdg_recovered(const double *dl, double *out) {
+        hdg_apply(dl, terr.p);
+        DGH1_DISPATCH(n, { k1_recovered<N><<<cell_grid(), CELL_BLOCK, 0, stream>>>(terr.p, trhs.p, out, speeds.p, line.p); })
+        HIPCHECK1(hipGetLastError());
+    }
+    // GMRES::solve on the trace system, unweighted (time_integrator.cpp:36)
+    void hdg_gmres(const double *b, double *x, int limit, double tol, int &its, double &rel) {
+        require1(hdg_ready, "Refresh the hybrid operator first");
+        const long long T = trace_size();
+        const int restart = static_cast<int>(std::min<long long>(T, cfg.restart));
+        SolveArgs a{nullptr, nullptr, nullptr, face_lu.p, ones.p, face_perm.p, b, x, V.p, w.p, z.p, res.p,
+                    result.p, 1 / tau, tol, T, restart, limit, hS.p, speeds.p};
+        DGH1_DISPATCH(n, { k1_gmres<N, true><<<1, SOLVE_THREADS, 0, stream>>>(a, line.p); })
+        HIPCHECK1(hipGetLastError());
+        double r[3];
+        HIPCHECK1(hipMemcpyAsync(r, result.p, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        sync();
+        its = static_cast<int>(r[0]);
+        rel = r[1];
+        const int status = static_cast<int>(r[2]);
+        require1(status != 1, "GMRES Arnoldi breakdown");
+        require1(status != 2, "Nonfinite GMRES residual");
+        require1(status != 3, "GMRES iteration limit reached");
+    }
+
+    // ---- TimeIntegrator::advance (time_integrator.cpp:45-207), method dg or hdg
     void advance(double *solution, dgh_run_stats &stats, std::vector<dgh_update_stats> &updates) {
         const dgh1_config &c = cfg;
+        const bool hdg = c.method == DGH_METHOD_HDG;  // dg and both run DG
         require1(admissible(solution), "Invalid initial solution");
         const double shock = 4 / ((3.0 + 1) * std::acos(-1.0));
         require1(c.steps > 0 || c.final_time < shock, "Final time reaches simple-wave shock");
@@ -1107,13 +1465,35 @@ struct dgh1_ctx {
                     const double allowance = 128 * std::numeric_limits<double>::epsilon() *
                                              ((state_norm(state.p) + old_norm) / tau + norm);
                     const double target = std::max(10 * c.linear_tolerance * norm, allowance);
-                    refresh(state.p);
-                    k1_axpby<<<vec_grid(), 256, 0, stream>>>(rhs.p, -1.0, residual.p, 0.0, residual.p, D);
-                    bj_refresh();
                     int its = 0;
                     double rel = 0;
-                    gmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance, its, rel);
-                    const double error_norm = dual_norm(res.p);  // equation error = -(rhs - J x)
+                    const double *equation_error = res.p;  // DG: -(rhs - J x), same norm
+                    if (!hdg) {
+                        refresh(state.p);
+                        k1_axpby<<<vec_grid(), 256, 0, stream>>>(rhs.p, -1.0, residual.p, 0.0, residual.p, D);
+                        bj_refresh();
+                        gmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance, its, rel);
+                    } else {
+                        // time_integrator.cpp:125-153: projected condensation, then up to four
+                        // Schur solves, each tightening ltol by 100 until the recovered DG
+                        // residual meets the target
+                        hdg_refresh(state.p, residual.p);
+                        double tolerance = c.linear_tolerance;
+                        bool valid = false;
+                        for (int attempt = 0; attempt < 4; ++attempt) {
+                            int cur = 0;
+                            hdg_gmres(trhs.p, tdir.p, c.max_linear, tolerance, cur, rel);
+                            its += cur;
+                            hdg_recover(tdir.p, direction.p);
+                            hdg_recovered(tdir.p, scratch.p);
+                            valid = dual_norm(scratch.p) <= target;
+                            if (valid) break;
+                            tolerance *= .01;
+                        }
+                        require1(valid, "Recovered DG residual criterion failed after Schur refinement");
+                        equation_error = scratch.p;
+                    }
+                    const double error_norm = dual_norm(equation_error);
                     require1(error_norm <= target, "DG Newton linear residual too large");
                     double damping = 1.;
                     bool accepted = false;
@@ -1319,6 +1699,11 @@ int dgh1_nodes(const dgh1_ctx *ctx, double *x) {
     for (int i = 0; i < ctx->n; ++i) x[i] = nodes[i];
     return DGH_OK;
 }
+int dgh1_weights(const dgh1_ctx *ctx, double *w) {
+    if (!ctx) return DGH_ERROR;
+    for (int i = 0; i < ctx->n; ++i) w[i] = ctx->hline.w[i];
+    return DGH_OK;
+}
 int dgh1_initial(dgh1_ctx *ctx, double *u) {
     return run1(ctx, [&] {
         ctx->initial(u);
@@ -1382,6 +1767,52 @@ int dgh1_error(dgh1_ctx *ctx, const double *u, double t, double *error3) {
 }
 int dgh1_integral(dgh1_ctx *ctx, const double *u, double *integral3) {
     return run1(ctx, [&] { ctx->integrals(u, 0, 0, integral3); });
+}
+int dgh1_hdg_refresh(dgh1_ctx *ctx, const double *u, const double *r) {
+    return run1(ctx, [&] { ctx->hdg_refresh(u, r); });
+}
+int dgh1_hdg_rhs(dgh1_ctx *ctx, double *rhs) {
+    return run1(ctx, [&] {
+        require1(ctx->hdg_ready, "Refresh the hybrid operator first");
+        HIPCHECK1(hipMemcpyAsync(rhs, ctx->trhs.p, ctx->trace_size() * sizeof(double), hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+        ctx->sync();
+    });
+}
+int dgh1_hdg_apply(dgh1_ctx *ctx, const double *x, double *y) {
+    return run1(ctx, [&] {
+        ctx->hdg_apply(x, y);
+        ctx->sync();
+    });
+}
+int dgh1_hdg_block(dgh1_ctx *ctx, int face, double *out) { return run1(ctx, [&] { ctx->hdg_block(face, out); }); }
+int dgh1_hdg_bj_apply(dgh1_ctx *ctx, const double *r, double *z) {
+    return run1(ctx, [&] {
+        ctx->hdg_bj_apply(r, z);
+        ctx->sync();
+    });
+}
+int dgh1_hdg_recover(dgh1_ctx *ctx, const double *dl, double *du) {
+    return run1(ctx, [&] {
+        ctx->hdg_recover(dl, du);
+        ctx->sync();
+    });
+}
+int dgh1_hdg_recovered_residual(dgh1_ctx *ctx, const double *dl, double *out) {
+    return run1(ctx, [&] {
+        ctx->hdg_recovered(dl, out);
+        ctx->sync();
+    });
+}
+int dgh1_hdg_gmres(dgh1_ctx *ctx, const double *rhs, double *x, int max_iterations, double tolerance,
+                   int *iterations, double *relative_residual) {
+    return run1(ctx, [&] {
+        int its = 0;
+        double rel = 0;
+        ctx->hdg_gmres(rhs, x, max_iterations, tolerance, its, rel);
+        *iterations = its;
+        *relative_residual = rel;
+    });
 }
 int dgh1_advance(dgh1_ctx *ctx, double *u, dgh_run_stats *run, dgh_update_stats *updates, int capacity,
                  int *count) {

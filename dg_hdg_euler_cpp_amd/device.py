@@ -527,6 +527,51 @@ class DeviceContext1D:
         self._call("dgh1_integral", _ptr(u), e)
         return list(e)
 
+    def tzeros(self):
+        return self._torch.zeros(3 * self.cells, dtype=self._torch.float64, device="cuda")
+
+    def hdg_refresh(self, u, r):
+        self._call("dgh1_hdg_refresh", _ptr(u), _ptr(r))
+
+    def hdg_rhs(self):
+        y = self.tzeros()
+        self._call("dgh1_hdg_rhs", _ptr(y))
+        return y
+
+    def hdg_apply(self, x):
+        y = self.tzeros()
+        self._call("dgh1_hdg_apply", _ptr(x), _ptr(y))
+        return y
+
+    def hdg_block(self, face):
+        out = (C.c_double * 9)()
+        self._call("dgh1_hdg_block", C.c_int(face), out)
+        import numpy as np
+
+        return np.array(out).reshape(3, 3, order="F")
+
+    def hdg_bj_apply(self, r):
+        z = self.tzeros()
+        self._call("dgh1_hdg_bj_apply", _ptr(r), _ptr(z))
+        return z
+
+    def hdg_recover(self, dl):
+        du = self.zeros()
+        self._call("dgh1_hdg_recover", _ptr(dl), _ptr(du))
+        return du
+
+    def hdg_recovered_residual(self, dl):
+        out = self.zeros()
+        self._call("dgh1_hdg_recovered_residual", _ptr(dl), _ptr(out))
+        return out
+
+    def hdg_gmres(self, rhs, limit, tol):
+        x = self.tzeros()
+        its, rel = C.c_int(), C.c_double()
+        self._call("dgh1_hdg_gmres", _ptr(rhs), _ptr(x), C.c_int(limit), C.c_double(tol), C.byref(its),
+                   C.byref(rel))
+        return x, its.value, rel.value
+
     def advance(self, u, capacity=100000):
         run = RunStats()
         ups = (UpdateStats * capacity)()

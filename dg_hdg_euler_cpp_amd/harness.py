@@ -19,9 +19,9 @@ reference's drivers and applies their acceptance rules:
   updates, warm-up flags 0,1,1,1, the summary mean equals the mean of the
   three measured iteration times).
 * accuracy1d: accuracy_test/1d/convergence.py:19-47 with `bin/dg_hdg_1d` --
-  k=2, DIRK3, CFL 2, T 0.2, GMRES(20) + block Jacobi per grid, log2 orders per
-  component, written to convergence.csv; the device's 1D path runs method dg,
-  so the rows are the reference table's dg rows (no DG/HDG difference columns).
+  k=2, DIRK3, CFL 2, T 0.2, GMRES(20) + block Jacobi, DG and HDG paired per
+  grid, log2 orders per component and the DG/HDG difference columns, written
+  to convergence.csv.
 A SUCCESS file is written when every run passed; any failure raises.
 """
 from __future__ import annotations
@@ -124,23 +124,29 @@ def accuracy1d(output: Path, exe: Path, grids):
     for cells in grids:
         prefix = output / f"n{cells}"
         print(f"Start N={cells}, k=2, DIRK3, CFL=2, T=0.2", flush=True)
-        run([exe, "--cells", cells, "--k", 2, "--integrator", "ssp-dirk3", "--cfl", 2, "--T", .2, "--method", "dg",
+        run([exe, "--cells", cells, "--k", 2, "--integrator", "ssp-dirk3", "--cfl", 2, "--T", .2, "--method", "both",
              "--precond", "block-jacobi", "--ordering", "natural", "--ntol", "1e-12", "--ltol", "1e-12",
              "--restart", 20, "--max-linear", 1500, "--max-newton", 30, "--safety", 1.1, "--prefix", prefix], prefix)
+        difference = rows(f"{prefix}_difference.csv")[0]
         summaries = rows(f"{prefix}_summary.csv")
-        require([r["method"] for r in summaries] == ["dg"], "Missing solver results")
-        data = summaries[0]
-        require(float(data["final_time"]) == .2 and data["integrator"] == "ssp-dirk3"
-                and int(data["fixed_newton_steps"]) == 0, "Unexpected convergence protocol")
-        for component in ("rho", "momentum", "energy"):
-            error = float(data[f"{component}_L2"])
-            require(math.isfinite(error) and error > 0, "Nonfinite/zero convergence error")
-            data[f"{component}_order"] = math.log2(previous[component] / error) if component in previous else ""
-            previous[component] = error
-        records.append(data)
+        require([r["method"] for r in summaries] == ["dg", "hdg"], "Missing paired solver results")
+        for data in summaries:
+            method = data["method"]
+            require(float(data["final_time"]) == .2 and data["integrator"] == "ssp-dirk3"
+                    and int(data["fixed_newton_steps"]) == 0, "Unexpected convergence protocol")
+            for component in ("rho", "momentum", "energy"):
+                error = float(data[f"{component}_L2"])
+                require(math.isfinite(error) and error > 0, "Nonfinite/zero convergence error")
+                key = method, component
+                data[f"{component}_order"] = math.log2(previous[key] / error) if key in previous else ""
+                previous[key] = error
+                data[f"difference_{component}_L2"] = difference[f"{component}_L2"]
+            data["difference_joint_L2"] = difference["joint_L2"]
+            data["difference_max_nodal"] = difference["max_nodal"]
+            records.append(data)
         write_csv(output / "convergence.csv", records)
         print(f"N={cells} completed", flush=True)
-    (output / "SUCCESS").write_text("All requested 1D DG grids completed.\n")
+    (output / "SUCCESS").write_text("All requested paired 1D grids completed.\n")
     return records
 
 

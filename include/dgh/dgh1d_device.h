@@ -16,8 +16,9 @@
  * reference's message text from dgh1_last_error(); vectors are DEVICE pointers
  * in the reference layout cell*3n + var*n + i (types.hpp:24-41 of dg_hdg_1d);
  * one context owns its buffers and one HIP stream, is synchronous and not
- * thread-safe. The device path runs method dg with GMRES(restart) and block
- * Jacobi (--linear gmres --precond block-jacobi), the reference's 1D protocol.
+ * thread-safe. The device path runs method dg or hdg with GMRES(restart) and
+ * block Jacobi (--linear gmres --precond block-jacobi), the reference's 1D
+ * protocol.
  */
 #ifndef DGH1D_DEVICE_H
 #define DGH1D_DEVICE_H
@@ -38,7 +39,7 @@ typedef struct dgh1_config {
     double cfl, final_time, safety;
     double nonlinear_tolerance, linear_tolerance;
     int integrator; /* 0 be, 1 ssp-dirk3 */
-    int method;     /* enum dgh_method; the device runs dg */
+    int method;     /* enum dgh_method; a context runs dg or hdg */
     int timing_only;
     char prefix[512];
 } dgh1_config;
@@ -59,6 +60,9 @@ long long dgh1_size(const dgh1_ctx *ctx);
 /* reference-element Gauss nodes (nodal_basis.hpp:34-45), n = k+1 values into
  * HOST memory, for the node coordinates of the solution output (main.cpp:91-94) */
 int dgh1_nodes(const dgh1_ctx *ctx, double *x);
+/* the matching Gauss weights (nodal_basis.hpp:34-45), n values into HOST
+ * memory, for the mass-weighted DG/HDG difference (main.cpp:86-94) */
+int dgh1_weights(const dgh1_ctx *ctx, double *w);
 /* Problem::initial (discretization.cpp:31-73): projected simple wave with
  * the cell-mean scaling. */
 int dgh1_initial(dgh1_ctx *ctx, double *u);
@@ -88,8 +92,31 @@ int dgh1_gmres(dgh1_ctx *ctx, const double *rhs, double *x, int max_iterations, 
 int dgh1_error(dgh1_ctx *ctx, const double *u, double t, double *error3);
 /* Problem::integral (discretization.cpp:148-155) */
 int dgh1_integral(dgh1_ctx *ctx, const double *u, double *integral3);
-/* TimeIntegrator::advance (time_integrator.cpp:45-207), method dg: u is
- * advanced in place; updates (host) may be NULL. */
+/* ---- condensed HDG (dg_hdg_1d/src/hybrid.cpp); trace vectors are face*3 + var,
+ * face f between cells f and f+1 ---------------------------------------- */
+/* HybridOperator::refresh_projected (hybrid.cpp:37-81) from the state and its
+ * DG residual, plus the face block-Jacobi LUs */
+int dgh1_hdg_refresh(dgh1_ctx *ctx, const double *u, const double *r);
+/* HybridOperator::rhs(), 3*cells values */
+int dgh1_hdg_rhs(dgh1_ctx *ctx, double *rhs);
+/* HybridOperator::apply (hybrid.cpp:82-101): the Schur operator */
+int dgh1_hdg_apply(dgh1_ctx *ctx, const double *x, double *y);
+/* HybridOperator::block(face, face) (hybrid.cpp:102-115), 3x3 column-major HOST */
+int dgh1_hdg_block(dgh1_ctx *ctx, int face, double *out);
+/* BlockJacobi over the face blocks (block_jacobi.hpp:24-39) */
+int dgh1_hdg_bj_apply(dgh1_ctx *ctx, const double *r, double *z);
+/* HybridOperator::recover (hybrid.cpp:128-137): du = -y - X dl */
+int dgh1_hdg_recover(dgh1_ctx *ctx, const double *dl, double *du);
+/* HybridOperator::recovered_residual (hybrid.cpp:116-127) */
+int dgh1_hdg_recovered_residual(dgh1_ctx *ctx, const double *dl, double *out);
+/* GMRES::solve on the Schur system with the face block Jacobi, unweighted
+ * (time_integrator.cpp:36) */
+int dgh1_hdg_gmres(dgh1_ctx *ctx, const double *rhs, double *x, int max_iterations, double tolerance,
+                   int *iterations, double *relative_residual);
+/* TimeIntegrator::advance (time_integrator.cpp:45-207) for the configured
+ * method (hdg: condensed HDG with the Schur refinement of :125-153; dg and
+ * both: DG): u is advanced in place; updates (host) may be NULL. The CLI runs
+ * --method both as one context per method. */
 int dgh1_advance(dgh1_ctx *ctx, double *u, dgh_run_stats *run, dgh_update_stats *updates, int capacity,
                  int *count);
 

@@ -143,7 +143,7 @@ def test_1d_cli_reports_errors_like_the_reference():
     exe = os.path.join(ROOT, "dg_hdg_euler_cpp_amd", "bin", "dg_hdg_1d")
     p = subprocess.run([exe, "--cells", "2"], capture_output=True, text=True)
     assert p.returncode == 1 and p.stderr.strip() == "ERROR: Invalid mesh or degree"
-    p = subprocess.run([exe, "--cells", "8"], capture_output=True, text=True)
-    assert p.returncode == 1 and p.stderr.strip() == "ERROR: The 1D device path runs the DG method: use --method dg"
+    p = subprocess.run([exe, "--precond", "block-ilu0"], capture_output=True, text=True)
+    assert p.returncode == 1 and p.stderr.strip() == "ERROR: The 1D device path preconditions with block Jacobi"
     p = subprocess.run([exe, "--help"], capture_output=True, text=True)
     assert p.returncode == 0 and "--cells" in p.stdout

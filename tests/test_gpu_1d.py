@@ -149,3 +149,99 @@ def test_1d_convergence_protocol_matches_reference(cells, torch_cuda):
     assert st["conservation_drift"] <= 1e-14
     for e, key in zip(err, ["rho_L2", "momentum_L2", "energy_L2"]):
         assert abs(e - float(ref[key])) <= 2e-9 * float(ref[key]), (key, e, ref[key])
+
+
+HDG_CASES = [("k2-32", ["--cells", "32", "--k", "2"]), ("k3-q5-7", ["--cells", "7", "--k", "3", "--quad", "5"]),
+             ("k0-6", ["--cells", "6", "--k", "0"]), ("k8-5", ["--cells", "5", "--k", "8"])]
+
+
+@pytest.mark.parametrize("name,args", HDG_CASES, ids=[c[0] for c in HDG_CASES])
+def test_1d_hdg_operators_match_oracle(name, args, torch_cuda):
+    """hybrid.cpp (1D) on the same state and DG residual: Schur right-hand
+    side, apply, face blocks, face block Jacobi, recovery, recovered residual
+    and the Schur GMRES."""
+    from dg_hdg_euler_cpp_amd import DeviceContext1D
+
+    torch = torch_cuda
+    print(f"\n[case] 1d hdg operators {args}", flush=True)
+    orc = O.Oracle1D(args)
+    with DeviceContext1D(args) as dev:
+        u_o = orc.initial()
+        u = torch.tensor(u_o, device="cuda")
+        rate = orc.freeze(u_o)
+        dev.freeze(u)
+        tau = 2.0 / rate
+        old = u_o * (1 + 1e-3 * np.sin(np.arange(u_o.size)))
+        orc.set_step(tau, old)
+        dev.set_step(tau, torch.tensor(old, device="cuda"))
+        r_o = orc.residual(u_o)
+        r = torch.tensor(r_o, device="cuda")
+        orc.refresh(u_o)
+        orc.hdg_refresh(u_o, r_o)
+        dev.hdg_refresh(u, r)
+        tol = 1e-12 if name != "k8-5" else 1e-10  # k=8 element blocks: kappa ~1e4
+        g_o = orc.hdg_rhs()
+        print(f"rhs rel {rel(dev.hdg_rhs().cpu().numpy(), g_o):.3e}")
+        assert rel(dev.hdg_rhs().cpu().numpy(), g_o) <= tol
+        dl = np.random.default_rng(4).standard_normal(3 * orc.cells)
+        dlt = torch.tensor(dl, device="cuda")
+        assert rel(dev.hdg_apply(dlt).cpu().numpy(), orc.hdg_apply(dl)) <= tol
+        for f in [0, orc.cells // 2, orc.cells - 1]:
+            assert rel(dev.hdg_block(f), orc.hdg_block(f, f)) <= tol, f
+        assert rel(dev.hdg_bj_apply(dlt).cpu().numpy(), orc.hdg_bj_apply(dl)) <= 100 * tol
+        assert rel(dev.hdg_recover(dlt).cpu().numpy(), orc.hdg_recover(dl)) <= tol
+        assert rel(dev.hdg_recovered_residual(dlt).cpu().numpy(), orc.hdg_recovered_residual(dl)) <= tol
+        x_d, its, rel_d = dev.hdg_gmres(torch.tensor(g_o, device="cuda"), 1500, 1e-12)
+        assert rel_d <= 1e-12 and its > 0
+        res = orc.hdg_apply(x_d.cpu().numpy()) - g_o
+        assert np.linalg.norm(res) <= 2e-12 * np.linalg.norm(g_o)
+
+
+HDG_RUNS = [("be-k2-16", ["--cells", "16", "--k", "2", "--integrator", "be", "--T", "0.05"]),
+            ("dirk3-k3-q5-12", ["--cells", "12", "--k", "3", "--quad", "5", "--T", "0.05"])]
+
+
+@pytest.mark.parametrize("name,args", HDG_RUNS, ids=[r[0] for r in HDG_RUNS])
+def test_1d_hdg_runs_match_oracle(name, args, torch_cuda):
+    from dg_hdg_euler_cpp_amd import DeviceContext1D
+
+    full = args + ["--cfl", "2", "--ntol", "1e-12", "--ltol", "1e-12", "--restart", "20", "--method", "hdg"]
+    print(f"\n[case] 1d hdg run {full}", flush=True)
+    orc = O.Oracle1D(full)
+    uN_o, st_o = orc.advance(orc.initial(), method="hdg")
+    with DeviceContext1D(full) as dev:
+        u = dev.initial()
+        st = dev.advance(u)
+        uN = u.cpu().numpy()
+    assert len(st["updates"]) == len(st_o["updates"])
+    diffs = []
+    for got, want in zip(st["updates"], st_o["updates"]):
+        w = want["residual_physical_rms"]
+        assert abs(got["residual_physical_rms"] - w) <= max(1e-8 * w, 1e-13), (got, want)
+        diffs.append(int(got["krylov_iterations"] - want["krylov_iterations"]))
+    print(f"Krylov differences per update {diffs}; final state rel {rel(uN, uN_o):.3e}")
+    assert max(abs(d) for d in diffs) <= 2
+    assert rel(uN, uN_o) <= 1e-11
+    assert st["conservation_drift"] <= 1e-14
+
+
+@pytest.mark.parametrize("cells", [32, 64, 128])
+def test_1d_hdg_convergence_protocol_matches_reference(cells, torch_cuda):
+    """The reference table's hdg rows (convergence_slurm_7622710.csv)."""
+    from dg_hdg_euler_cpp_amd import DeviceContext1D
+
+    ref = [r for r in GOLD["rows"] if r["method"] == "hdg" and int(r["cells"]) == cells][0]
+    args = ["--cells", str(cells)] + GOLD["args"] + ["--method", "hdg"]
+    with DeviceContext1D(args) as dev:
+        u = dev.initial()
+        st = dev.advance(u)
+        err = dev.error_l2(u, st["final_time"])
+    print(f"\n[case] 1d hdg convergence {cells}: steps {st['steps']} Newton {st['newton_iterations']} "
+          f"Krylov {st['krylov_iterations']} (reference {ref['steps']}, {ref['newton_iterations']}, "
+          f"{ref['krylov_iterations']}), {st['seconds']:.3f} s", flush=True)
+    assert st["steps"] == int(ref["steps"]) and st["final_time"] == 0.2
+    assert st["newton_iterations"] == int(ref["newton_iterations"])
+    assert abs(st["krylov_iterations"] - int(ref["krylov_iterations"])) <= 2
+    assert st["conservation_drift"] <= 1e-14
+    for e, key in zip(err, ["rho_L2", "momentum_L2", "energy_L2"]):
+        assert abs(e - float(ref[key])) <= 2e-9 * float(ref[key]), (key, e, ref[key])
