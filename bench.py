@@ -96,26 +96,30 @@ def run_ilu(args, nx, k, init, precision="fp64"):
 def run_1d():
     """The reference's 1D accuracy protocol at 128 cells (accuracy_test/1d/
     convergence.py:19-25: k=2, DIRK3, CFL 2, T 0.2, GMRES(20) + block Jacobi)
-    on the device 1D path, one warm-up run then one timed run; beside it the
-    reference's archived single-core time for the same run
-    (convergence_slurm_7622710.csv:6, a different CPU, not a same-run figure)."""
+    on the device 1D path, DG and HDG, each one warm-up run then one timed run;
+    beside them the reference's archived single-core times for the same runs
+    (convergence_slurm_7622710.csv:6-7, a different CPU, not same-run figures)."""
     from dg_hdg_euler_cpp_amd import DeviceContext1D
 
     args = ["--cells", "128", "--k", "2", "--integrator", "ssp-dirk3", "--cfl", "2", "--T", "0.2", "--method", "dg",
             "--ntol", "1e-12", "--ltol", "1e-12", "--restart", "20", "--max-linear", "1500", "--max-newton", "30",
             "--safety", "1.1"]
-    with DeviceContext1D(args) as dev:
-        dev.advance(dev.initial())
-        u = dev.initial()
-        st = dev.advance(u)
-        err = dev.error_l2(u, st["final_time"])
-    log(f"[bench] 1d dg 128 cells k=2 DIRK3: {st['newton_iterations']} updates, {st['krylov_iterations']} Krylov, "
-        f"{st['seconds']:.4f} s")
-    return {"workload": "1D DG, 128 cells, k=2, DIRK3, CFL 2, T 0.2, GMRES(20) + block Jacobi",
-            "time_loop_s": st["seconds"], "newton_updates": st["newton_iterations"],
-            "krylov_iterations": st["krylov_iterations"], "ms_per_update": 1e3 * st["seconds"] / st["newton_iterations"],
-            "rho_L2": err[0], "reference_time_loop_s_published": 0.45550448400000004,
-            "reference_mean_newton_s_published": 0.0028767540316455686}
+    out = {"workload": "1D, 128 cells, k=2, DIRK3, CFL 2, T 0.2, GMRES(20) + block Jacobi"}
+    published = {"dg": (0.45550448400000004, 0.0028767540316455686), "hdg": (0.116675362, 0.00073255362025316452)}
+    for method in ("dg", "hdg"):
+        with DeviceContext1D(args[:-1] + [method]) as dev:
+            dev.advance(dev.initial())
+            u = dev.initial()
+            st = dev.advance(u)
+            err = dev.error_l2(u, st["final_time"])
+        log(f"[bench] 1d {method} 128 cells k=2 DIRK3: {st['newton_iterations']} updates, "
+            f"{st['krylov_iterations']} Krylov, {st['seconds']:.4f} s")
+        out[method] = {"time_loop_s": st["seconds"], "newton_updates": st["newton_iterations"],
+                       "krylov_iterations": st["krylov_iterations"],
+                       "ms_per_update": 1e3 * st["seconds"] / st["newton_iterations"], "rho_L2": err[0],
+                       "reference_time_loop_s_published": published[method][0],
+                       "reference_mean_newton_s_published": published[method][1]}
+    return out
 
 
 def run_precision_check(args, nx, k, init, precision):
