@@ -487,7 +487,7 @@ def main():
         "hdg_fp64": hdg,
         "ilu0_redblack_fp64": ilu,
         "ilu0_redblack_mixed": ilu_mixed,
-        "oned_dg": oned,
+        "oned": oned,
         "jv": {"dof_per_s": g["size"] / dev["jv_avg_s"], "achieved_GBs": jv_bytes(g) / dev["jv_avg_s"] / 1e9,
                "avg_launch_s": dev["jv_avg_s"]},
         "cpu_baseline": None if cpu is None else {k: v for k, v in cpu.items() if k != "seconds_per_update"},
