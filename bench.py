@@ -101,13 +101,13 @@ def run_1d():
     (convergence_slurm_7622710.csv:6-7, a different CPU, not same-run figures)."""
     from dg_hdg_euler_cpp_amd import DeviceContext1D
 
-    args = ["--cells", "128", "--k", "2", "--integrator", "ssp-dirk3", "--cfl", "2", "--T", "0.2", "--method", "dg",
+    args = ["--cells", "128", "--k", "2", "--integrator", "ssp-dirk3", "--cfl", "2", "--T", "0.2",
             "--ntol", "1e-12", "--ltol", "1e-12", "--restart", "20", "--max-linear", "1500", "--max-newton", "30",
             "--safety", "1.1"]
     out = {"workload": "1D, 128 cells, k=2, DIRK3, CFL 2, T 0.2, GMRES(20) + block Jacobi"}
     published = {"dg": (0.45550448400000004, 0.0028767540316455686), "hdg": (0.116675362, 0.00073255362025316452)}
     for method in ("dg", "hdg"):
-        with DeviceContext1D(args[:-1] + [method]) as dev:
+        with DeviceContext1D(args + ["--method", method]) as dev:
             dev.advance(dev.initial())
             u = dev.initial()
             st = dev.advance(u)
