@@ -436,142 +436,6 @@ __global__ __launch_bounds__(CELL_BLOCK) void k1_apply(const double *x, double *
     if (e < LP->cells) apply_cell<N>(e, x, y, coef, fl, fr, inv_step, *LP);
 }
 
-// DGOperator::block(row, row) (dg_operator.cpp:50-79, volume_jacobian.cpp:
-// 105-132), column-major 3n x 3n with index v*n + i, then (factorize != 0) the
-// partial-pivot LU of dense_lu.hpp:10-46 in place: max |a| pivot, first row on
-// ties, row interchanges across the full row, unit-lower L; rowsrc holds
-// (P b)[i] = b[rowsrc[i]].
-template <int N>
-__global__ __launch_bounds__(CELL_BLOCK) void k1_block(const double *coef, const double *fl, const double *fr,
-                                                       double inv_step, double *blocks, int *rowsrc, int *flag,
-                                                       int factorize, const Line *__restrict__ LP) {
-    constexpr int m = 3 * N;
-    const Line &L = *LP;
-    const int row = blockIdx.x * CELL_BLOCK + threadIdx.x;
-    if (row >= L.cells) return;
-    double *A = blocks + static_cast<long long>(row) * m * m;
-    auto at = [&](int i, int j) -> double & { return A[i + m * j]; };
-    for (int w = 0; w < 3; ++w)
-        for (int j = 0; j < N; ++j)
-            for (int v = 0; v < 3; ++v)
-                for (int i = 0; i < N; ++i) {
-                    double value = 0;
-                    if (N > 1) {
-                        if (L.Q == N) {
-                            value = -coef[(static_cast<long long>(row) * N + j) * 9 + v + 3 * w] * L.wd[j + N * i];
-                        } else {
-                            double s = 0;
-                            for (int q = 0; q < L.Q; ++q)
-                                s += L.wd[q + L.Q * i] *
-                                     (coef[(static_cast<long long>(row) * L.Q + q) * 9 + v + 3 * w] * L.vq[q + L.Q * j]);
-                            value = -s;
-                        }
-                    }
-                    at(v * N + i, w * N + j) = value;
-                }
-    for (int k = 0; k < m; ++k) at(k, k) += L.mass[k % N] * inv_step;
-    // left face (row - 1 | row): the cell is its right element, trial left, R
-    // derivative, test -left; right face (row | row + 1): trial right, L
-    // derivative, test +right
-    const int p = prev_cell(row, L.cells);
-    for (int side = 0; side < 2; ++side) {
-        const double *test = side == 0 ? L.left : L.right, *trial = side == 0 ? L.left : L.right;
-        const double sign = side == 0 ? -1.0 : 1.0;
-        const double *d = side == 0 ? fr + 9 * p : fl + 9 * row;
-        for (int w = 0; w < 3; ++w)
-            for (int j = 0; j < N; ++j)
-                for (int v = 0; v < 3; ++v)
-                    for (int i = 0; i < N; ++i) at(v * N + i, w * N + j) += (sign * trial[j] * d[v + 3 * w]) * test[i];
-    }
-    if (!factorize) return;
-    int *perm = rowsrc + static_cast<long long>(row) * m;
-    for (int i = 0; i < m; ++i) perm[i] = i;
-    for (int k = 0; k < m; ++k) {
-        int piv = k;
-        double best = fabs(at(k, k));
-        for (int i = k + 1; i < m; ++i)
-            if (fabs(at(i, k)) > best) {
-                best = fabs(at(i, k));
-                piv = i;
-            }
-        if (piv != k) {
-            for (int j = 0; j < m; ++j) {
-                const double t = at(k, j);
-                at(k, j) = at(piv, j);
-                at(piv, j) = t;
-            }
-            const int t = perm[k];
-            perm[k] = perm[piv];
-            perm[piv] = t;
-        }
-        const double pivot = at(k, k);
-        if (pivot != 0)
-            for (int i = k + 1; i < m; ++i) at(i, k) /= pivot;
-        for (int j = k + 1; j < m; ++j) {
-            const double ukj = at(k, j);
-            for (int i = k + 1; i < m; ++i) at(i, j) -= at(i, k) * ukj;
-        }
-    }
-    bool ok = true;
-    for (int k = 0; k < m * m; ++k) ok = ok && isfinite(A[k]);
-    for (int k = 0; k < m; ++k) ok = ok && fabs(at(k, k)) > 2.2250738585072014e-308;
-    if (!ok) atomicOr(flag, 8);
-}
-
-// BlockJacobi::apply of one block (block_jacobi.hpp:24-39): permute, unit
-// lower solve, upper solve
-template <int N>
-__device__ __forceinline__ void bj_cell(int e, const double *r, double *z, const double *factors, const int *rowsrc) {
-    constexpr int m = 3 * N;
-    const double *A = factors + static_cast<long long>(e) * m * m;
-    const int *perm = rowsrc + static_cast<long long>(e) * m;
-    const double *rb = r + static_cast<long long>(e) * m;
-    double y[m];
-#pragma unroll
-    for (int i = 0; i < m; ++i) y[i] = rb[perm[i]];
-#pragma unroll
-    for (int j = 0; j < m; ++j) {
-        const double yj = y[j];
-#pragma unroll
-        for (int i = j + 1; i < m; ++i) y[i] -= A[i + m * j] * yj;
-    }
-#pragma unroll
-    for (int j = m - 1; j >= 0; --j) {
-        y[j] /= A[j + m * j];
-        const double yj = y[j];
-#pragma unroll
-        for (int i = 0; i < j; ++i) y[i] -= A[i + m * j] * yj;
-    }
-    double *zb = z + static_cast<long long>(e) * m;
-#pragma unroll
-    for (int i = 0; i < m; ++i) zb[i] = y[i];
-}
-template <int N>
-__global__ __launch_bounds__(CELL_BLOCK) void k1_bj_apply(const double *r, double *z, const double *factors,
-                                                          const int *rowsrc, int cells) {
-    const int e = blockIdx.x * CELL_BLOCK + threadIdx.x;
-    if (e < cells) bj_cell<N>(e, r, z, factors, rowsrc);
-}
-
-// ------------------------------------------------------ condensed HDG
-// hybrid.cpp:42-81 (1D) for cell e: A = volume block + M / tau + the HDG face
-// terms shape_i shape_j d_s(v, w) with d_s = sign_s F'(U_s) + stab_s I (side 0:
-// sign -1, stab b of face e-1; side 1: sign +1, stab a of face e); B(v n + i,
-// 3 s + v) = -shape_i stab_s; C(3 s + w, v n + i) = shape_i d_s(w, v). The
-// partial-pivot LU of A (kept in factors) gives X = A^-1 B (m x 6), y = A^-1 r;
-// S = -C X (6 x 6) and C y (the cell's part of the Schur right-hand side).
-template <int N>
-__device__ __forceinline__ void hdg_side(const double *u, int e, int side, const double *speeds, const Line &L,
-                                         double d[9], double &stab, const double *&shape) {
-    const int face = side == 0 ? prev_cell(e, L.cells) : e;
-    shape = side == 0 ? L.left : L.right;
-    stab = side == 0 ? speeds[2 * face + 1] : speeds[2 * face];
-    const double sign = side == 0 ? -1.0 : 1.0;
-    double z[3], J[9];
-    sample1<N>(u, e, shape, z);
-    jacobian1(z, L.gamma, J);
-    for (int k = 0; k < 9; ++k) d[k] = sign * J[k] + stab * (k % 4 == 0 ? 1.0 : 0.0);
-}
 // permuted unit-lower + upper solve with one cell's packed-in-place LU
 template <int M>
 __device__ __forceinline__ void lu_solve_cell(const double *A, const int *perm, const double *rhs, double *out) {
@@ -629,6 +493,91 @@ __device__ bool lu_in_place(double *A, int *perm, int m) {
     for (int k = 0; k < m * m; ++k) ok = ok && isfinite(A[k]);
     for (int k = 0; k < m; ++k) ok = ok && fabs(at(k, k)) > 2.2250738585072014e-308;
     return ok;
+}
+// DGOperator::block(row, row) (dg_operator.cpp:50-79, volume_jacobian.cpp:
+// 105-132), column-major 3n x 3n with index v*n + i, then (factorize != 0) the
+// partial-pivot LU of dense_lu.hpp:10-46 in place: max |a| pivot, first row on
+// ties, row interchanges across the full row, unit-lower L; rowsrc holds
+// (P b)[i] = b[rowsrc[i]].
+template <int N>
+__global__ __launch_bounds__(CELL_BLOCK) void k1_block(const double *coef, const double *fl, const double *fr,
+                                                       double inv_step, double *blocks, int *rowsrc, int *flag,
+                                                       int factorize, const Line *__restrict__ LP) {
+    constexpr int m = 3 * N;
+    const Line &L = *LP;
+    const int row = blockIdx.x * CELL_BLOCK + threadIdx.x;
+    if (row >= L.cells) return;
+    double *A = blocks + static_cast<long long>(row) * m * m;
+    auto at = [&](int i, int j) -> double & { return A[i + m * j]; };
+    for (int w = 0; w < 3; ++w)
+        for (int j = 0; j < N; ++j)
+            for (int v = 0; v < 3; ++v)
+                for (int i = 0; i < N; ++i) {
+                    double value = 0;
+                    if (N > 1) {
+                        if (L.Q == N) {
+                            value = -coef[(static_cast<long long>(row) * N + j) * 9 + v + 3 * w] * L.wd[j + N * i];
+                        } else {
+                            double s = 0;
+                            for (int q = 0; q < L.Q; ++q)
+                                s += L.wd[q + L.Q * i] *
+                                     (coef[(static_cast<long long>(row) * L.Q + q) * 9 + v + 3 * w] * L.vq[q + L.Q * j]);
+                            value = -s;
+                        }
+                    }
+                    at(v * N + i, w * N + j) = value;
+                }
+    for (int k = 0; k < m; ++k) at(k, k) += L.mass[k % N] * inv_step;
+    // left face (row - 1 | row): the cell is its right element, trial left, R
+    // derivative, test -left; right face (row | row + 1): trial right, L
+    // derivative, test +right
+    const int p = prev_cell(row, L.cells);
+    for (int side = 0; side < 2; ++side) {
+        const double *test = side == 0 ? L.left : L.right, *trial = side == 0 ? L.left : L.right;
+        const double sign = side == 0 ? -1.0 : 1.0;
+        const double *d = side == 0 ? fr + 9 * p : fl + 9 * row;
+        for (int w = 0; w < 3; ++w)
+            for (int j = 0; j < N; ++j)
+                for (int v = 0; v < 3; ++v)
+                    for (int i = 0; i < N; ++i) at(v * N + i, w * N + j) += (sign * trial[j] * d[v + 3 * w]) * test[i];
+    }
+    if (!factorize) return;
+    if (!lu_in_place(A, rowsrc + static_cast<long long>(row) * m, m)) atomicOr(flag, 8);
+}
+
+// BlockJacobi::apply of one block (block_jacobi.hpp:24-39): permute, unit
+// lower solve, upper solve
+template <int N>
+__device__ __forceinline__ void bj_cell(int e, const double *r, double *z, const double *factors, const int *rowsrc) {
+    constexpr int m = 3 * N;
+    lu_solve_cell<m>(factors + static_cast<long long>(e) * m * m, rowsrc + static_cast<long long>(e) * m,
+                     r + static_cast<long long>(e) * m, z + static_cast<long long>(e) * m);
+}
+template <int N>
+__global__ __launch_bounds__(CELL_BLOCK) void k1_bj_apply(const double *r, double *z, const double *factors,
+                                                          const int *rowsrc, int cells) {
+    const int e = blockIdx.x * CELL_BLOCK + threadIdx.x;
+    if (e < cells) bj_cell<N>(e, r, z, factors, rowsrc);
+}
+
+// ------------------------------------------------------ condensed HDG
+// hybrid.cpp:42-81 (1D) for cell e: A = volume block + M / tau + the HDG face
+// terms shape_i shape_j d_s(v, w) with d_s = sign_s F'(U_s) + stab_s I (side 0:
+// sign -1, stab b of face e-1; side 1: sign +1, stab a of face e); B(v n + i,
+// 3 s + v) = -shape_i stab_s; C(3 s + w, v n + i) = shape_i d_s(w, v). The
+// partial-pivot LU of A (kept in factors) gives X = A^-1 B (m x 6), y = A^-1 r;
+// S = -C X (6 x 6) and C y (the cell's part of the Schur right-hand side).
+template <int N>
+__device__ __forceinline__ void hdg_side(const double *u, int e, int side, const double *speeds, const Line &L,
+                                         double d[9], double &stab, const double *&shape) {
+    const int face = side == 0 ? prev_cell(e, L.cells) : e;
+    shape = side == 0 ? L.left : L.right;
+    stab = side == 0 ? speeds[2 * face + 1] : speeds[2 * face];
+    const double sign = side == 0 ? -1.0 : 1.0;
+    double z[3], J[9];
+    sample1<N>(u, e, shape, z);
+    jacobian1(z, L.gamma, J);
+    for (int k = 0; k < 9; ++k) d[k] = sign * J[k] + stab * (k % 4 == 0 ? 1.0 : 0.0);
 }
 template <int N>
 __global__ __launch_bounds__(CELL_BLOCK) void k1_condense(const double *u, const double *r, const double *speeds,
