@@ -1005,9 +1005,28 @@ void hip_check1(hipError_t e, const char *what) {
 }
 #define HIPCHECK1(call) hip_check1((call), #call)
 
+// device buffer that frees itself (also when a constructor throws half way)
 template <class T> struct Buf {
     T *p = nullptr;
     size_t n = 0;
+    Buf() = default;
+    Buf(const Buf &) = delete;
+    Buf &operator=(const Buf &) = delete;
+    Buf(Buf &&o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    Buf &operator=(Buf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~Buf() { release(); }
     void alloc(size_t count) {
         if (p && n >= count) return;
         release();
@@ -1072,7 +1091,6 @@ struct dgh1_ctx {
         m = 3 * n;
         D = static_cast<long long>(cells) * m;
         h = 2.0 / cells;
-        HIPCHECK1(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         const Basis b = make_basis(n, 1.0, 1.0, Q > n ? Q : 0);
         hline.n = n;
         hline.Q = Q;
@@ -1126,6 +1144,7 @@ struct dgh1_ctx {
         for (auto *b2 : {&trhs, &tdir, &terr, &ones}) b2->alloc(3 * cells);
         const std::vector<double> one(3 * cells, 1.0);
         HIPCHECK1(hipMemcpy(ones.p, one.data(), 3 * cells * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHECK1(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));  // last: nothing throws after it
         flag.alloc(1);
         rate_bits.alloc(1);
     }
