@@ -457,6 +457,9 @@ class DeviceContext1D:
 
     def _call(self, name, *args):
         L = _lib1()
+        # torch's fills and copies of the arguments run on torch's stream; the
+        # library's own stream does not wait for them
+        self._torch.cuda.synchronize()
         if getattr(L, name)(self.h, *args) != 0:
             raise DeviceError(L.dgh1_last_error(self.h).decode())
 
