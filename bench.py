@@ -180,8 +180,8 @@ def run_device(args, nx, k, init, precision):
                 jv_avg_s=jv_ms / 1e3 / max(1, jv_n), geom=geom)
 
 
-FACTOR_BYTES = {"fp64": 8, "mixed": 4, "mixed-bf16": 2}
-VECTOR_BYTES = {"fp64": 8, "mixed": 4, "mixed-bf16": 4}
+FACTOR_BYTES = {"fp64": 8, "mixed": 4, "mixed-bf16": 2, "mixed-fp16": 2}
+VECTOR_BYTES = {"fp64": 8, "mixed": 4, "mixed-bf16": 4, "mixed-fp16": 4}
 
 
 def bj_bytes(g, precision="fp64"):
@@ -282,6 +282,18 @@ def probe(args):
             dev.ilu_apply(x, y)
         torch.cuda.synchronize()
         log("[probe] ilu_apply fp64 x3")
+    with DeviceContext(solver_args(args.nx, args.k, args.init, "mixed-fp16", 1, 0)) as dev:
+        u = dev.initial()
+        rate = dev.freeze(u)
+        dev.set_step(5.0 / rate, u)
+        dev.dg_refresh(u)
+        dev.bj_refresh_mixed()
+        x32 = torch.randn(dev.size, dtype=torch.float32, device="cuda")
+        y32 = torch.zeros_like(x32)
+        for _ in range(3):
+            dev.bj_apply_f32(x32, y32)
+        torch.cuda.synchronize()
+        log("[probe] bj_apply fp16 factors x3")
     print(json.dumps({"probe": "done", "nx": args.nx, "k": args.k}), flush=True)
 
 
@@ -378,7 +390,7 @@ def main():
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--init", default="diagonal", choices=["seeded", "diagonal"])
-    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed", "mixed-bf16"])
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed", "mixed-bf16", "mixed-fp16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-hdg", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the untimed mixed-vs-FP64 update comparison")
@@ -418,9 +430,11 @@ def main():
                                    f"Schur GMRES(20) + face block Jacobi, ntol=ltol=1e-12"},
             "hdg_fp64": hdg}), flush=True)
         return
-    forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    # with both: FP64, mixed precision with FP32-stored factors, and the
+    # headline form, mixed precision with FP16-stored factors
+    forms = ["fp64", "mixed", "mixed-fp16"] if args.precision == "both" else [args.precision]
     runs = {p: run_device(args, args.nx, args.k, args.init, p) for p in forms}
-    head = forms[-1]  # the headline form: mixed precision when both are run
+    head = forms[-1]
     dev = runs[head]
     g = dev["geom"]
 
@@ -447,11 +461,14 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     equal = None
-    if len(forms) == 2:
-        a, b = forms_out["fp64"]["rms"], forms_out["mixed"]["rms"]
-        equal = {"rms_fp64": a, "rms_mixed": b,
-                 "max_rel_diff_first_two_updates": max(abs(x - y) / abs(x) for x, y in zip(a[:2], b[:2])),
-                 "speedup_mixed_over_fp64": forms_out["mixed"]["updates_per_s"] / forms_out["fp64"]["updates_per_s"]}
+    if "fp64" in forms and len(forms) > 1:
+        a = forms_out["fp64"]["rms"]
+        equal = {"rms_fp64": a}
+        for p in forms[1:]:
+            b = forms_out[p]["rms"]
+            equal[p] = {"rms": b,
+                        "max_rel_diff_first_two_updates": max(abs(x - y) / abs(x) for x, y in zip(a[:2], b[:2])),
+                        "speedup_over_fp64": forms_out[p]["updates_per_s"] / forms_out["fp64"]["updates_per_s"]}
     hdg = None if args.no_hdg or args.precision != "both" else run_hdg(args, args.nx, args.k, args.init)
     oned = None if args.no_1d or args.precision != "both" else run_1d()
     ilu = None if args.no_ilu or args.precision != "both" else run_ilu(args, args.nx, args.k, args.init)
@@ -472,11 +489,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == "fp64" else ("f64+f32" if head == "mixed" else "f64+f32+bf16"),
+        "dtype": "f64" if head == "fp64" else "f64+f32",
         "data": f"synthetic ({args.init} state{', seed 2025' if args.init == 'seeded' else ''})",
         "config": {"workload": f"2D DG BE Newton update, k={args.k}, {args.nx}x{args.nx} periodic quads, "
                                f"CFL 5, GMRES(20) + block-Jacobi LU, ntol=ltol=1e-12",
                    "nx": args.nx, "ny": args.nx, "k": args.k, "init": args.init, "precision": head,
+                   "factor_storage": {"fp64": "f64", "mixed": "f32", "mixed-bf16": "bf16",
+                                      "mixed-fp16": "f16, U scaled by 2^-e per block"}[head],
                    "krylov_per_update": krylov},
         "roofline": {"kernel": "bj_apply", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -88,7 +88,8 @@ struct dgh_ctx {
     DevBuf<double> kbasis, kw, kz, kwork, kres, ky;
     // mixed precision: FP32 / BF16 factors, FP32 inner Krylov space, FGMRES Z
     DevBuf<float> factors_f32, ibasis, iw, it, is, ires, ib, zbasis;
-    DevBuf<unsigned short> factors_bf16;
+    DevBuf<unsigned short> factors_bf16;  // 2-byte factors (BF16 or FP16) in the pair layout
+    DevBuf<float> factor_scale;           // FP16: 2^-e per block (U' = sigma U)
     DevBuf<int> rowsrc32;
     DevBuf<double> check_dir;
     bool bj32_ready = false;
@@ -184,6 +185,7 @@ struct dgh_ctx {
         DevBuf<float> *fbufs[] = {&factors_f32, &ibasis, &iw, &it, &is, &ires, &ib, &zbasis};
         for (auto *b : fbufs) b->release(bytes);
         factors_bf16.release(bytes);
+        factor_scale.release(bytes);
         rowsrc32.release(bytes);
         face_rowsrc.release(bytes);
         factors32.release(bytes);
@@ -783,6 +785,9 @@ struct dgh_ctx {
 
     // ---- mixed precision: FP32 operator, FP32/BF16 block Jacobi, FP32 inner
     //      GMRES inside FP64 flexible GMRES (no reference counterpart)
+    bool two_byte_factors() const {
+        return cfg.precision == DGH_PRECISION_MIXED_BF16 || cfg.precision == DGH_PRECISION_MIXED_FP16;
+    }
     void bj_refresh_mixed() {
         require(refreshed, "Refresh the volume Jacobian before assembling a block");
         factors_f32.alloc(static_cast<size_t>(cells) * m * m, bytes);
@@ -796,17 +801,22 @@ struct dgh_ctx {
         require(read_flag(2) == 0, "Singular block Jacobi factor");
         pack_blocks(m, factors_f32.p, cells);
         ilu32_ready = false;
-        if (cfg.precision == DGH_PRECISION_MIXED_BF16) {
-            factors_bf16.alloc(static_cast<size_t>(cells) * m * m, bytes);
-            HIPCHECK(launch_to_bf16(factors_f32.p, factors_bf16.p, static_cast<long long>(cells) * m * m, stream));
+        if (two_byte_factors()) {
+            factors_bf16.alloc(static_cast<size_t>(cells) * pair_factor_stride(m), bytes);
+            factor_scale.alloc(cells, bytes);
+            timed("pack", [&] {
+                HIPCHECK(launch_to_pairs(m, factors_f32.p, factors_bf16.p, factor_scale.p, cells,
+                                         cfg.precision == DGH_PRECISION_MIXED_FP16, stream));
+            });
         }
         bj32_ready = true;
     }
     void bj_apply32(const float *r, float *z) {
         require(bj32_ready, "Refresh the block Jacobi preconditioner first");
         timed("bj_apply", [&] {
-            if (cfg.precision == DGH_PRECISION_MIXED_BF16)
-                HIPCHECK(launch_bj_apply_bf16(m, factors_bf16.p, rowsrc32.p, r, z, cells, stream));
+            if (two_byte_factors())
+                HIPCHECK(launch_bj_apply_pairs(m, factors_bf16.p, factor_scale.p, rowsrc32.p, r, z, cells,
+                                               cfg.precision == DGH_PRECISION_MIXED_FP16, stream));
             else
                 HIPCHECK(launch_bj_apply(m, factors_f32.p, rowsrc32.p, r, z, cells, stream));
         });
@@ -1147,6 +1157,7 @@ struct dgh_ctx {
                         sync();
                         factors_f32.release(bytes);
                         factors_bf16.release(bytes);
+        factor_scale.release(bytes);
                         ilu_Y32.release(bytes);
                         bj32_ready = false;
                         ilu32_ready = false;

@@ -60,7 +60,7 @@ void validate_config(const dgh_config &c) {
                 c.linear_tolerance < 1,
             "Invalid solver tolerance or wave-speed safety");
     require(c.method >= DGH_METHOD_DG && c.method <= DGH_METHOD_BOTH, "Method must be dg, hdg, or both");
-    require(c.precision >= DGH_PRECISION_FP64 && c.precision <= DGH_PRECISION_MIXED_BF16, "Invalid precision");
+    require(c.precision >= DGH_PRECISION_FP64 && c.precision <= DGH_PRECISION_MIXED_FP16, "Invalid precision");
     require(c.inner_restart > 0 && c.inner_restart <= 31 && c.inner_max > 0, "Invalid iteration limit");
     require(std::isfinite(c.inner_tolerance) && c.inner_tolerance > 0 && c.inner_tolerance < 1,
             "Invalid solver tolerance or wave-speed safety");
@@ -73,7 +73,7 @@ void validate_config(const dgh_config &c) {
         require(c.nx % 2 == 0 && c.ny % 2 == 0 && c.nx >= 4 && c.ny >= 4,
                 "Block ILU(0) on the device runs in red-black order: nx and ny must be even and >= 4");
         require(c.method == DGH_METHOD_DG, "Block ILU(0) on the device is built for --method dg");
-        require(c.precision != DGH_PRECISION_MIXED_BF16,
+        require(c.precision != DGH_PRECISION_MIXED_BF16 && c.precision != DGH_PRECISION_MIXED_FP16,
                 "Block ILU(0) on the device is built for --precision fp64 and mixed");
         require(!c.direct, "Block ILU(0) preconditions GMRES; --linear direct takes no preconditioner");
     } else {
@@ -163,6 +163,7 @@ dgh_config parse_config(int argc, const char *const *argv) {
             if (name == "fp64") c.precision = DGH_PRECISION_FP64;
             else if (name == "mixed") c.precision = DGH_PRECISION_MIXED;
             else if (name == "mixed-bf16") c.precision = DGH_PRECISION_MIXED_BF16;
+            else if (name == "mixed-fp16") c.precision = DGH_PRECISION_MIXED_FP16;
             else throw std::runtime_error("Invalid precision");
         } else if (option == "--inner-restart") c.inner_restart = integer();
         else if (option == "--inner-max") c.inner_max = integer();
@@ -182,7 +183,7 @@ dgh_config parse_config(int argc, const char *const *argv) {
                          "--ntol R --ltol R --safety S --prefix PATH\n"
                          "--integrator ssp-dirk3|be --newton-steps N --warmup-updates N\n"
                          "--timing-only --check-update --no-save-solution\n"
-                         "--device gpu --precision fp64|mixed|mixed-bf16 --inner-restart N --inner-max N\n"
+                         "--device gpu --precision fp64|mixed|mixed-bf16|mixed-fp16 --inner-restart N --inner-max N\n"
                          "--inner-tol R --seed S --graph\n";
             throw std::runtime_error("help");
         } else

@@ -144,12 +144,15 @@ hipError_t launch_bj_apply_colour(int m, const double *factors, const int *rowsr
                                   int nblocks, int colour, int nx, hipStream_t s);
 hipError_t launch_bj_apply_colour(int m, const float *factors, const int *rowsrc, const float *r, float *z,
                                   int nblocks, int colour, int nx, hipStream_t s);
-// factors: bfloat16 storage (FP32 arithmetic)
-hipError_t launch_bj_apply_bf16(int m, const void *factors, const int *rowsrc, const float *r, float *z,
-                                int nblocks, hipStream_t s);
+// FP64 <-> FP32 vector conversion
 hipError_t launch_convert(const double *src, float *dst, long long count, hipStream_t s);
 hipError_t launch_convert(const float *src, double *dst, long long count, hipStream_t s);
-hipError_t launch_to_bf16(const float *src, void *dst, long long count, hipStream_t s);
+// 2-byte factors in the pair layout (BF16, or FP16 with a per-block scale of U)
+long long pair_factor_stride(int m);
+hipError_t launch_to_pairs(int m, const float *packed, void *out, float *sigma, int nblocks, int fp16,
+                           hipStream_t s);
+hipError_t launch_bj_apply_pairs(int m, const void *factors, const float *sigma, const int *rowsrc, const float *r,
+                                 float *z, int nblocks, int fp16, hipStream_t s);
 
 // GMRES building blocks over vectors of length `count` with the DG metric
 // inv_mass(node) (weights == nullptr) or a per-entry weight vector; dot

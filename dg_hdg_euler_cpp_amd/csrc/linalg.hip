@@ -3,6 +3,7 @@
 #include "dgh_internal.hpp"
 
 #include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 
 #include <cfloat>
 #include <cstdint>
@@ -531,11 +532,7 @@ __global__ __launch_bounds__(LT) void k_lu(T *__restrict__ blocks, int *__restri
 #ifndef DGH_BJ_RING32
 #define DGH_BJ_RING32 4
 #endif
-#ifndef DGH_BJ_RING16
-#define DGH_BJ_RING16 4
-#endif
-constexpr int BJ_RING64 = DGH_BJ_RING64, BJ_RING32 = DGH_BJ_RING32, BJ_RING16 = DGH_BJ_RING16;
-__device__ __forceinline__ float to_compute(__hip_bfloat16 v) { return __bfloat162float(v); }
+constexpr int BJ_RING64 = DGH_BJ_RING64, BJ_RING32 = DGH_BJ_RING32;
 __device__ __forceinline__ float to_compute(float v) { return v; }
 __device__ __forceinline__ double to_compute(double v) { return v; }
 
@@ -548,11 +545,6 @@ __device__ __forceinline__ double to_compute(double v) { return v; }
 template <typename T> __device__ __forceinline__ T factor_load(const T *p) {
     if constexpr (!DGH_BJ_NONTEMPORAL) {
         return *p;
-    } else if constexpr (std::is_same<T, __hip_bfloat16>::value) {
-        const unsigned short bits = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
-        T out;
-        __builtin_memcpy(&out, &bits, sizeof(out));
-        return out;
     } else {
         return __builtin_nontemporal_load(p);
     }
@@ -582,7 +574,8 @@ __global__ __launch_bounds__(BLOCK) void k_pack(const T *__restrict__ square, T 
     }
 }
 
-// TF: stored factor type (double, float or bfloat16); TV: vector and
+// TF: stored factor type (double or float; 2-byte factors use
+// k_bj_apply_pairs); TV: vector and
 // arithmetic type (double for the FP64 path, float for the inner FP32 solve).
 template <int M, typename TF, typename TV, int U>
 __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ factors, const int *__restrict__ rowsrc,
@@ -690,10 +683,210 @@ __global__ void k_convert(const TS *__restrict__ src, TD *__restrict__ dst, long
          i += static_cast<long long>(gridDim.x) * BLOCK)
         dst[i] = static_cast<TD>(src[i]);
 }
-__global__ void k_float_to_bf16(const float *__restrict__ src, __hip_bfloat16 *__restrict__ dst, long long count) {
-    for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
-         i += static_cast<long long>(gridDim.x) * BLOCK)
-        dst[i] = __float2bfloat16(src[i]);
+
+// ------------------------------------------------ two-byte factor layout
+// Factors stored in 2 bytes (BF16, or FP16 with a power-of-two scale of U per
+// block) are read as aligned 4-byte pairs of consecutive rows, so that a wave
+// load moves 256 bytes instead of 128. Each column segment of the packed
+// layout starts at an even row and has even length: the lower part of column c
+// holds rows s(c) = 2 floor((c+1)/2) .. M-1 (row c itself as a zero when c is
+// even), the upper part rows 0 .. c rounded up to even (a zero pad row). M is
+// even (4 n^2). Block stride: pair_stride(M) entries.
+__host__ __device__ __forceinline__ long long pair_lower(int M, int c) {
+    return static_cast<long long>(c) * M - 2LL * ((static_cast<long long>(c) * c) / 4);
+}
+__host__ __device__ __forceinline__ long long pair_cumulative_upper(long long n) {  // columns 0..n
+    return 2 * (n + 1) + 2 * ((n * n) / 4);
+}
+__host__ __device__ __forceinline__ long long pair_upper(int M, int c) {
+    return pair_lower(M, M) + pair_cumulative_upper(M - 1) - pair_cumulative_upper(c);
+}
+__host__ __device__ __forceinline__ long long pair_stride(int M) {
+    return ((pair_lower(M, M) + pair_cumulative_upper(M - 1)) + 63) / 64 * 64;
+}
+__device__ __forceinline__ unsigned short half_bits(float v, bool fp16) {
+    if (fp16) return __half_as_ushort(__float2half_rn(v));
+    const __hip_bfloat16 b = __float2bfloat16(v);
+    unsigned short u;
+    __builtin_memcpy(&u, &b, 2);
+    return u;
+}
+__device__ __forceinline__ float half_value(unsigned short u, bool fp16) {
+    if (fp16) return __half2float(__ushort_as_half(u));
+    __hip_bfloat16 b;
+    __builtin_memcpy(&b, &u, 2);
+    return __bfloat162float(b);
+}
+// One block per factor: the packed FP32 LU (k_pack layout) into the pair
+// layout in 2 bytes. FP16 scales U by sigma = 2^-e, e = floor(log2 max|U|), so
+// that the largest entry lies in [1, 2) (L is unscaled: |L_ij| <= 1 after
+// partial pivoting); BF16 keeps sigma = 1.
+__global__ __launch_bounds__(BLOCK) void k_to_pairs(const float *__restrict__ packed, unsigned short *__restrict__ out,
+                                                    float *__restrict__ sigma, int M, int fp16) {
+    __shared__ float red[BLOCK / 64];
+    __shared__ float scale;
+    const long long mm = static_cast<long long>(M) * M, stride = pair_stride(M);
+    const float *src = packed + blockIdx.x * mm;
+    unsigned short *dst = out + blockIdx.x * stride;
+    const long long nl = static_cast<long long>(M) * (M - 1) / 2;  // lower entries, then upper
+    float mx = 0;
+    if (fp16)
+        for (long long t = nl + threadIdx.x; t < mm; t += BLOCK) mx = fmaxf(mx, fabsf(src[t]));
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0;
+        for (int w = 0; w < BLOCK / 64; ++w) m = fmaxf(m, red[w]);
+        scale = (fp16 && m > 0) ? ldexpf(1.0f, -ilogbf(m)) : 1.0f;
+        sigma[blockIdx.x] = scale;
+    }
+    for (long long t = threadIdx.x; t < stride; t += BLOCK) dst[t] = 0;
+    __syncthreads();
+    const float sc = scale;
+    for (long long t = threadIdx.x; t < mm; t += BLOCK) {
+        // invert k_pack's layout: lower parts of columns 0..M-2, then upper parts of M-1..0
+        int c, row;
+        float v;
+        if (t < nl) {
+            // column c with packed_lower(M, c) <= t < packed_lower(M, c + 1)
+            int lo = 0, hi = M - 2;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) / 2;
+                if (packed_lower(M, mid) <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            c = lo;
+            row = c + 1 + static_cast<int>(t - packed_lower(M, c));
+            v = src[t];
+            dst[pair_lower(M, c) + (row - 2 * ((c + 1) / 2))] = half_bits(v, fp16);
+        } else {
+            // column c with packed_upper(M, c) <= t < packed_upper(M, c) + c + 1 (descending c)
+            int lo = 0, hi = M - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) / 2;
+                if (packed_upper(M, mid) <= t) hi = mid;
+                else lo = mid + 1;
+            }
+            c = lo;
+            row = static_cast<int>(t - packed_upper(M, c));
+            v = src[t] * sc;
+            dst[pair_upper(M, c) + row] = half_bits(v, fp16);
+        }
+    }
+}
+
+// Block Jacobi apply on the pair layout: lane l owns rows 128 a + 2 l and
+// 128 a + 2 l + 1 of chunk a and reads both with one 4-byte load per column;
+// otherwise as k_bj_apply (ring of U columns in flight, pivot values by
+// readlane). With FP16 factors x is multiplied by sigma between the two
+// sweeps, since U' = sigma U gives U^-1 b = U'^-1 (sigma b).
+template <int M, bool FP16, int U>
+__global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *__restrict__ factors,
+                                                          const float *__restrict__ sigma,
+                                                          const int *__restrict__ rowsrc, const float *__restrict__ r,
+                                                          float *__restrict__ z, int nblocks) {
+    static_assert(M % 2 == 0 && 128 % U == 0 && U % 2 == 0, "pair layout");
+    constexpr int NA = (M + 127) / 128;
+    const int lane = threadIdx.x % 64;
+    const long long blk = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + threadIdx.x / 64;
+    if (blk >= nblocks) return;
+    const unsigned *F = reinterpret_cast<const unsigned *>(factors + blk * pair_stride(M));
+    float x[NA][2];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int row = 128 * a + 2 * lane + e;
+            x[a][e] = row < M ? r[blk * M + rowsrc[blk * M + row]] : 0.f;
+        }
+    unsigned buf[U][NA];
+    auto lo16 = [](unsigned w) { return static_cast<unsigned short>(w & 0xffffu); };
+    auto hi16 = [](unsigned w) { return static_cast<unsigned short>(w >> 16); };
+    // lower part of column c from the pair holding row s(c), chunks a >= a0
+    auto load_lower = [&](unsigned(&b)[NA], int c, int a0) {
+        const int s0 = 2 * ((c + 1) / 2);
+        const unsigned *colp = F + (pair_lower(M, c) - s0) / 2;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            const int r0 = 128 * a + 2 * lane;
+            if (a >= a0) b[a] = (r0 >= s0 && r0 < M) ? __builtin_nontemporal_load(colp + r0 / 2) : 0u;
+        }
+    };
+    // upper part of column c (rows 0..c and the pad), chunks a <= a1
+    auto load_upper = [&](unsigned(&b)[NA], int c, int a1) {
+        const unsigned *colp = F + pair_upper(M, c) / 2;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            const int r0 = 128 * a + 2 * lane;
+            if (a <= a1) b[a] = r0 <= c ? __builtin_nontemporal_load(colp + r0 / 2) : 0u;
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < U; ++u) load_lower(buf[u], u, 0);
+#pragma unroll
+    for (int ja = 0; ja < NA; ++ja) {
+        const int jend = min(128, M - ja * 128);
+#pragma clang loop unroll(disable)
+        for (int jl = 0; jl < jend; jl += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (jl + u < jend) {
+                    const int j = ja * 128 + jl + u;
+                    const float xj = readlane_d(x[ja][u & 1], (jl + u) >> 1);
+#pragma unroll
+                    for (int a = ja; a < NA; ++a) {
+                        x[a][0] -= half_value(lo16(buf[u][a]), FP16) * xj;
+                        x[a][1] -= half_value(hi16(buf[u][a]), FP16) * xj;
+                    }
+                    if (j + U < M) load_lower(buf[u], j + U, ja);
+                }
+            }
+        }
+    }
+    if (FP16) {
+        const float sc = sigma[blk];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) x[a][0] *= sc, x[a][1] *= sc;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) load_upper(buf[u], M - 1 - u, NA - 1);
+#pragma unroll
+    for (int ja = NA - 1; ja >= 0; --ja) {
+        const int jend = min(128, M - ja * 128);
+        const int t0 = M - ja * 128 - jend;
+#pragma clang loop unroll(disable)
+        for (int g = 0; g < jend; g += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int jl = jend - 1 - g - u;
+                if (jl >= 0) {
+                    const int j = ja * 128 + jl;
+                    unsigned(&cur)[NA] = buf[(t0 + u) % U];
+                    // jend is even, so the parity of jl is that of u + 1
+                    const unsigned dw = cur[ja];
+                    const float dv = half_value((u & 1) ? lo16(dw) : hi16(dw), FP16);
+                    const float diag = readlane_d(dv, jl >> 1);
+                    const float xj = readlane_d(x[ja][(u & 1) ? 0 : 1], jl >> 1) / diag;
+                    if (lane == (jl >> 1)) x[ja][(u & 1) ? 0 : 1] = xj;
+#pragma unroll
+                    for (int a = 0; a <= ja; ++a) {
+                        const int r0 = 128 * a + 2 * lane;
+                        if (r0 < j) x[a][0] -= half_value(lo16(cur[a]), FP16) * xj;
+                        if (r0 + 1 < j) x[a][1] -= half_value(hi16(cur[a]), FP16) * xj;
+                    }
+                    if (j - U >= 0) load_upper(cur, j - U, ja);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int row = 128 * a + 2 * lane + e;
+            if (row < M) z[blk * M + row] = x[a][e];
+        }
 }
 
 // ----------------------------------------------------------------- GMRES
@@ -1150,11 +1343,28 @@ hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const
         k_bj_apply<M, float, float, BJ_RING32><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks, -1, 0);
     })
 }
-hipError_t launch_bj_apply_bf16(int m, const void *factors, const int *rowsrc, const float *r, float *z,
-                                int nblocks, hipStream_t s) {
-    const auto *f = static_cast<const __hip_bfloat16 *>(factors);
+long long pair_factor_stride(int m) { return pair_stride(m); }
+hipError_t launch_to_pairs(int m, const float *packed, void *out, float *sigma, int nblocks, int fp16,
+                           hipStream_t s) {
+    k_to_pairs<<<nblocks, BLOCK, 0, s>>>(packed, static_cast<unsigned short *>(out), sigma, m, fp16);
+    return hipGetLastError();
+}
+// columns in flight per wave (measured at 256^2, k=8: FP16 2.67 / 2.77 / 3.18 ms
+// and BF16 2.68 / 2.53 / 3.30 ms with 4 / 8 / 2 columns)
+#ifndef DGH_BJ_RING_FP16
+#define DGH_BJ_RING_FP16 4
+#endif
+#ifndef DGH_BJ_RING_BF16
+#define DGH_BJ_RING_BF16 8
+#endif
+hipError_t launch_bj_apply_pairs(int m, const void *factors, const float *sigma, const int *rowsrc, const float *r,
+                                 float *z, int nblocks, int fp16, hipStream_t s) {
+    const auto *f = static_cast<const unsigned short *>(factors);
     DGH_DISPATCH_M(m, {
-        k_bj_apply<M, __hip_bfloat16, float, BJ_RING16><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, rowsrc, r, z, nblocks, -1, 0);
+        if (fp16)
+            k_bj_apply_pairs<M, true, DGH_BJ_RING_FP16><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, sigma, rowsrc, r, z, nblocks);
+        else
+            k_bj_apply_pairs<M, false, DGH_BJ_RING_BF16><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, sigma, rowsrc, r, z, nblocks);
     })
 }
 // In place, `chunk` blocks at a time through `scratch` (chunk * m*m entries):
@@ -1184,10 +1394,6 @@ hipError_t launch_convert(const double *src, float *dst, long long count, hipStr
 }
 hipError_t launch_convert(const float *src, double *dst, long long count, hipStream_t s) {
     k_convert<float, double><<<4096, BLOCK, 0, s>>>(src, dst, count);
-    return hipGetLastError();
-}
-hipError_t launch_to_bf16(const float *src, void *dst, long long count, hipStream_t s) {
-    k_float_to_bf16<<<4096, BLOCK, 0, s>>>(src, static_cast<__hip_bfloat16 *>(dst), count);
     return hipGetLastError();
 }
 

@@ -46,7 +46,8 @@ typedef struct dgh_ctx dgh_ctx;
 enum dgh_precision {
     DGH_PRECISION_FP64 = 0,       /* reference arithmetic */
     DGH_PRECISION_MIXED = 1,      /* FP32 block LU + FP32 inner GMRES in FP64 FGMRES */
-    DGH_PRECISION_MIXED_BF16 = 2  /* as MIXED with BF16-stored LU factors */
+    DGH_PRECISION_MIXED_BF16 = 2, /* as MIXED with BF16-stored LU factors */
+    DGH_PRECISION_MIXED_FP16 = 3  /* as MIXED with FP16-stored LU factors (U scaled by 2^-e per block) */
 };
 enum dgh_testcase { DGH_DIAGONAL = 0, DGH_WAVE = 1, DGH_AXIS = 2, DGH_SEEDED = 3 };
 enum dgh_method { DGH_METHOD_DG = 0, DGH_METHOD_HDG = 1, DGH_METHOD_BOTH = 2 };
@@ -104,7 +105,7 @@ typedef struct dgh_adjoint_stats {
 size_t dgh_config_size(void);
 int dgh_config_default(dgh_config *out);
 /* Parses reference CLI flags (argv[0] is skipped) plus --device gpu,
- * --precision fp64|mixed|mixed-bf16, --inner-restart N, --inner-max N,
+ * --precision fp64|mixed|mixed-bf16|mixed-fp16, --inner-restart N, --inner-max N,
  * --inner-tol R and --seed S. On error
  * writes the reference message into err. */
 int dgh_config_parse(int argc, const char *const *argv, dgh_config *out, char *err, size_t errlen);
@@ -181,8 +182,8 @@ int dgh_ilu_refresh(dgh_ctx *ctx);
 int dgh_ilu_apply(dgh_ctx *ctx, const double *r, double *z);
 
 /* ---- mixed precision (no reference counterpart) ---------------------------- */
-/* Block Jacobi factors in FP32 (and BF16 storage for DGH_PRECISION_MIXED_BF16)
- * from the same element blocks as dgh_bj_refresh. */
+/* Block Jacobi factors in FP32 (and 2-byte BF16 / FP16 storage for
+ * DGH_PRECISION_MIXED_BF16 / _FP16) from the same element blocks as dgh_bj_refresh. */
 int dgh_bj_refresh_mixed(dgh_ctx *ctx);
 /* FP32 vectors: z = P32^-1 r and y = J x (Jacobian snapshot of dgh_dg_refresh). */
 int dgh_bj_apply_f32(dgh_ctx *ctx, const float *r, float *z);

@@ -92,6 +92,9 @@ def test_config_struct_layout_and_defaults():
                                     "--precision", "mixed", "--device", "gpu", "--restart", "20"]))
     assert (c.nx, c.degree, c.integrator, c.steps, c.precision, c.restart) == (256, 8, 0, 1, 1, 20)
     assert c.graph == 0 and Cfg.from_buffer_copy(parse(["--graph"])).graph == 1
+    # 2-byte factor storage: BF16, or FP16 with a per-block scale of U
+    assert Cfg.from_buffer_copy(parse(["--precision", "mixed-bf16"])).precision == 2
+    assert Cfg.from_buffer_copy(parse(["--precision", "mixed-fp16"])).precision == 3
     # device block ILU(0): DG, FP64, red-black order on an even grid
     c = Cfg.from_buffer_copy(parse(["--precond", "block-ilu0", "--ordering", "redblack", "--method", "dg"]))
     assert (c.precond, c.ordering, c.colamd) == (1, 2, 0)
@@ -110,7 +113,10 @@ def test_config_struct_layout_and_defaults():
     (["--precond", "block-ilu0", "--ordering", "redblack"], "Block ILU(0) on the device is built for --method dg"),
     (["--precond", "block-ilu0", "--ordering", "redblack", "--method", "dg", "--precision", "mixed-bf16"],
      "Block ILU(0) on the device is built for --precision fp64 and mixed"),
+    (["--precond", "block-ilu0", "--ordering", "redblack", "--method", "dg", "--precision", "mixed-fp16"],
+     "Block ILU(0) on the device is built for --precision fp64 and mixed"),
     (["--ordering", "redblack"], "Red-black ordering applies to --precond block-ilu0"),
+    (["--precision", "fp8"], "Invalid precision"),
     (["--linear", "lu"], "Invalid linear solver"),
     (["--integrator", "rk4"], "Invalid integrator"),
     (["--method", "fem"], "Method must be dg, hdg, or both"),

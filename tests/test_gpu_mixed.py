@@ -30,7 +30,7 @@ def mass_vector(orc):
     return np.tile(node, 4 * orc.cells)
 
 
-@pytest.mark.parametrize("precision,bound", [("mixed", 1e-5), ("mixed-bf16", 6e-2)])
+@pytest.mark.parametrize("precision,bound", [("mixed", 1e-5), ("mixed-bf16", 6e-2), ("mixed-fp16", 1e-2)])
 @pytest.mark.parametrize("k,nx", [(2, 8), (8, 12)])
 def test_fp32_pieces_and_fgmres(precision, bound, k, nx, torch_cuda):
     from dg_hdg_euler_cpp_amd import DeviceContext
@@ -77,7 +77,7 @@ def test_fp32_pieces_and_fgmres(precision, bound, k, nx, torch_cuda):
     dev.close()
 
 
-@pytest.mark.parametrize("precision", ["mixed", "mixed-bf16"])
+@pytest.mark.parametrize("precision", ["mixed", "mixed-bf16", "mixed-fp16"])
 def test_mixed_newton_runs_match_reference(precision, torch_cuda):
     from dg_hdg_euler_cpp_amd import DeviceContext
 
@@ -100,10 +100,12 @@ def test_mixed_newton_runs_match_reference(precision, torch_cuda):
         assert abs(err[v] - float(summ[k])) <= 1e-9 * float(summ[k])
 
 
-def test_mixed_update_error_vs_fp64_256(torch_cuda):
-    """256^2, k=8: two BE Newton updates with FP32 factors + FP32 inner GMRES;
-    each update's direction is compared with the FP64 path's solution of the
-    same system (precision_error) and the RMS sequence with the reference's."""
+@pytest.mark.parametrize("precision", ["mixed", "mixed-fp16"])
+def test_mixed_update_error_vs_fp64_256(precision, torch_cuda):
+    """256^2, k=8: two BE Newton updates with FP32 factors (stored in FP32, or
+    in FP16 with a per-block scale of U) + FP32 inner GMRES; each update's
+    direction is compared with the FP64 path's solution of the same system
+    (precision_error) and the RMS sequence with the reference's."""
     from dg_hdg_euler_cpp_amd import DeviceContext
 
     run = GOLD["runs"]["k8_256_be"]
@@ -111,11 +113,13 @@ def test_mixed_update_error_vs_fp64_256(torch_cuda):
     args[args.index("--newton-steps") + 1] = "2"
     args[args.index("--warmup-updates") + 1] = "0"
     args = [a for a in args if a not in ("--method", "dg")]
-    with DeviceContext(args + ["--method", "dg", "--precision", "mixed", "--check-update"]) as dev:
+    with DeviceContext(args + ["--method", "dg", "--precision", precision, "--check-update"]) as dev:
         u = dev.initial()
         st = dev.advance(u)
     for it, got in enumerate(st["updates"]):
         w = float(run["iterations"][it]["residual_physical_rms"])
+        print(f"{precision} update {it}: inner Krylov {got['krylov_iterations']}, "
+              f"error vs FP64 {got['precision_error']:.3e}", flush=True)
         assert abs(got["residual_physical_rms"] - w) <= [1e-13, 1e-10][it] * w
         assert 0 <= got["precision_error"] <= 1e-9, got["precision_error"]
 
