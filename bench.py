@@ -329,7 +329,8 @@ def summarize_counters(fetch_csv, write_csv, out_json):
     cells, m = 256 * 256, 324
     d = cells * m
     alg = {"k_bj_apply<324, double": cells * m * m * 8 + cells * m * 4 + 2 * d * 8,
-           "k_bj_apply<324, float": cells * m * m * 4 + cells * m * 4 + 2 * d * 4}
+           "k_bj_apply<324, float": cells * m * m * 4 + cells * m * 4 + 2 * d * 4,
+           "k_bj_apply_pairs<324": cells * m * m * 2 + cells * m * 4 + 2 * d * 4}
     out = {"fetch_scale": sf, "write_scale": sw, "kernels": {}}
     print(f"# copy kernel (1 GiB read + 1 GiB write): FETCH_SIZE reads {1 / sf:.4f} GiB, WRITE_SIZE {1 / sw:.4f} GiB")
     print(f"# scale factors: fetch x{sf:.4f}, write x{sw:.4f}")
@@ -337,15 +338,15 @@ def summarize_counters(fetch_csv, write_csv, out_json):
           f"{'algorithmic GB':>15s} {'ratio':>6s}")
     full = {}
     for (name, grid) in fetch:
-        if name.startswith("void dgh::"):
+        if name.startswith(("void dgh::", "dgh::")):
             full[name] = max(full.get(name, 0), grid)
     for (name, grid) in fetch:
-        if not name.startswith("void dgh::"):
+        if not name.startswith(("void dgh::", "dgh::")):
             continue
         f = sum(fetch[(name, grid)]) / len(fetch[(name, grid)]) * sf
         ws = write.get((name, grid), [0.0])
         w = sum(ws) / len(ws) * sw
-        short = re.sub(r"^void dgh::\(anonymous namespace\)::", "", name)
+        short = re.sub(r"^(void )?dgh::\(anonymous namespace\)::", "", name)
         short = short[:short.index("(")] if "(" in short else short
         a = next((v for k, v in alg.items() if short.startswith(k)), None) if grid == full[name] else None
         key = short if grid == full[name] else f"{short} [one colour]"
