@@ -533,6 +533,16 @@ __global__ __launch_bounds__(LT) void k_lu(T *__restrict__ blocks, int *__restri
 #define DGH_BJ_RING32 4
 #endif
 constexpr int BJ_RING64 = DGH_BJ_RING64, BJ_RING32 = DGH_BJ_RING32;
+// Blocks of at most this many 64-row chunks (128 rows in the 2-byte kernel)
+// run branch-free: every lane loads every factor entry and the sweeps mask at
+// the use instead of predicating the loads, and the ring is refilled with a
+// clamped column index instead of a test for the end. Measured at 256^2, k=8:
+// the 36-row HDG face blocks gain (condensed HDG update 1.33 -> 1.27 s); the
+// 324-row element blocks lose 1-6% (the extra full-width loads and selects),
+// so they keep predicated loads and the tested refill.
+#ifndef DGH_BJ_BRANCH_FREE_CHUNKS
+#define DGH_BJ_BRANCH_FREE_CHUNKS 1
+#endif
 __device__ __forceinline__ float to_compute(float v) { return v; }
 __device__ __forceinline__ double to_compute(double v) { return v; }
 
@@ -600,22 +610,35 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
     // Packed factors (k_pack): the strict lower parts of columns 0..M-2 back
     // to back, then the upper parts (diagonal included) of columns M-1..0, so
     // that both sweeps stream their columns contiguously.
-    // strictly lower part of column c, chunks a >= a0
+    // Small blocks (BF): every lane loads, lanes outside the column's segment
+    // read its first entry (inside the block: for c = M-1 the first upper
+    // entry), and the sweeps mask those lanes where they use the value, so
+    // that no load sits under a branch. Otherwise the loads are predicated and
+    // the lanes outside the segment hold zeros.
+    constexpr bool BF = NA <= DGH_BJ_BRANCH_FREE_CHUNKS;
+    // strictly lower part of column c (rows c+1..M-1), chunks a >= a0
     auto load_lower = [&](T(&b)[NA], int c, int a0) {
         const T *colp = F + packed_lower(M, c);
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int row = a * 64 + lane;
-            if (a >= a0) b[a] = (row > c && row < M) ? factor_load(colp + (row - c - 1)) : T(0);
+            const bool in = row > c && row < M;
+            if (a >= a0) {
+                if constexpr (BF) b[a] = factor_load(colp + (in ? row - c - 1 : 0));
+                else b[a] = in ? factor_load(colp + (row - c - 1)) : T(0);
+            }
         }
     };
-    // upper part (diagonal included) of column c, chunks a <= a1
+    // upper part (rows 0..c, diagonal included) of column c, chunks a <= a1
     auto load_upper = [&](T(&b)[NA], int c, int a1) {
         const T *colp = F + packed_upper(M, c);
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int row = a * 64 + lane;
-            if (a <= a1) b[a] = row <= c ? factor_load(colp + row) : T(0);
+            if (a <= a1) {
+                if constexpr (BF) b[a] = factor_load(colp + (row <= c ? row : 0));
+                else b[a] = row <= c ? factor_load(colp + row) : T(0);
+            }
         }
     };
     // forward substitution with the unit lower L: x_i -= L_ij x_j for i > j;
@@ -630,12 +653,18 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
         for (int jl = 0; jl < jend; jl += U) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                if (jl + u < jend) {
+                if (jend % U == 0 || jl + u < jend) {
                     const int j = ja * 64 + jl + u;
                     const TV xj = readlane_d(x[ja], jl + u);
 #pragma unroll
-                    for (int a = ja; a < NA; ++a) x[a] -= static_cast<TV>(to_compute(buf[u][a])) * xj;
-                    if (j + U < M) load_lower(buf[u], j + U, ja);
+                    for (int a = ja; a < NA; ++a) {
+                        const int row = a * 64 + lane;
+                        const TV y = x[a] - static_cast<TV>(to_compute(buf[u][a])) * xj;
+                        // BF: rows j+1..M-1 only, other lanes hold an unrelated entry
+                        if (!BF || ((a > ja || row > j) && (a < NA - 1 || row < M))) x[a] = y;
+                    }
+                    if (BF) load_lower(buf[u], min(j + U, M - 1), ja);  // spare in-bounds load at the end
+                    else if (j + U < M) load_lower(buf[u], j + U, ja);
                 }
             }
         }
@@ -654,7 +683,7 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int jl = jend - 1 - g - u;
-                if (jl >= 0) {
+                if (jend % U == 0 || jl >= 0) {
                     const int j = ja * 64 + jl;
                     T(&cur)[NA] = buf[(t0 + u) % U];
                     const TV diag = readlane_d(static_cast<TV>(to_compute(cur[ja])), jl);
@@ -665,7 +694,8 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
                         const int row = a * 64 + lane;
                         if (row < j) x[a] -= static_cast<TV>(to_compute(cur[a])) * xj;
                     }
-                    if (j - U >= 0) load_upper(cur, j - U, ja);
+                    if (BF) load_upper(cur, max(j - U, 0), ja);  // spare in-bounds load at the end
+                    else if (j - U >= 0) load_upper(cur, j - U, ja);
                 }
             }
         }
@@ -803,6 +833,10 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
     unsigned buf[U][NA];
     auto lo16 = [](unsigned w) { return static_cast<unsigned short>(w & 0xffffu); };
     auto hi16 = [](unsigned w) { return static_cast<unsigned short>(w >> 16); };
+    // Small blocks (BF) as in k_bj_apply: every lane loads (lanes outside the
+    // segment read its first pair, for c = M-1 the first upper pair) and the
+    // sweeps mask those lanes at the use; otherwise predicated loads and zeros.
+    constexpr bool BF = NA <= DGH_BJ_BRANCH_FREE_CHUNKS;
     // lower part of column c from the pair holding row s(c), chunks a >= a0
     auto load_lower = [&](unsigned(&b)[NA], int c, int a0) {
         const int s0 = 2 * ((c + 1) / 2);
@@ -810,7 +844,11 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int r0 = 128 * a + 2 * lane;
-            if (a >= a0) b[a] = (r0 >= s0 && r0 < M) ? __builtin_nontemporal_load(colp + r0 / 2) : 0u;
+            const bool in = r0 >= s0 && r0 < M;
+            if (a >= a0) {
+                if constexpr (BF) b[a] = __builtin_nontemporal_load(colp + (in ? r0 : s0) / 2);
+                else b[a] = in ? __builtin_nontemporal_load(colp + r0 / 2) : 0u;
+            }
         }
     };
     // upper part of column c (rows 0..c and the pad), chunks a <= a1
@@ -819,7 +857,10 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int r0 = 128 * a + 2 * lane;
-            if (a <= a1) b[a] = r0 <= c ? __builtin_nontemporal_load(colp + r0 / 2) : 0u;
+            if (a <= a1) {
+                if constexpr (BF) b[a] = __builtin_nontemporal_load(colp + (r0 <= c ? r0 : 0) / 2);
+                else b[a] = r0 <= c ? __builtin_nontemporal_load(colp + r0 / 2) : 0u;
+            }
         }
     };
 #pragma unroll
@@ -831,15 +872,20 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
         for (int jl = 0; jl < jend; jl += U) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                if (jl + u < jend) {
+                if (jend % U == 0 || jl + u < jend) {
                     const int j = ja * 128 + jl + u;
                     const float xj = readlane_d(x[ja][u & 1], (jl + u) >> 1);
 #pragma unroll
                     for (int a = ja; a < NA; ++a) {
-                        x[a][0] -= half_value(lo16(buf[u][a]), FP16) * xj;
-                        x[a][1] -= half_value(hi16(buf[u][a]), FP16) * xj;
+                        // BF: rows j+1..M-1 only, other lanes hold an unrelated pair
+                        const int r0 = 128 * a + 2 * lane;
+                        const float y0 = x[a][0] - half_value(lo16(buf[u][a]), FP16) * xj;
+                        const float y1 = x[a][1] - half_value(hi16(buf[u][a]), FP16) * xj;
+                        if (!BF || ((a > ja || r0 > j) && (a < NA - 1 || r0 < M))) x[a][0] = y0;
+                        if (!BF || ((a > ja || r0 + 1 > j) && (a < NA - 1 || r0 + 1 < M))) x[a][1] = y1;
                     }
-                    if (j + U < M) load_lower(buf[u], j + U, ja);
+                    if (BF) load_lower(buf[u], min(j + U, M - 1), ja);  // spare in-bounds load at the end
+                    else if (j + U < M) load_lower(buf[u], j + U, ja);
                 }
             }
         }
@@ -860,7 +906,7 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int jl = jend - 1 - g - u;
-                if (jl >= 0) {
+                if (jend % U == 0 || jl >= 0) {
                     const int j = ja * 128 + jl;
                     unsigned(&cur)[NA] = buf[(t0 + u) % U];
                     // jend is even, so the parity of jl is that of u + 1
@@ -875,7 +921,8 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
                         if (r0 < j) x[a][0] -= half_value(lo16(cur[a]), FP16) * xj;
                         if (r0 + 1 < j) x[a][1] -= half_value(hi16(cur[a]), FP16) * xj;
                     }
-                    if (j - U >= 0) load_upper(cur, j - U, ja);
+                    if (BF) load_upper(cur, max(j - U, 0), ja);  // spare in-bounds load at the end
+                    else if (j - U >= 0) load_upper(cur, j - U, ja);
                 }
             }
         }
