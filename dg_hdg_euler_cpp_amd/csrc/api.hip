@@ -799,8 +799,9 @@ struct dgh_ctx {
         });
         timed("lu", [&] { HIPCHECK(launch_lu(m, factors_f32.p, rowsrc32.p, cells, flags.p + 2, stream)); });
         require(read_flag(2) == 0, "Singular block Jacobi factor");
-        pack_blocks(m, factors_f32.p, cells);
         ilu32_ready = false;
+        // the 2-byte forms convert the square LU factors directly; only the
+        // FP32 apply reads the packed layout
         if (two_byte_factors()) {
             factors_bf16.alloc(static_cast<size_t>(cells) * pair_factor_stride(m), bytes);
             factor_scale.alloc(cells, bytes);
@@ -808,6 +809,8 @@ struct dgh_ctx {
                 HIPCHECK(launch_to_pairs(m, factors_f32.p, factors_bf16.p, factor_scale.p, cells,
                                          cfg.precision == DGH_PRECISION_MIXED_FP16, stream));
             });
+        } else {
+            pack_blocks(m, factors_f32.p, cells);
         }
         bj32_ready = true;
     }

@@ -147,9 +147,10 @@ hipError_t launch_bj_apply_colour(int m, const float *factors, const int *rowsrc
 // FP64 <-> FP32 vector conversion
 hipError_t launch_convert(const double *src, float *dst, long long count, hipStream_t s);
 hipError_t launch_convert(const float *src, double *dst, long long count, hipStream_t s);
-// 2-byte factors in the pair layout (BF16, or FP16 with a per-block scale of U)
+// 2-byte factors in the pair layout (BF16, or FP16 with a per-block scale of U),
+// converted from the square column-major FP32 LU factors
 long long pair_factor_stride(int m);
-hipError_t launch_to_pairs(int m, const float *packed, void *out, float *sigma, int nblocks, int fp16,
+hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, int nblocks, int fp16,
                            hipStream_t s);
 hipError_t launch_bj_apply_pairs(int m, const void *factors, const float *sigma, const int *rowsrc, const float *r,
                                  float *z, int nblocks, int fp16, hipStream_t s);

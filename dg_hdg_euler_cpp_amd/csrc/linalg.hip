@@ -747,21 +747,23 @@ __device__ __forceinline__ float half_value(unsigned short u, bool fp16) {
     __builtin_memcpy(&b, &u, 2);
     return __bfloat162float(b);
 }
-// One block per factor: the packed FP32 LU (k_pack layout) into the pair
-// layout in 2 bytes. FP16 scales U by sigma = 2^-e, e = floor(log2 max|U|), so
-// that the largest entry lies in [1, 2) (L is unscaled: |L_ij| <= 1 after
-// partial pivoting); BF16 keeps sigma = 1.
-__global__ __launch_bounds__(BLOCK) void k_to_pairs(const float *__restrict__ packed, unsigned short *__restrict__ out,
+// One block per factor: the square column-major FP32 LU (k_lu's output:
+// unit L strictly below the diagonal, U on and above it) into the pair layout
+// in 2 bytes. FP16 scales U by sigma = 2^-e, e = floor(log2 max|U|), so that
+// the largest entry lies in [1, 2) (L is unscaled: |L_ij| <= 1 after partial
+// pivoting); BF16 keeps sigma = 1.
+__global__ __launch_bounds__(BLOCK) void k_to_pairs(const float *__restrict__ square, unsigned short *__restrict__ out,
                                                     float *__restrict__ sigma, int M, int fp16) {
     __shared__ float red[BLOCK / 64];
     __shared__ float scale;
-    const long long mm = static_cast<long long>(M) * M, stride = pair_stride(M);
-    const float *src = packed + blockIdx.x * mm;
+    const int mm = M * M;
+    const long long stride = pair_stride(M);
+    const float *src = square + blockIdx.x * static_cast<long long>(mm);
     unsigned short *dst = out + blockIdx.x * stride;
-    const long long nl = static_cast<long long>(M) * (M - 1) / 2;  // lower entries, then upper
     float mx = 0;
     if (fp16)
-        for (long long t = nl + threadIdx.x; t < mm; t += BLOCK) mx = fmaxf(mx, fabsf(src[t]));
+        for (int t = threadIdx.x; t < mm; t += BLOCK)
+            if (t % M <= t / M) mx = fmaxf(mx, fabsf(src[t]));
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
     if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = mx;
     __syncthreads();
@@ -774,35 +776,11 @@ __global__ __launch_bounds__(BLOCK) void k_to_pairs(const float *__restrict__ pa
     for (long long t = threadIdx.x; t < stride; t += BLOCK) dst[t] = 0;
     __syncthreads();
     const float sc = scale;
-    for (long long t = threadIdx.x; t < mm; t += BLOCK) {
-        // invert k_pack's layout: lower parts of columns 0..M-2, then upper parts of M-1..0
-        int c, row;
-        float v;
-        if (t < nl) {
-            // column c with packed_lower(M, c) <= t < packed_lower(M, c + 1)
-            int lo = 0, hi = M - 2;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) / 2;
-                if (packed_lower(M, mid) <= t) lo = mid;
-                else hi = mid - 1;
-            }
-            c = lo;
-            row = c + 1 + static_cast<int>(t - packed_lower(M, c));
-            v = src[t];
-            dst[pair_lower(M, c) + (row - 2 * ((c + 1) / 2))] = half_bits(v, fp16);
-        } else {
-            // column c with packed_upper(M, c) <= t < packed_upper(M, c) + c + 1 (descending c)
-            int lo = 0, hi = M - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi) / 2;
-                if (packed_upper(M, mid) <= t) hi = mid;
-                else lo = mid + 1;
-            }
-            c = lo;
-            row = static_cast<int>(t - packed_upper(M, c));
-            v = src[t] * sc;
-            dst[pair_upper(M, c) + row] = half_bits(v, fp16);
-        }
+    for (int t = threadIdx.x; t < mm; t += BLOCK) {
+        const int c = t / M, row = t % M;
+        const float v = src[t];
+        if (row > c) dst[pair_lower(M, c) + (row - 2 * ((c + 1) / 2))] = half_bits(v, fp16);
+        else dst[pair_upper(M, c) + row] = half_bits(v * sc, fp16);
     }
 }
 
@@ -1391,9 +1369,9 @@ hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const
     })
 }
 long long pair_factor_stride(int m) { return pair_stride(m); }
-hipError_t launch_to_pairs(int m, const float *packed, void *out, float *sigma, int nblocks, int fp16,
+hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, int nblocks, int fp16,
                            hipStream_t s) {
-    k_to_pairs<<<nblocks, BLOCK, 0, s>>>(packed, static_cast<unsigned short *>(out), sigma, m, fp16);
+    k_to_pairs<<<nblocks, BLOCK, 0, s>>>(square, static_cast<unsigned short *>(out), sigma, m, fp16);
     return hipGetLastError();
 }
 // columns in flight per wave (measured at 256^2, k=8: FP16 2.67 / 2.77 / 3.18 ms
