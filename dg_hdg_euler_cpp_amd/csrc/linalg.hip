@@ -540,6 +540,16 @@ constexpr int BJ_RING64 = DGH_BJ_RING64, BJ_RING32 = DGH_BJ_RING32;
 // the 36-row HDG face blocks gain (condensed HDG update 1.33 -> 1.27 s); the
 // 324-row element blocks lose 1-6% (the extra full-width loads and selects),
 // so they keep predicated loads and the tested refill.
+// Block-Jacobi applies: factor loads as raw buffer loads with scalar column
+// offsets, bit-identical to the predicated loads. Measured at 256^2, k=8: the
+// FP16 apply 2.68-2.71 -> 2.60 ms (kept); the FP64 and FP32 applies unchanged
+// within run-to-run noise (8.8-9.0 and 4.5-4.6 ms either way; off).
+#ifndef DGH_BJ_PAIRS_BUFFER
+#define DGH_BJ_PAIRS_BUFFER 1
+#endif
+#ifndef DGH_BJ_BUFFER
+#define DGH_BJ_BUFFER 0
+#endif
 #ifndef DGH_BJ_BRANCH_FREE_CHUNKS
 #define DGH_BJ_BRANCH_FREE_CHUNKS 1
 #endif
@@ -595,7 +605,10 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
     using T = TF;
     constexpr int NA = (M + 63) / 64;
     const int lane = threadIdx.x % 64;
-    const long long blk = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + threadIdx.x / 64;
+    // the wave index is uniform across the wave: say so, so that the block's
+    // addresses stay in scalar registers
+    const long long blk = static_cast<long long>(blockIdx.x) * (BLOCK / 64) +
+                          __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64));
     if (blk >= nblocks) return;
     const T *F = factors + blk * M * M;
     // vector position of the block: its cell (compact colour storage) or itself
@@ -616,6 +629,43 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
     // that no load sits under a branch. Otherwise the loads are predicated and
     // the lanes outside the segment hold zeros.
     constexpr bool BF = NA <= DGH_BJ_BRANCH_FREE_CHUNKS;
+#if DGH_BJ_BUFFER
+    // Raw buffer loads (as in k_bj_apply_pairs): scalar column offsets, fixed
+    // lane offsets, and an offset past the end for lanes outside the column's
+    // segment, which load 0. The descriptor starts M entries before the block
+    // so that the lower part's column offset stays non-negative.
+    constexpr unsigned OOB = 0x80000000u;
+    constexpr int S = sizeof(T);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<T *>(F - M), 0, (M * M + M) * S, 0x00020000);
+    unsigned voff[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) voff[a] = 64 * a + lane < M ? static_cast<unsigned>((64 * a + lane) * S) : OOB;
+    auto raw_load = [&](unsigned vo, int soff) -> T {
+        if constexpr (sizeof(T) == 8) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, vo, soff, 2);
+            return __builtin_bit_cast(T, v);
+        } else {
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, vo, soff, 2));
+        }
+    };
+    // strictly lower part of column c (rows c+1..M-1), chunks a >= a0; the
+    // segment starts in chunk a0 or a0 + 1 (c < 64 (a0 + 1) + U)
+    auto load_lower = [&](T(&b)[NA], int c, int a0) {
+        const int soff = static_cast<int>(packed_lower(M, c) - c - 1 + M) * S;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+            if (a >= a0) b[a] = raw_load((a <= a0 + 1 && 64 * a + lane <= c) ? OOB : voff[a], soff);
+    };
+    // upper part (rows 0..c, diagonal included) of column c, chunks a <= a1;
+    // c lies in chunk a1 or a1 - 1
+    auto load_upper = [&](T(&b)[NA], int c, int a1) {
+        const int soff = static_cast<int>(packed_upper(M, c) + M) * S;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+            if (a <= a1) b[a] = raw_load((a >= a1 - 1 && 64 * a + lane > c) ? OOB : voff[a], soff);
+    };
+#else
     // strictly lower part of column c (rows c+1..M-1), chunks a >= a0
     auto load_lower = [&](T(&b)[NA], int c, int a0) {
         const T *colp = F + packed_lower(M, c);
@@ -641,6 +691,7 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
             }
         }
     };
+#endif
     // forward substitution with the unit lower L: x_i -= L_ij x_j for i > j;
     // column j sits in ring slot j % U
 #pragma unroll
@@ -797,7 +848,10 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
     static_assert(M % 2 == 0 && 128 % U == 0 && U % 2 == 0, "pair layout");
     constexpr int NA = (M + 127) / 128;
     const int lane = threadIdx.x % 64;
-    const long long blk = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + threadIdx.x / 64;
+    // the wave index is uniform across the wave: say so, so that the block's
+    // addresses stay in scalar registers
+    const long long blk = static_cast<long long>(blockIdx.x) * (BLOCK / 64) +
+                          __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64));
     if (blk >= nblocks) return;
     const unsigned *F = reinterpret_cast<const unsigned *>(factors + blk * pair_stride(M));
     float x[NA][2];
@@ -815,6 +869,43 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
     // segment read its first pair, for c = M-1 the first upper pair) and the
     // sweeps mask those lanes at the use; otherwise predicated loads and zeros.
     constexpr bool BF = NA <= DGH_BJ_BRANCH_FREE_CHUNKS;
+#if DGH_BJ_PAIRS_BUFFER
+    // Raw buffer loads: the column's start is a scalar offset and each lane's
+    // offset within the column is fixed per chunk, so no per-load address
+    // arithmetic runs on the vector ALUs. Lanes outside the column's segment
+    // get an offset past the block's end, for which the load returns 0.
+    constexpr unsigned OOB = 0x80000000u;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<unsigned *>(F), 0, static_cast<int>(pair_stride(M) * 2), 0x00020000);
+    unsigned voff[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) voff[a] = 128 * a + 2 * lane < M ? 256u * a + 4u * lane : OOB;
+    // lower part of column c (rows s(c)..M-1), chunks a >= a0; for c < M the
+    // segment starts in chunk a0 or a0 + 1, so later chunks need no test
+    auto load_lower = [&](unsigned(&b)[NA], int c, int a0) {
+        const int s0 = 2 * ((c + 1) / 2);
+        const int soff = 2 * static_cast<int>(pair_lower(M, c) - s0);
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            if (a >= a0) {
+                const unsigned vo = (a <= a0 + 1 && 128 * a + 2 * lane < s0) ? OOB : voff[a];
+                b[a] = __builtin_amdgcn_raw_buffer_load_b32(rs, vo, soff, 2);
+            }
+        }
+    };
+    // upper part of column c (rows 0..c and the pad), chunks a <= a1; c lies
+    // in chunk a1 or a1 - 1, so earlier chunks need no test
+    auto load_upper = [&](unsigned(&b)[NA], int c, int a1) {
+        const int soff = 2 * static_cast<int>(pair_upper(M, c));
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            if (a <= a1) {
+                const unsigned vo = (a >= a1 - 1 && 128 * a + 2 * lane > c) ? OOB : voff[a];
+                b[a] = __builtin_amdgcn_raw_buffer_load_b32(rs, vo, soff, 2);
+            }
+        }
+    };
+#else
     // lower part of column c from the pair holding row s(c), chunks a >= a0
     auto load_lower = [&](unsigned(&b)[NA], int c, int a0) {
         const int s0 = 2 * ((c + 1) / 2);
@@ -841,6 +932,7 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
             }
         }
     };
+#endif
 #pragma unroll
     for (int u = 0; u < U; ++u) load_lower(buf[u], u, 0);
 #pragma unroll
