@@ -1,5 +1,6 @@
-"""Mixed-precision device path (no reference counterpart): FP32 / BF16-stored
-block-Jacobi factors, FP32 Jv, FP32 inner GMRES inside FP64 flexible GMRES.
+"""Mixed-precision device path (no reference counterpart): FP32, FP16 (with a
+power-of-two scale of U per block) or BF16-stored block-Jacobi factors, FP32
+Jv, FP32 inner GMRES inside FP64 flexible GMRES.
 
 The FP64 oracle pins it: the FP32 pieces against round-off bounds of their
 storage precision, the FGMRES solution against the oracle's own linear system
