@@ -59,10 +59,21 @@ def load_library():
     return L
 
 
-def _ptr(t):
-    """Device pointer of a float64 torch tensor (contiguous), as c_void_p."""
+def _ptr(t, dtype, numel):
+    """Device pointer of a torch tensor, as c_void_p, after checking it against
+    what the C entry point assumes: a contiguous tensor on the HIP device with
+    exactly `numel` entries of `dtype`. The kernels index the raw pointer, so a
+    CPU tensor, another dtype or a shorter buffer would be read or written out
+    of bounds; they are refused here instead. None passes through (optional
+    arguments)."""
     if t is None:
         return None
+    if not getattr(t, "is_cuda", False):
+        raise DeviceError("device vectors must be tensors on the HIP device")
+    if t.dtype != dtype:
+        raise DeviceError(f"device vector has dtype {t.dtype}, expected {dtype}")
+    if t.numel() != numel:
+        raise DeviceError(f"device vector has {t.numel()} entries, expected {numel}")
     if not t.is_contiguous():
         raise DeviceError("device vectors must be contiguous")
     return C.c_void_p(t.data_ptr())
@@ -118,7 +129,19 @@ class DeviceContext:
     def __exit__(self, *exc):
         self.close()
 
-    # ---- helpers
+    # ---- helpers: checked device pointers of the vectors each entry point takes
+    def _v(self, t):
+        return _ptr(t, self._torch.float64, self.size)
+
+    def _v32(self, t):
+        return _ptr(t, self._torch.float32, self.size)
+
+    def _tv(self, t):
+        return _ptr(t, self._torch.float64, self.trace_size)
+
+    def _speeds(self, t):
+        return _ptr(t, self._torch.float64, 2 * self.faces * self.n)
+
     def _call(self, name, *args):
         L = load_library()
         self._torch.cuda.synchronize()
@@ -143,74 +166,74 @@ class DeviceContext:
     # ---- state / Problem
     def initial(self):
         u = self.zeros()
-        self._call("dgh_initial", _ptr(u))
+        self._call("dgh_initial", self._v(u))
         return u
 
     def seeded(self, seed):
         u = self.zeros()
-        self._call("dgh_seeded", C.c_uint64(seed), _ptr(u))
+        self._call("dgh_seeded", C.c_uint64(seed), self._v(u))
         return u
 
     def freeze(self, u):
         rate = C.c_double()
-        self._call("dgh_freeze", _ptr(u), C.byref(rate))
+        self._call("dgh_freeze", self._v(u), C.byref(rate))
         return rate.value
 
     def speeds(self):
         out = self.zeros(2 * self.faces * self.n)
-        self._call("dgh_speeds", _ptr(out))
+        self._call("dgh_speeds", self._speeds(out))
         return out
 
     def set_step(self, tau, old=None):
-        self._call("dgh_set_step", C.c_double(tau), _ptr(old))
+        self._call("dgh_set_step", C.c_double(tau), self._v(old))
 
     def dg_residual(self, u, out=None):
         r = self.zeros() if out is None else out
-        self._call("dgh_dg_residual", _ptr(u), _ptr(r))
+        self._call("dgh_dg_residual", self._v(u), self._v(r))
         return r
 
     def admissible(self, u):
         ok = C.c_int()
-        self._call("dgh_admissible", _ptr(u), C.byref(ok))
+        self._call("dgh_admissible", self._v(u), C.byref(ok))
         return bool(ok.value)
 
     def residual_norm(self, r):
         out = C.c_double()
-        self._call("dgh_residual_norm", _ptr(r), C.byref(out))
+        self._call("dgh_residual_norm", self._v(r), C.byref(out))
         return out.value
 
     def dual_norm(self, x):
         out = C.c_double()
-        self._call("dgh_dual_norm", _ptr(x), C.byref(out))
+        self._call("dgh_dual_norm", self._v(x), C.byref(out))
         return out.value
 
     def state_norm(self, x):
         out = C.c_double()
-        self._call("dgh_state_norm", _ptr(x), C.byref(out))
+        self._call("dgh_state_norm", self._v(x), C.byref(out))
         return out.value
 
     def integral(self, u):
         out = (C.c_double * 4)()
-        self._call("dgh_integral", _ptr(u), out)
+        self._call("dgh_integral", self._v(u), out)
         return list(out)
 
     def error_l2(self, u, t):
         out = (C.c_double * 4)()
-        self._call("dgh_error_l2", _ptr(u), C.c_double(t), out)
+        self._call("dgh_error_l2", self._v(u), C.c_double(t), out)
         return list(out)
 
     # ---- DG operator / block Jacobi / GMRES
     def dg_refresh(self, u):
-        self._call("dgh_dg_refresh", _ptr(u))
+        self._call("dgh_dg_refresh", self._v(u))
 
     def dg_apply(self, x, out=None):
         y = self.zeros() if out is None else out
-        self._call("dgh_dg_apply", _ptr(x), _ptr(y))
+        self._call("dgh_dg_apply", self._v(x), self._v(y))
         return y
 
     def dg_apply_transpose(self, w, out=None):
         z = self.zeros() if out is None else out
-        self._call("dgh_dg_apply_transpose", _ptr(w), _ptr(z))
+        self._call("dgh_dg_apply_transpose", self._v(w), self._v(z))
         return z
 
     def dg_block(self, row):
@@ -225,7 +248,7 @@ class DeviceContext:
 
     def bj_apply(self, r, out=None):
         z = self.zeros() if out is None else out
-        self._call("dgh_bj_apply", _ptr(r), _ptr(z))
+        self._call("dgh_bj_apply", self._v(r), self._v(z))
         return z
 
     def ilu_refresh(self):
@@ -233,7 +256,7 @@ class DeviceContext:
 
     def ilu_apply(self, r, out=None):
         z = self.zeros() if out is None else out
-        self._call("dgh_ilu_apply", _ptr(r), _ptr(z))
+        self._call("dgh_ilu_apply", self._v(r), self._v(z))
         return z
 
     def bj_refresh_mixed(self):
@@ -241,12 +264,12 @@ class DeviceContext:
 
     def bj_apply_f32(self, r, out=None):
         z = self._torch.zeros(self.size, dtype=self._torch.float32, device="cuda") if out is None else out
-        self._call("dgh_bj_apply_f32", _ptr(r), _ptr(z))
+        self._call("dgh_bj_apply_f32", self._v32(r), self._v32(z))
         return z
 
     def dg_apply_f32(self, x, out=None):
         y = self._torch.zeros(self.size, dtype=self._torch.float32, device="cuda") if out is None else out
-        self._call("dgh_dg_apply_f32", _ptr(x), _ptr(y))
+        self._call("dgh_dg_apply_f32", self._v32(x), self._v32(y))
         return y
 
     def dg_fgmres(self, rhs, max_iterations, tolerance):
@@ -254,7 +277,7 @@ class DeviceContext:
         its = C.c_int()
         outer = C.c_int()
         rel = C.c_double()
-        self._call("dgh_dg_fgmres", _ptr(rhs), _ptr(x), C.c_int(max_iterations), C.c_double(tolerance),
+        self._call("dgh_dg_fgmres", self._v(rhs), self._v(x), C.c_int(max_iterations), C.c_double(tolerance),
                    C.byref(its), C.byref(outer), C.byref(rel))
         return x, its.value, outer.value, rel.value
 
@@ -263,8 +286,8 @@ class DeviceContext:
         res = self.zeros()
         its = C.c_int()
         rel = C.c_double()
-        self._call("dgh_dg_gmres", _ptr(rhs), _ptr(x), C.c_int(max_iterations), C.c_double(tolerance),
-                   C.byref(its), C.byref(rel), _ptr(res))
+        self._call("dgh_dg_gmres", self._v(rhs), self._v(x), C.c_int(max_iterations), C.c_double(tolerance),
+                   C.byref(its), C.byref(rel), self._v(res))
         return x, its.value, rel.value, res
 
     # ---- condensed HDG
@@ -272,16 +295,16 @@ class DeviceContext:
         return self.zeros(self.trace_size)
 
     def hdg_refresh_projected(self, u, r):
-        self._call("dgh_hdg_refresh_projected", _ptr(u), _ptr(r))
+        self._call("dgh_hdg_refresh_projected", self._v(u), self._v(r))
 
     def hdg_rhs(self):
         out = self.tzeros()
-        self._call("dgh_hdg_rhs", _ptr(out))
+        self._call("dgh_hdg_rhs", self._tv(out))
         return out
 
     def hdg_apply(self, x, out=None):
         y = self.tzeros() if out is None else out
-        self._call("dgh_hdg_apply", _ptr(x), _ptr(y))
+        self._call("dgh_hdg_apply", self._tv(x), self._tv(y))
         return y
 
     def hdg_block(self, face):
@@ -296,17 +319,17 @@ class DeviceContext:
 
     def hdg_bj_apply(self, r, out=None):
         z = self.tzeros() if out is None else out
-        self._call("dgh_hdg_bj_apply", _ptr(r), _ptr(z))
+        self._call("dgh_hdg_bj_apply", self._tv(r), self._tv(z))
         return z
 
     def hdg_recover(self, dl, out=None):
         du = self.zeros() if out is None else out
-        self._call("dgh_hdg_recover", _ptr(dl), _ptr(du))
+        self._call("dgh_hdg_recover", self._tv(dl), self._v(du))
         return du
 
     def hdg_recovered_residual(self, e, out=None):
         r = self.zeros() if out is None else out
-        self._call("dgh_hdg_recovered_residual", _ptr(e), _ptr(r))
+        self._call("dgh_hdg_recovered_residual", self._tv(e), self._v(r))
         return r
 
     def hdg_gmres(self, rhs, max_iterations, tolerance):
@@ -314,15 +337,15 @@ class DeviceContext:
         res = self.tzeros()
         its = C.c_int()
         rel = C.c_double()
-        self._call("dgh_hdg_gmres", _ptr(rhs), _ptr(x), C.c_int(max_iterations), C.c_double(tolerance),
-                   C.byref(its), C.byref(rel), _ptr(res))
+        self._call("dgh_hdg_gmres", self._tv(rhs), self._tv(x), C.c_int(max_iterations), C.c_double(tolerance),
+                   C.byref(its), C.byref(rel), self._tv(res))
         return x, its.value, rel.value, res
 
     def _run(self, entry, u, capacity):
         run = RunStats()
         upd = (UpdateStats * capacity)()
         count = C.c_int()
-        self._call(entry, _ptr(u), C.byref(run), upd, C.c_int(capacity), C.byref(count))
+        self._call(entry, self._v(u), C.byref(run), upd, C.c_int(capacity), C.byref(count))
         stats = {f: getattr(run, f) for f, _ in RunStats._fields_}
         stats["updates"] = [{f: getattr(upd[i], f) for f, _ in UpdateStats._fields_}
                             for i in range(min(count.value, capacity))]
@@ -337,7 +360,7 @@ class DeviceContext:
 
     def bj_apply_transposed(self, r, out=None):
         z = self.zeros() if out is None else out
-        self._call("dgh_bj_apply_transposed", _ptr(r), _ptr(z))
+        self._call("dgh_bj_apply_transposed", self._v(r), self._v(z))
         return z
 
     def dg_gmres_transposed(self, rhs, max_iterations, tolerance):
@@ -345,8 +368,8 @@ class DeviceContext:
         res = self.zeros()
         its = C.c_int()
         rel = C.c_double()
-        self._call("dgh_dg_gmres_transposed", _ptr(rhs), _ptr(x), C.c_int(max_iterations), C.c_double(tolerance),
-                   C.byref(its), C.byref(rel), _ptr(res))
+        self._call("dgh_dg_gmres_transposed", self._v(rhs), self._v(x), C.c_int(max_iterations), C.c_double(tolerance),
+                   C.byref(its), C.byref(rel), self._v(res))
         return x, its.value, rel.value, res
 
     def trajectory_record(self, u, capacity=100000):
@@ -360,13 +383,13 @@ class DeviceContext:
     def adjoint_backward(self, final_gradient):
         grad = self.zeros()
         st = AdjointStats()
-        self._call("dgh_adjoint_backward", _ptr(final_gradient), _ptr(grad), C.byref(st))
+        self._call("dgh_adjoint_backward", self._v(final_gradient), self._v(grad), C.byref(st))
         return grad, {f: getattr(st, f) for f, _ in AdjointStats._fields_}
 
     def quadrant_density(self, u, with_gradient=True):
         value = C.c_double()
         g = self.zeros() if with_gradient else None
-        self._call("dgh_quadrant_density", _ptr(u), C.byref(value), _ptr(g) if g is not None else None)
+        self._call("dgh_quadrant_density", self._v(u), C.byref(value), self._v(g) if g is not None else None)
         return (value.value, g) if with_gradient else value.value
 
     # ---- timing hooks
@@ -455,6 +478,12 @@ class DeviceContext1D:
     def __exit__(self, *exc):
         self.close()
 
+    def _v(self, t):
+        return _ptr(t, self._torch.float64, self.size)
+
+    def _tv(self, t):
+        return _ptr(t, self._torch.float64, 3 * self.cells)
+
     def _call(self, name, *args):
         L = _lib1()
         # torch's fills and copies of the arguments run on torch's stream; the
@@ -468,35 +497,35 @@ class DeviceContext1D:
 
     def initial(self):
         u = self.zeros()
-        self._call("dgh1_initial", _ptr(u))
+        self._call("dgh1_initial", self._v(u))
         return u
 
     def freeze(self, u):
         rate = C.c_double()
         speeds = (C.c_double * (2 * self.cells))()
-        self._call("dgh1_freeze", _ptr(u), C.byref(rate), speeds)
+        self._call("dgh1_freeze", self._v(u), C.byref(rate), speeds)
         self.speeds = list(speeds)
         return rate.value
 
     def set_step(self, tau, old):
-        self._call("dgh1_set_step", C.c_double(tau), _ptr(old))
+        self._call("dgh1_set_step", C.c_double(tau), self._v(old))
 
     def residual(self, u):
         r = self.zeros()
-        self._call("dgh1_residual", _ptr(u), _ptr(r))
+        self._call("dgh1_residual", self._v(u), self._v(r))
         return r
 
     def admissible(self, u):
         ok = C.c_int()
-        self._call("dgh1_admissible", _ptr(u), C.byref(ok))
+        self._call("dgh1_admissible", self._v(u), C.byref(ok))
         return bool(ok.value)
 
     def refresh(self, u):
-        self._call("dgh1_refresh", _ptr(u))
+        self._call("dgh1_refresh", self._v(u))
 
     def apply(self, x):
         y = self.zeros()
-        self._call("dgh1_apply", _ptr(x), _ptr(y))
+        self._call("dgh1_apply", self._v(x), self._v(y))
         return y
 
     def block(self, row):
@@ -511,39 +540,39 @@ class DeviceContext1D:
 
     def bj_apply(self, r):
         z = self.zeros()
-        self._call("dgh1_bj_apply", _ptr(r), _ptr(z))
+        self._call("dgh1_bj_apply", self._v(r), self._v(z))
         return z
 
     def gmres(self, rhs, limit, tol):
         x = self.zeros()
         its, rel = C.c_int(), C.c_double()
-        self._call("dgh1_gmres", _ptr(rhs), _ptr(x), C.c_int(limit), C.c_double(tol), C.byref(its), C.byref(rel))
+        self._call("dgh1_gmres", self._v(rhs), self._v(x), C.c_int(limit), C.c_double(tol), C.byref(its), C.byref(rel))
         return x, its.value, rel.value
 
     def error_l2(self, u, t):
         e = (C.c_double * 3)()
-        self._call("dgh1_error", _ptr(u), C.c_double(t), e)
+        self._call("dgh1_error", self._v(u), C.c_double(t), e)
         return list(e)
 
     def integral(self, u):
         e = (C.c_double * 3)()
-        self._call("dgh1_integral", _ptr(u), e)
+        self._call("dgh1_integral", self._v(u), e)
         return list(e)
 
     def tzeros(self):
         return self._torch.zeros(3 * self.cells, dtype=self._torch.float64, device="cuda")
 
     def hdg_refresh(self, u, r):
-        self._call("dgh1_hdg_refresh", _ptr(u), _ptr(r))
+        self._call("dgh1_hdg_refresh", self._v(u), self._v(r))
 
     def hdg_rhs(self):
         y = self.tzeros()
-        self._call("dgh1_hdg_rhs", _ptr(y))
+        self._call("dgh1_hdg_rhs", self._tv(y))
         return y
 
     def hdg_apply(self, x):
         y = self.tzeros()
-        self._call("dgh1_hdg_apply", _ptr(x), _ptr(y))
+        self._call("dgh1_hdg_apply", self._tv(x), self._tv(y))
         return y
 
     def hdg_block(self, face):
@@ -555,23 +584,23 @@ class DeviceContext1D:
 
     def hdg_bj_apply(self, r):
         z = self.tzeros()
-        self._call("dgh1_hdg_bj_apply", _ptr(r), _ptr(z))
+        self._call("dgh1_hdg_bj_apply", self._tv(r), self._tv(z))
         return z
 
     def hdg_recover(self, dl):
         du = self.zeros()
-        self._call("dgh1_hdg_recover", _ptr(dl), _ptr(du))
+        self._call("dgh1_hdg_recover", self._tv(dl), self._v(du))
         return du
 
     def hdg_recovered_residual(self, dl):
         out = self.zeros()
-        self._call("dgh1_hdg_recovered_residual", _ptr(dl), _ptr(out))
+        self._call("dgh1_hdg_recovered_residual", self._tv(dl), self._v(out))
         return out
 
     def hdg_gmres(self, rhs, limit, tol):
         x = self.tzeros()
         its, rel = C.c_int(), C.c_double()
-        self._call("dgh1_hdg_gmres", _ptr(rhs), _ptr(x), C.c_int(limit), C.c_double(tol), C.byref(its),
+        self._call("dgh1_hdg_gmres", self._tv(rhs), self._tv(x), C.c_int(limit), C.c_double(tol), C.byref(its),
                    C.byref(rel))
         return x, its.value, rel.value
 
@@ -579,7 +608,7 @@ class DeviceContext1D:
         run = RunStats()
         ups = (UpdateStats * capacity)()
         count = C.c_int()
-        self._call("dgh1_advance", _ptr(u), C.byref(run), ups, C.c_int(capacity), C.byref(count))
+        self._call("dgh1_advance", self._v(u), C.byref(run), ups, C.c_int(capacity), C.byref(count))
         out = {k: getattr(run, k) for k, _ in RunStats._fields_}
         out["updates"] = [{k: getattr(ups[i], k) for k, _ in UpdateStats._fields_}
                           for i in range(min(count.value, capacity))]

@@ -1,28 +1,30 @@
-"""Experiment drivers for the device executable, with the reference's protocols.
+"""The reference's experiment protocols, run through the device executables.
 
-    python -m dg_hdg_euler_cpp_amd.harness check    --output DIR
-    python -m dg_hdg_euler_cpp_amd.harness accuracy --output DIR [--grids 32 64 128 256]
-    python -m dg_hdg_euler_cpp_amd.harness speed    --output DIR [--cells 256] [--degrees 8 7 ...]
+    python -m dg_hdg_euler_cpp_amd.harness check      --output DIR
+    python -m dg_hdg_euler_cpp_amd.harness accuracy   --output DIR [--grids 32 64 128 256]
+    python -m dg_hdg_euler_cpp_amd.harness speed      --output DIR [--cells 256] [--degrees 8 7 ...]
     python -m dg_hdg_euler_cpp_amd.harness accuracy1d --output DIR [--grids 32 64 128 256 512]
 
-Each runs `bin/dg_hdg_2d --device gpu` with the command lines of the
-reference's drivers and applies their acceptance rules:
-* check: code_test/2d/test_solver.py:13-23 -- 4x4, k=2, DIRK3 / BE / entropy
-  wave, DG and HDG with --check-update; DG/HDG max nodal difference <= 1e-9,
-  conservation drift <= 1e-10;
-* accuracy: accuracy_test/2d/convergence.py:20-47 -- Q2, DIRK3, CFL 2, T 0.2,
-  paired DG/HDG per grid; observed orders per component and the DG/HDG
-  difference columns, written to convergence.csv;
-* speed: speed_test/2d/benchmark.py:21-43 -- BE, CFL 5, four Newton updates
-  with the first as warm-up, DG then HDG per degree; timings.csv and
-  speedup.csv, with the checks of scripts/postprocess.py:190-195 (four
-  updates, warm-up flags 0,1,1,1, the summary mean equals the mean of the
-  three measured iteration times).
-* accuracy1d: accuracy_test/1d/convergence.py:19-47 with `bin/dg_hdg_1d` --
-  k=2, DIRK3, CFL 2, T 0.2, GMRES(20) + block Jacobi, DG and HDG paired per
-  grid, log2 orders per component and the DG/HDG difference columns, written
-  to convergence.csv.
-A SUCCESS file is written when every run passed; any failure raises.
+Each protocol is data: the cases it runs (one solver invocation each), the
+command-line flags of a case, the acceptance rules evaluated on the CSV files a
+case wrote, and the reports assembled from every case so far. One runner
+executes any protocol, so the four experiments differ only in their tables.
+
+What is kept from the reference's drivers is what their outputs are judged by:
+the command lines, the acceptance thresholds and the CSV schemas of the
+reports (convergence.csv, timings.csv, speedup.csv), which its postprocessing
+reads (scripts/postprocess.py:180-200):
+  * check      -- code_test/2d/test_solver.py:13-22: 4x4, k=2, CFL 2, T 0.02,
+                  DIRK3 / BE / entropy wave, DG and HDG with --check-update;
+                  DG/HDG max nodal difference <= 1e-9, conservation drift <= 1e-10;
+  * accuracy   -- accuracy_test/2d/convergence.py:17-45: Q2, DIRK3, CFL 2,
+                  T 0.2, DG and HDG paired per grid; observed orders per
+                  component and the DG/HDG difference columns;
+  * speed      -- speed_test/2d/benchmark.py:19-43: BE, CFL 5, one step of four
+                  Newton updates, the first a warm-up, DG then HDG per degree;
+  * accuracy1d -- accuracy_test/1d/convergence.py:17-48 with bin/dg_hdg_1d.
+A SUCCESS file is written when every case passed every rule; the first failed
+rule raises ProtocolFailure naming the case and the rule.
 """
 from __future__ import annotations
 
@@ -33,185 +35,313 @@ import os
 import subprocess
 import sys
 import time
+from dataclasses import dataclass, field
 from pathlib import Path
+from typing import Callable
 
 EXE = Path(__file__).resolve().parent / "bin" / "dg_hdg_2d"
 EXE1D = Path(__file__).resolve().parent / "bin" / "dg_hdg_1d"
 
 
-def rows(path):
-    with Path(path).open(newline="") as stream:
-        return list(csv.DictReader(stream))
+class ProtocolFailure(RuntimeError):
+    """A solver invocation failed, or its output broke an acceptance rule."""
 
 
-def write_csv(path, data):
-    if not data:
+# ----------------------------------------------------------------- records
+@dataclass
+class Case:
+    """One solver invocation: the variables that name it and its flags."""
+    label: str
+    variables: dict
+    flags: list
+
+
+@dataclass
+class Outcome:
+    """The files one case wrote, parsed, and the resources it used."""
+    case: Case
+    summary: list
+    iterations: list
+    difference: dict | None
+    usage: dict
+
+
+@dataclass
+class Rule:
+    """An acceptance rule: `holds(outcome)` must be true for every case."""
+    text: str
+    holds: Callable[[Outcome], bool]
+
+
+@dataclass
+class Protocol:
+    dimension: int                                  # 2 -> bin/dg_hdg_2d, 1 -> bin/dg_hdg_1d
+    cases: Callable[[dict], list]                   # options -> [Case]
+    rules: list = field(default_factory=list)
+    reports: Callable[[list, dict], dict] | None = None  # outcomes so far -> {file name: rows}
+    success: str = ""
+
+
+def _read_csv(path: Path):
+    if not path.exists():
+        return []
+    with path.open(newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _write_csv(path: Path, table):
+    if not table:
         return
-    with Path(path).open("w", newline="") as stream:
-        writer = csv.DictWriter(stream, fieldnames=list(data[0]))
-        writer.writeheader()
-        writer.writerows(data)
+    columns = []
+    for row in table:  # union of the columns, in first-seen order
+        columns += [c for c in row if c not in columns]
+    with path.open("w", newline="") as f:
+        out = csv.DictWriter(f, fieldnames=columns, restval="")
+        out.writeheader()
+        out.writerows(table)
 
 
-def run(command, prefix):
-    """Run one solver invocation, log to PREFIX.log, return wall seconds."""
-    command = [str(part) for part in command]
-    start = time.perf_counter()
-    with Path(f"{prefix}.log").open("w") as log:
-        proc = subprocess.run(command, stdout=log, stderr=subprocess.STDOUT)
-    if proc.returncode:
-        raise RuntimeError(f"Command failed: {' '.join(command)}; see {prefix}.log")
-    return {"wall_s": time.perf_counter() - start}
+def _invoke(exe: Path, case: Case, prefix: Path) -> dict:
+    """Runs the solver with its output in PREFIX.log; returns the child's wall
+    time, CPU times and peak resident set (from wait4, this child only)."""
+    argv = [str(exe)] + [str(a) for a in case.flags] + ["--prefix", str(prefix)]
+    t0 = time.perf_counter()
+    with open(f"{prefix}.log", "w") as log:
+        child = subprocess.Popen(argv, stdout=log, stderr=subprocess.STDOUT)
+        _, status, ru = os.wait4(child.pid, 0)
+    wall = time.perf_counter() - t0
+    code = os.waitstatus_to_exitcode(status)
+    if code != 0:
+        raise ProtocolFailure(f"{case.label}: solver exited with status {code} ({' '.join(argv)}); "
+                              f"its output is in {prefix}.log")
+    return {"wall_s": wall, "user_cpu_s": ru.ru_utime, "system_cpu_s": ru.ru_stime, "peak_rss_kib": ru.ru_maxrss}
 
 
-def require(ok, message):
-    if not ok:
-        raise RuntimeError(message)
+def execute(name: str, output: Path, exe: Path, **options):
+    """Runs every case of protocol `name` in order, checks each against the
+    protocol's rules as soon as it finishes, rewrites the reports after each
+    case, and writes SUCCESS at the end. Returns the final reports."""
+    proto = PROTOCOLS[name]
+    output = Path(output)
+    output.mkdir(parents=True, exist_ok=True)
+    outcomes, reports = [], {}
+    for case in proto.cases(options):
+        prefix = output / case.label
+        print(f"[{name}] {case.label}: start", flush=True)
+        usage = _invoke(exe, case, prefix)
+        diff = _read_csv(Path(f"{prefix}_difference.csv"))
+        got = Outcome(case, _read_csv(Path(f"{prefix}_summary.csv")), _read_csv(Path(f"{prefix}_iterations.csv")),
+                      diff[0] if diff else None, usage)
+        for rule in proto.rules:
+            try:
+                ok = bool(rule.holds(got))
+            except (KeyError, ValueError, IndexError, TypeError, ZeroDivisionError) as e:
+                ok, rule_text = False, f"{rule.text} (output unreadable: {e!r})"
+            else:
+                rule_text = rule.text
+            if not ok:
+                raise ProtocolFailure(f"{name}: case {case.label} fails: {rule_text}")
+        outcomes.append(got)
+        if proto.reports is not None:
+            reports = proto.reports(outcomes, options)
+            for fname, table in reports.items():
+                _write_csv(output / fname, table)
+        print(f"[{name}] {case.label}: passed ({usage['wall_s']:.2f} s)", flush=True)
+    (output / "SUCCESS").write_text(proto.success + "\n")
+    return reports
 
 
-def check(output: Path, exe: Path):
-    for name, extra in [("dirk3", ["--T", .02]), ("be", ["--integrator", "be", "--T", .02]),
-                        ("entropy", ["--testcase", "wave", "--T", .02])]:
-        prefix = output / name
-        run([exe, "--device", "gpu", "--nx", 4, "--ny", 4, "--k", 2, "--cfl", 2, "--method", "both",
-             "--check-update", "--ntol", "1e-12", "--ltol", "1e-12", "--no-save-solution", *extra,
-             "--prefix", prefix], prefix)
-        diff = rows(f"{prefix}_difference.csv")[0]
-        require(float(diff["max_nodal"]) <= 1e-9, f"{name}: DG/HDG discrepancy")
-        for summary in rows(f"{prefix}_summary.csv"):
-            require(float(summary["max_conservation_drift"]) <= 1e-10, f"{name}: conservation")
-        print(f"{name} passed", flush=True)
-    (output / "SUCCESS").write_text("All production-path 2D checks passed.\n")
+# ------------------------------------------------------- shared pieces
+def _num(x) -> float:
+    return float(x)
 
 
-def accuracy(output: Path, exe: Path, grids):
-    require(all(n >= 3 for n in grids) and grids == sorted(set(grids)), "Grids must be distinct and increasing, N>=3")
-    records, previous = [], {}
-    for n in grids:
-        prefix = output / f"n{n}"
-        print(f"Start N={n}, Q2, DIRK3, CFL=2, T=0.2", flush=True)
-        run([exe, "--device", "gpu", "--nx", n, "--ny", n, "--k", 2, "--integrator", "ssp-dirk3", "--cfl", 2,
-             "--T", .2, "--method", "both", "--precond", "block-jacobi", "--ordering", "natural", "--ntol", "1e-12",
-             "--ltol", "1e-12", "--restart", 20, "--max-linear", 1500, "--max-newton", 30, "--safety", 1.1,
-             "--no-save-solution", "--prefix", prefix], prefix)
-        difference = rows(f"{prefix}_difference.csv")[0]
-        summaries = rows(f"{prefix}_summary.csv")
-        require([r["method"] for r in summaries] == ["dg", "hdg"], "Missing paired results")
-        for data in summaries:
-            require(float(data["final_time"]) == .2 and data["integrator"] == "ssp-dirk3"
-                    and int(data["fixed_newton_steps"]) == 0, "Incorrect convergence protocol")
-            method = data["method"]
-            for component in ["rho", "momentum_x", "momentum_y", "energy"]:
-                error = float(data[f"{component}_L2"])
-                require(math.isfinite(error) and error > 0, "Invalid error")
-                key = (method, component)
-                data[f"{component}_order"] = (math.log(previous[key][1] / error) / math.log(n / previous[key][0])
-                                              if key in previous else "")
-                previous[key] = (n, error)
-                data[f"difference_{component}_L2"] = difference[f"{component}_L2"]
-            data["difference_joint_L2"] = difference["joint_L2"]
-            data["difference_max_nodal"] = difference["max_nodal"]
-            records.append(data)
-        write_csv(output / "convergence.csv", records)
-        print(f"N={n} completed", flush=True)
-    (output / "SUCCESS").write_text("All requested paired 2D grids completed.\n")
-    return records
+def _order(coarse, fine):
+    """Observed order between two (cells per direction, error) samples."""
+    (n0, e0), (n1, e1) = coarse, fine
+    return math.log(e0 / e1) / math.log(n1 / n0)
 
 
-def accuracy1d(output: Path, exe: Path, grids):
-    require(all(n >= 3 for n in grids) and grids == sorted(set(grids)), "Grids must be distinct and increasing, N>=3")
-    records, previous = [], {}
-    for cells in grids:
-        prefix = output / f"n{cells}"
-        print(f"Start N={cells}, k=2, DIRK3, CFL=2, T=0.2", flush=True)
-        run([exe, "--cells", cells, "--k", 2, "--integrator", "ssp-dirk3", "--cfl", 2, "--T", .2, "--method", "both",
-             "--precond", "block-jacobi", "--ordering", "natural", "--ntol", "1e-12", "--ltol", "1e-12",
-             "--restart", 20, "--max-linear", 1500, "--max-newton", 30, "--safety", 1.1, "--prefix", prefix], prefix)
-        difference = rows(f"{prefix}_difference.csv")[0]
-        summaries = rows(f"{prefix}_summary.csv")
-        require([r["method"] for r in summaries] == ["dg", "hdg"], "Missing paired solver results")
-        for data in summaries:
-            method = data["method"]
-            require(float(data["final_time"]) == .2 and data["integrator"] == "ssp-dirk3"
-                    and int(data["fixed_newton_steps"]) == 0, "Unexpected convergence protocol")
-            for component in ("rho", "momentum", "energy"):
-                error = float(data[f"{component}_L2"])
-                require(math.isfinite(error) and error > 0, "Nonfinite/zero convergence error")
-                key = method, component
-                data[f"{component}_order"] = math.log2(previous[key] / error) if key in previous else ""
-                previous[key] = error
-                data[f"difference_{component}_L2"] = difference[f"{component}_L2"]
-            data["difference_joint_L2"] = difference["joint_L2"]
-            data["difference_max_nodal"] = difference["max_nodal"]
-            records.append(data)
-        write_csv(output / "convergence.csv", records)
-        print(f"N={cells} completed", flush=True)
-    (output / "SUCCESS").write_text("All requested paired 1D grids completed.\n")
-    return records
+def _convergence_table(outcomes, components):
+    """Summary rows of every grid, DG then HDG, with the observed order of each
+    component against the previous grid of the same method and the DG/HDG
+    difference columns of that grid (the schema of the reference's
+    convergence.csv)."""
+    history = {}  # (method, component) -> [(n, error)]
+    table = []
+    for o in outcomes:
+        n = o.case.variables["n"]
+        for row in o.summary:
+            row = dict(row)
+            for c in components:
+                series = history.setdefault((row["method"], c), [])
+                err = _num(row[f"{c}_L2"])
+                row[f"{c}_order"] = _order(series[-1], (n, err)) if series else ""
+                series.append((n, err))
+                row[f"difference_{c}_L2"] = o.difference[f"{c}_L2"]
+            row["difference_joint_L2"] = o.difference["joint_L2"]
+            row["difference_max_nodal"] = o.difference["max_nodal"]
+            table.append(row)
+    return table
 
 
-def speed(output: Path, exe: Path, cells, degrees):
-    require(cells >= 3 and len(set(degrees)) == len(degrees) and all(k in range(9) for k in degrees),
-            "Invalid mesh or degree list")
-    records, ratios = [], []
-    for k in degrees:
-        pair = {}
-        for method in ["dg", "hdg"]:
-            prefix = output / f"k{k}_{method}"
-            print(f"Start {method}: N={cells}x{cells}, k={k}, exact block Jacobi", flush=True)
-            resources = run([exe, "--device", "gpu", "--nx", cells, "--ny", cells, "--k", k, "--method", method,
-                             "--integrator", "be", "--steps", 1, "--newton-steps", 4, "--warmup-updates", 1,
-                             "--cfl", 5, "--precond", "block-jacobi", "--ordering", "natural", "--ntol", "1e-12",
-                             "--ltol", "1e-12", "--restart", 20, "--max-linear", 1500, "--safety", 1.1,
-                             "--timing-only", "--prefix", prefix], prefix)
-            summary = rows(f"{prefix}_summary.csv")[0]
-            updates = rows(f"{prefix}_iterations.csv")
-            require(len(updates) == 4 and [r["measured"] for r in updates] == ["0", "1", "1", "1"]
-                    and int(summary["measured_updates"]) == 3, "Expected exactly four updates, first warmup")
-            require(int(summary["steps"]) == 1 and int(summary["newton_iterations"]) == 4
-                    and int(summary["fixed_newton_steps"]) == 4 and summary["integrator"] == "be",
-                    "Expected four BE updates")
-            mean = sum(float(r["iteration_s"]) for r in updates[1:]) / 3
-            require(math.isfinite(mean) and mean > 0 and math.isclose(mean, float(summary["mean_newton_s"]),
-                                                                          rel_tol=1e-13), "Incorrect mean Newton time")
-            summary.update(resources)
-            summary["warmup_s"] = updates[0]["iteration_s"]
-            records.append(summary)
-            pair[method] = mean
-            write_csv(output / "timings.csv", records)
-            print(f"Completed {method} k={k}: mean={mean:.8g} s", flush=True)
-        ratios.append(dict(k=k, nx=cells, ny=cells, preconditioner="block-jacobi", dg_jacobi_local="dense",
-                           hdg_jacobi_local="face-dense", integrator="be", cfl=5, updates=4, warmup=1,
-                           measured_updates=3, dg_mean_s=pair["dg"], hdg_mean_s=pair["hdg"],
-                           dg_over_hdg=pair["dg"] / pair["hdg"]))
-        write_csv(output / "speedup.csv", ratios)
-    (output / "SUCCESS").write_text("All requested sequential 2D timing pairs completed.\n")
-    return records, ratios
+def _paired_convergence_rules(components):
+    def errors_positive(o):
+        return all(math.isfinite(_num(r[f"{c}_L2"])) and _num(r[f"{c}_L2"]) > 0 for r in o.summary for c in components)
+
+    return [
+        Rule("the summary holds one DG row then one HDG row", lambda o: [r["method"] for r in o.summary] == ["dg", "hdg"]),
+        Rule("each row ran DIRK3 to T = 0.2 without fixed Newton counts",
+             lambda o: all(_num(r["final_time"]) == 0.2 and r["integrator"] == "ssp-dirk3"
+                           and int(r["fixed_newton_steps"]) == 0 for r in o.summary)),
+        Rule("every L2 error is finite and positive", errors_positive),
+        Rule("the run wrote its DG/HDG difference file", lambda o: o.difference is not None),
+    ]
+
+
+def _grids(options, default):
+    grids = list(options.get("grids") or default)
+    if any(n < 3 for n in grids) or grids != sorted(set(grids)):
+        raise ProtocolFailure(f"grids must be increasing, without repeats, each at least 3: {grids}")
+    return grids
+
+
+_SOLVE = ["--ntol", "1e-12", "--ltol", "1e-12", "--precond", "block-jacobi", "--ordering", "natural"]
+_DIRK3_T02 = ["--integrator", "ssp-dirk3", "--cfl", 2, "--T", 0.2, "--method", "both", "--restart", 20,
+              "--max-linear", 1500, "--max-newton", 30, "--safety", 1.1]
+
+# ------------------------------------------------------------ check
+_CHECK_VARIANTS = {"dirk3": ["--T", 0.02], "be": ["--integrator", "be", "--T", 0.02],
+                   "entropy": ["--testcase", "wave", "--T", 0.02]}
+
+
+def _check_cases(options):
+    base = ["--device", "gpu", "--nx", 4, "--ny", 4, "--k", 2, "--cfl", 2, "--method", "both", "--check-update",
+            "--ntol", "1e-12", "--ltol", "1e-12", "--no-save-solution"]
+    return [Case(name, {"variant": name}, base + extra) for name, extra in _CHECK_VARIANTS.items()]
+
+
+# ---------------------------------------------------------- accuracy
+_COMPONENTS_2D = ("rho", "momentum_x", "momentum_y", "energy")
+_COMPONENTS_1D = ("rho", "momentum", "energy")
+
+
+def _accuracy_cases(options):
+    return [Case(f"n{n}", {"n": n}, ["--device", "gpu", "--nx", n, "--ny", n, "--k", 2] + _DIRK3_T02 + _SOLVE
+                 + ["--no-save-solution"]) for n in _grids(options, [32, 64, 128, 256])]
+
+
+def _accuracy1d_cases(options):
+    return [Case(f"n{n}", {"n": n}, ["--cells", n, "--k", 2] + _DIRK3_T02 + _SOLVE)
+            for n in _grids(options, [32, 64, 128, 256, 512])]
+
+
+# ------------------------------------------------------------- speed
+def _speed_cases(options):
+    cells, degrees = options.get("cells", 256), list(options.get("degrees") or range(8, -1, -1))
+    if cells < 3 or len(set(degrees)) != len(degrees) or not all(0 <= k <= 8 for k in degrees):
+        raise ProtocolFailure(f"speed: need cells >= 3 and distinct degrees in 0..8, got {cells}, {degrees}")
+    return [Case(f"k{k}_{method}", {"k": k, "method": method, "cells": cells},
+                 ["--device", "gpu", "--nx", cells, "--ny", cells, "--k", k, "--method", method, "--integrator", "be",
+                  "--steps", 1, "--newton-steps", 4, "--warmup-updates", 1, "--cfl", 5, "--restart", 20,
+                  "--max-linear", 1500, "--safety", 1.1, "--timing-only"] + _SOLVE)
+            for k in degrees for method in ("dg", "hdg")]
+
+
+def _measured_mean(o):
+    """Mean iteration time of the updates flagged as measured."""
+    t = [_num(r["iteration_s"]) for r in o.iterations if r["measured"] == "1"]
+    return sum(t) / len(t)
+
+
+_SPEED_RULES = [
+    Rule("four updates, the first flagged as warm-up and three measured",
+         lambda o: [r["measured"] for r in o.iterations] == ["0", "1", "1", "1"]
+         and int(o.summary[0]["measured_updates"]) == 3),
+    Rule("one BE step of exactly four fixed Newton updates",
+         lambda o: int(o.summary[0]["steps"]) == 1 and int(o.summary[0]["newton_iterations"]) == 4
+         and int(o.summary[0]["fixed_newton_steps"]) == 4 and o.summary[0]["integrator"] == "be"),
+    Rule("the summary's mean_newton_s is the mean of the three measured iteration times",
+         lambda o: _measured_mean(o) > 0 and math.isfinite(_measured_mean(o))
+         and math.isclose(_measured_mean(o), _num(o.summary[0]["mean_newton_s"]), rel_tol=1e-13)),
+]
+
+
+def _speed_reports(outcomes, options):
+    timings, means = [], {}
+    for o in outcomes:
+        row = dict(o.summary[0])
+        row.update(o.usage)
+        row["warmup_s"] = o.iterations[0]["iteration_s"]
+        timings.append(row)
+        means[(o.case.variables["k"], o.case.variables["method"])] = _measured_mean(o)
+    speedup = []
+    for o in outcomes:  # one row per degree whose DG and HDG cases have both run
+        k, cells = o.case.variables["k"], o.case.variables["cells"]
+        if o.case.variables["method"] != "hdg" or (k, "dg") not in means:
+            continue
+        dg, hdg = means[(k, "dg")], means[(k, "hdg")]
+        speedup.append({"k": k, "nx": cells, "ny": cells, "preconditioner": "block-jacobi",
+                        "dg_jacobi_local": "dense", "hdg_jacobi_local": "face-dense", "integrator": "be", "cfl": 5,
+                        "updates": 4, "warmup": 1, "measured_updates": 3, "dg_mean_s": dg, "hdg_mean_s": hdg,
+                        "dg_over_hdg": dg / hdg})
+    return {"timings.csv": timings, "speedup.csv": speedup}
+
+
+PROTOCOLS = {
+    "check": Protocol(
+        dimension=2, cases=_check_cases,
+        rules=[Rule("DG and HDG final states agree to 1e-9 nodally",
+                    lambda o: o.difference is not None and _num(o.difference["max_nodal"]) <= 1e-9),
+               Rule("conservation drift of every method <= 1e-10",
+                    lambda o: len(o.summary) > 0 and all(_num(r["max_conservation_drift"]) <= 1e-10
+                                                         for r in o.summary))],
+        success="2D production checks (DIRK3, BE, entropy wave; DG and HDG): all rules held."),
+    "accuracy": Protocol(
+        dimension=2, cases=_accuracy_cases, rules=_paired_convergence_rules(_COMPONENTS_2D),
+        reports=lambda outcomes, opt: {"convergence.csv": _convergence_table(outcomes, _COMPONENTS_2D)},
+        success="2D Q2 DIRK3 convergence sweep: every grid ran DG and HDG and passed."),
+    "accuracy1d": Protocol(
+        dimension=1, cases=_accuracy1d_cases, rules=_paired_convergence_rules(_COMPONENTS_1D),
+        reports=lambda outcomes, opt: {"convergence.csv": _convergence_table(outcomes, _COMPONENTS_1D)},
+        success="1D k=2 DIRK3 convergence sweep: every grid ran DG and HDG and passed."),
+    "speed": Protocol(
+        dimension=2, cases=_speed_cases, rules=_SPEED_RULES, reports=_speed_reports,
+        success="2D BE timing pairs (four updates, one warm-up): every degree ran DG and HDG and passed."),
+}
+
+
+# ------------------------------------------------- entry points by name
+def check(output: Path, exe: Path = EXE):
+    return execute("check", output, exe)
+
+
+def accuracy(output: Path, exe: Path = EXE, grids=None):
+    return execute("accuracy", output, exe, grids=grids)["convergence.csv"]
+
+
+def accuracy1d(output: Path, exe: Path = EXE1D, grids=None):
+    return execute("accuracy1d", output, exe, grids=grids)["convergence.csv"]
+
+
+def speed(output: Path, exe: Path = EXE, cells=256, degrees=None):
+    r = execute("speed", output, exe, cells=cells, degrees=degrees)
+    return r["timings.csv"], r["speedup.csv"]
 
 
 def main(argv=None):
-    parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    parser.add_argument("mode", choices=["check", "accuracy", "speed", "accuracy1d"])
-    parser.add_argument("--output", required=True)
-    parser.add_argument("--exe", default=None)
-    parser.add_argument("--grids", type=int, nargs="+", default=None)
-    parser.add_argument("--cells", type=int, default=256)
-    parser.add_argument("--degrees", type=int, nargs="+", default=list(range(8, -1, -1)))
-    args = parser.parse_args(argv)
-    exe = Path(args.exe or (EXE1D if args.mode == "accuracy1d" else EXE))
-    grids = args.grids or ([32, 64, 128, 256, 512] if args.mode == "accuracy1d" else [32, 64, 128, 256])
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("protocol", choices=sorted(PROTOCOLS))
+    ap.add_argument("--output", required=True)
+    ap.add_argument("--exe", default=None, help="solver executable (default: the in-tree build)")
+    ap.add_argument("--grids", type=int, nargs="+", default=None)
+    ap.add_argument("--cells", type=int, default=256)
+    ap.add_argument("--degrees", type=int, nargs="+", default=None)
+    a = ap.parse_args(argv)
+    exe = Path(a.exe) if a.exe else (EXE1D if PROTOCOLS[a.protocol].dimension == 1 else EXE)
     if not exe.is_file():
         raise SystemExit(f"{exe} is missing: build it with `python -m dg_hdg_euler_cpp_amd.build`")
-    output = Path(args.output)
-    output.mkdir(parents=True, exist_ok=True)
-    if args.mode == "check":
-        check(output, exe)
-    elif args.mode == "accuracy":
-        accuracy(output, exe, grids)
-    elif args.mode == "accuracy1d":
-        accuracy1d(output, exe, grids)
-    else:
-        speed(output, exe, args.cells, args.degrees)
+    execute(a.protocol, Path(a.output), exe, grids=a.grids, cells=a.cells, degrees=a.degrees)
     return 0
 
 

@@ -153,3 +153,38 @@ def test_1d_cli_reports_errors_like_the_reference():
     assert p.returncode == 1 and p.stderr.strip() == "ERROR: The 1D device path preconditions with block Jacobi"
     p = subprocess.run([exe, "--help"], capture_output=True, text=True)
     assert p.returncode == 0 and "--cells" in p.stdout
+
+
+def test_vector_pointer_checks_refuse_mismatched_buffers():
+    """The ctypes binding refuses, before any kernel runs, buffers that the
+    kernels would index out of bounds: host tensors, another dtype, another
+    length, non-contiguous views."""
+    import torch
+
+    from dg_hdg_euler_cpp_amd.device import DeviceError, _ptr
+
+    class FakeDeviceTensor:
+        is_cuda = True
+
+        def __init__(self, dtype, n, contiguous=True):
+            self.dtype, self.n, self.contiguous = dtype, n, contiguous
+
+        def numel(self):
+            return self.n
+
+        def is_contiguous(self):
+            return self.contiguous
+
+        def data_ptr(self):
+            return 4096
+
+    assert _ptr(None, torch.float64, 10) is None
+    assert _ptr(FakeDeviceTensor(torch.float64, 10), torch.float64, 10).value == 4096
+    with pytest.raises(DeviceError, match="HIP device"):
+        _ptr(torch.zeros(10, dtype=torch.float64), torch.float64, 10)
+    with pytest.raises(DeviceError, match="dtype"):
+        _ptr(FakeDeviceTensor(torch.float32, 10), torch.float64, 10)
+    with pytest.raises(DeviceError, match="9 entries, expected 10"):
+        _ptr(FakeDeviceTensor(torch.float64, 9), torch.float64, 10)
+    with pytest.raises(DeviceError, match="contiguous"):
+        _ptr(FakeDeviceTensor(torch.float64, 10, contiguous=False), torch.float64, 10)

@@ -18,6 +18,8 @@ CASES = [
     ("wave-k4-6x5", ["--nx", "6", "--ny", "5", "--k", "4", "--testcase", "wave"], None),
     ("seeded-k8-16x16", ["--nx", "16", "--ny", "16", "--k", "8", "--testcase", "seeded"], 2025),
     ("diagonal-k8-12x12", ["--nx", "12", "--ny", "12", "--k", "8", "--testcase", "diagonal"], None),
+    ("axis-k4-6x5", ["--nx", "6", "--ny", "5", "--k", "4", "--testcase", "axis"], None),
+    ("axis-k8-5x4", ["--nx", "5", "--ny", "4", "--k", "8", "--testcase", "axis"], None),
 ]
 
 
