@@ -51,7 +51,28 @@ struct Accumulator {
         flush();
         return sum + comp;
     }
+    // a whole block's sum, summed elsewhere in the same order add() would
+    void add_block(double block_sum) {
+        block = block_sum;
+        flush();
+    }
 };
+
+// The Accumulator's sum of term(0..count-1): each 256-term block summed in
+// order (one thread per block), the block sums compensated in block order.
+template <class Term> double compensated_sum(size_t count, Term term) {
+    const long long blocks = static_cast<long long>((count + 255) / 256);
+    Vec part(static_cast<size_t>(blocks));
+    parallel_for(blocks, [&](long long b) {
+        const size_t begin = static_cast<size_t>(b) * 256, end = std::min(count, begin + 256);
+        Accumulator s;  // one block: its result is the plain in-order sum
+        for (size_t i = begin; i < end; ++i) s.add(term(i));
+        part[static_cast<size_t>(b)] = s.result();
+    });
+    Accumulator acc;
+    for (double x : part) acc.add_block(x);
+    return acc.result();
+}
 
 // ----------------------------------------------------------------- config
 // dg_hdg_2d/src/config.cpp:6-22
@@ -505,46 +526,74 @@ void Problem::lift(int side, const double *values, double *nodal, double scale) 
     }
 }
 
+// The face loops of discretization.cpp:123-140 and dg_operator.cpp:10-21 visit
+// face = axis * cells + cell in order and lift each face value into its cell
+// (side 2 axis + 1, +1) and then into the +axis neighbour (side 2 axis, -1).
+// Per cell that is: the two x faces in face order, then the two y faces in
+// face order; gathering them per cell in that order gives the same sums.
+void Problem::lift_faces(const Vec &face_values, Vec &r) const {
+    const int m = layout.block_size(), fw = face_width(), C = layout.cells;
+    parallel_for(C, [&](long long c) {
+        const int cell = static_cast<int>(c);
+        double *nodal = r.data() + static_cast<size_t>(cell) * m;
+        for (int axis = 0; axis < NDIM; ++axis) {
+            const int own = axis * C + cell, other = axis * C + neighbor(cell, 2 * axis);
+            const double *fo = face_values.data() + static_cast<size_t>(own) * fw;
+            const double *fn = face_values.data() + static_cast<size_t>(other) * fw;
+            if (other < own) {
+                lift(2 * axis, fn, nodal, -1);
+                lift(2 * axis + 1, fo, nodal);
+            } else {
+                lift(2 * axis + 1, fo, nodal);
+                lift(2 * axis, fn, nodal, -1);
+            }
+        }
+    });
+}
 // discretization.cpp:111-122
 void Problem::volume_residual(const Vec &u, Vec &r) const {
     r.resize(u.size());
-    for (size_t i = 0; i < u.size(); ++i) r[i] = mass[i] * (u[i] - old[i]) / step_size;
+    parallel_for(static_cast<long long>(u.size()), [&](long long i) { r[i] = mass[i] * (u[i] - old[i]) / step_size; });
     if (basis.nodes == 1) return;
     const int P = basis.points * basis.points, m = layout.block_size();
-    Vec points(static_cast<size_t>(P) * NVAR), fx(points.size()), fy(points.size());
-    for (int cell = 0; cell < layout.cells; ++cell) {
-        tensor.interpolate(u.data() + static_cast<size_t>(cell) * m, points.data());
+    struct Scratch { Vec points, fx, fy; };
+    parallel_for(layout.cells, [&] { return Scratch{Vec(static_cast<size_t>(P) * NVAR), Vec(static_cast<size_t>(P) * NVAR), Vec(static_cast<size_t>(P) * NVAR)}; },
+                 [&](long long c, Scratch &w) {
+        const int cell = static_cast<int>(c);
+        tensor.interpolate(u.data() + static_cast<size_t>(cell) * m, w.points.data());
         for (int q = 0; q < P; ++q) {
-            State s = {points[q], points[P + q], points[2 * P + q], points[3 * P + q]};
+            State s = {w.points[q], w.points[P + q], w.points[2 * P + q], w.points[3 * P + q]};
             auto prim = example.gas.primitive(s);
             State a = example.gas.flux(s, prim, 0), b = example.gas.flux(s, prim, 1);
-            for (int v = 0; v < NVAR; ++v) { fx[v * P + q] = a[v]; fy[v * P + q] = b[v]; }
+            for (int v = 0; v < NVAR; ++v) { w.fx[v * P + q] = a[v]; w.fy[v * P + q] = b[v]; }
         }
-        tensor.subtract_weak(fx.data(), fy.data(), r.data() + static_cast<size_t>(cell) * m);
-    }
+        tensor.subtract_weak(w.fx.data(), w.fy.data(), r.data() + static_cast<size_t>(cell) * m);
+    });
 }
 // discretization.cpp:123-140
 void Problem::dg_residual(const Vec &u, Vec &r) const {
     volume_residual(u, r);
-    const int n = basis.nodes, m = layout.block_size();
-    Vec left(n * NVAR), right(n * NVAR), flux(n * NVAR);
-    for (int axis = 0; axis < NDIM; ++axis)
-        for (int cell = 0; cell < layout.cells; ++cell) {
-            int side = 2 * axis + 1, next = neighbor(cell, side), face = axis * layout.cells + cell;
-            face_values(u.data() + static_cast<size_t>(cell) * m, side, left.data());
-            face_values(u.data() + static_cast<size_t>(next) * m, side - 1, right.data());
-            for (int q = 0; q < n; ++q) {
-                State l, rr;
-                for (int v = 0; v < NVAR; ++v) { l[v] = left[v * n + q]; rr[v] = right[v * n + q]; }
-                auto s = speeds[static_cast<size_t>(face) * n + q];
-                State fl = example.gas.flux(l, axis), fr = example.gas.flux(rr, axis);
-                double scale = face_weight(face, q) / (s.a + s.b);
-                for (int v = 0; v < NVAR; ++v)
-                    flux[v * n + q] = scale * (s.b * fl[v] + s.a * fr[v] + s.a * s.b * (l[v] - rr[v]));
-            }
-            lift(side, flux.data(), r.data() + static_cast<size_t>(cell) * m);
-            lift(side - 1, flux.data(), r.data() + static_cast<size_t>(next) * m, -1);
+    const int n = basis.nodes, m = layout.block_size(), fw = face_width();
+    Vec fluxes(static_cast<size_t>(face_count()) * fw);
+    parallel_for(face_count(), [&] { return std::make_pair(Vec(n * NVAR), Vec(n * NVAR)); },
+                 [&](long long f, std::pair<Vec, Vec> &lr) {
+        const int face = static_cast<int>(f), axis = face / layout.cells, cell = face % layout.cells;
+        const int side = 2 * axis + 1, next = neighbor(cell, side);
+        Vec &left = lr.first, &right = lr.second;
+        double *flux = fluxes.data() + static_cast<size_t>(face) * fw;
+        face_values(u.data() + static_cast<size_t>(cell) * m, side, left.data());
+        face_values(u.data() + static_cast<size_t>(next) * m, side - 1, right.data());
+        for (int q = 0; q < n; ++q) {
+            State l, rr;
+            for (int v = 0; v < NVAR; ++v) { l[v] = left[v * n + q]; rr[v] = right[v * n + q]; }
+            auto s = speeds[static_cast<size_t>(face) * n + q];
+            State fl = example.gas.flux(l, axis), fr = example.gas.flux(rr, axis);
+            double scale = face_weight(face, q) / (s.a + s.b);
+            for (int v = 0; v < NVAR; ++v)
+                flux[v * n + q] = scale * (s.b * fl[v] + s.a * fr[v] + s.a * s.b * (l[v] - rr[v]));
         }
+    });
+    lift_faces(fluxes, r);
 }
 // discretization.cpp:141-162
 void Problem::hybrid_residual(const Vec &u, const Vec &lambda, Vec &r, Vec &g) const {
@@ -581,22 +630,23 @@ void Problem::hybrid_residual(const Vec &u, const Vec &lambda, Vec &r, Vec &g) c
 double Problem::freeze(const Vec &u) {
     old = u;
     const int n = basis.nodes, m = layout.block_size();
-    Vec left(n * NVAR), right(n * NVAR);
-    for (int axis = 0; axis < NDIM; ++axis)
-        for (int cell = 0; cell < layout.cells; ++cell) {
-            int side = 2 * axis + 1, next = neighbor(cell, side), face = axis * layout.cells + cell;
-            face_values(u.data() + static_cast<size_t>(cell) * m, side, left.data());
-            face_values(u.data() + static_cast<size_t>(next) * m, side - 1, right.data());
-            for (int q = 0; q < n; ++q) {
-                State l, r;
-                for (int v = 0; v < NVAR; ++v) { l[v] = left[v * n + q]; r[v] = right[v * n + q]; }
-                speeds[static_cast<size_t>(face) * n + q] = einfeldt(l, r, example.gas, config.safety, axis);
-            }
+    parallel_for(face_count(), [&] { return std::make_pair(Vec(n * NVAR), Vec(n * NVAR)); },
+                 [&](long long f, std::pair<Vec, Vec> &lr) {
+        const int face = static_cast<int>(f), axis = face / layout.cells, cell = face % layout.cells;
+        const int side = 2 * axis + 1, next = neighbor(cell, side);
+        Vec &left = lr.first, &right = lr.second;
+        face_values(u.data() + static_cast<size_t>(cell) * m, side, left.data());
+        face_values(u.data() + static_cast<size_t>(next) * m, side - 1, right.data());
+        for (int q = 0; q < n; ++q) {
+            State l, r;
+            for (int v = 0; v < NVAR; ++v) { l[v] = left[v * n + q]; r[v] = right[v * n + q]; }
+            speeds[static_cast<size_t>(face) * n + q] = einfeldt(l, r, example.gas, config.safety, axis);
         }
-    double rate = 0;
+    });
     const int P = basis.points * basis.points;
-    Vec points(static_cast<size_t>(P) * NVAR);
-    for (int cell = 0; cell < layout.cells; ++cell) {
+    Vec cell_rate(static_cast<size_t>(layout.cells));  // the max over cells below is exact in any order
+    parallel_for(layout.cells, [&] { return Vec(static_cast<size_t>(P) * NVAR); }, [&](long long c, Vec &points) {
+        const int cell = static_cast<int>(c);
         double alpha[2] = {0, 0};
         auto fs = faces(cell);
         for (int side = 0; side < NFACES; ++side)
@@ -613,8 +663,10 @@ double Problem::freeze(const Vec &u) {
             for (int axis = 0; axis < NDIM; ++axis)
                 alpha[axis] = std::max(alpha[axis], std::abs(prim.velocity[axis]) + sound);
         }
-        rate = std::max(rate, alpha[0] / hx + alpha[1] / hy);
-    }
+        cell_rate[static_cast<size_t>(cell)] = alpha[0] / hx + alpha[1] / hy;
+    });
+    double rate = 0;
+    for (double x : cell_rate) rate = std::max(rate, x);
     return rate;
 }
 // discretization.cpp:196-210 (trace projection Phi(U))
@@ -652,26 +704,27 @@ bool Problem::physical(const double *states, int rows) const {
 bool Problem::admissible(const Vec &u) const {
     if (static_cast<int>(u.size()) != layout.size()) return false;
     const int n = basis.nodes, m = layout.block_size(), P = basis.points * basis.points;
-    Vec points(static_cast<size_t>(P) * NVAR), values(n * NVAR);
-    for (int cell = 0; cell < layout.cells; ++cell) {
-        const double *nodal = u.data() + static_cast<size_t>(cell) * m;
-        if (!physical(nodal, n * n)) return false;
-        if (basis.points != basis.nodes) {
-            tensor.interpolate(nodal, points.data());
-            if (!physical(points.data(), P)) return false;
+    std::vector<char> ok(static_cast<size_t>(layout.cells), 1);
+    parallel_for(layout.cells, [&] { return std::make_pair(Vec(static_cast<size_t>(P) * NVAR), Vec(n * NVAR)); },
+                 [&](long long c, std::pair<Vec, Vec> &w) {
+        const double *nodal = u.data() + static_cast<size_t>(c) * m;
+        bool good = physical(nodal, n * n);
+        if (good && basis.points != basis.nodes) {
+            tensor.interpolate(nodal, w.first.data());
+            good = physical(w.first.data(), P);
         }
-        for (int side = 0; side < NFACES; ++side) {
-            face_values(nodal, side, values.data());
-            if (!physical(values.data(), n)) return false;
+        for (int side = 0; good && side < NFACES; ++side) {
+            face_values(nodal, side, w.second.data());
+            good = physical(w.second.data(), n);
         }
-    }
-    return true;
+        ok[static_cast<size_t>(c)] = good;
+    });
+    return std::all_of(ok.begin(), ok.end(), [](char x) { return x != 0; });
 }
 // discretization.cpp:234-236
 double Problem::residual_norm(const Vec &r) const {
-    Accumulator s;
-    for (size_t i = 0; i < r.size(); ++i) s.add(r[i] * r[i] * inverse_mass[i]);
-    return std::sqrt(s.result() / (example.lx * example.ly));
+    const double s = compensated_sum(r.size(), [&](size_t i) { return r[i] * r[i] * inverse_mass[i]; });
+    return std::sqrt(s / (example.lx * example.ly));
 }
 // discretization.cpp:237-242
 State Problem::integral(const Vec &u) const {
@@ -829,9 +882,9 @@ void VolumeJacobian::refresh(const Vec &state) {
     require(p_.step_size > 0, "Set the time step before refreshing the Jacobian");
     inverse_step = 1 / p_.step_size;
     const int P = p_.basis.points * p_.basis.points, m = p_.layout.block_size();
-    Vec points(static_cast<size_t>(P) * NVAR);
     if (p_.basis.nodes > 1)
-        for (int cell = 0; cell < p_.layout.cells; ++cell) {
+        parallel_for(p_.layout.cells, [&] { return Vec(static_cast<size_t>(P) * NVAR); }, [&](long long c, Vec &points) {
+            const int cell = static_cast<int>(c);
             p_.tensor.interpolate(state.data() + static_cast<size_t>(cell) * m, points.data());
             for (int q = 0; q < P; ++q) {
                 auto prim = p_.example.gas.primitive({points[q], points[P + q], points[2 * P + q], points[3 * P + q]});
@@ -839,7 +892,7 @@ void VolumeJacobian::refresh(const Vec &state) {
                 primitives(q, 3 * cell + 1) = prim.velocity[1];
                 primitives(q, 3 * cell + 2) = prim.enthalpy;
             }
-        }
+        });
     ready_ = true;
 }
 // volume_jacobian.cpp:43-58 (dF_x, dF_y from the cached (u, v, H))
@@ -864,15 +917,18 @@ void VolumeJacobian::point_action(int cell, const double *d, double *fx, double 
 void VolumeJacobian::apply(const Vec &direction, Vec &result) const {
     require(ready_, "Refresh the volume Jacobian before applying it");
     result.resize(direction.size());
-    for (size_t i = 0; i < direction.size(); ++i) result[i] = p_.mass[i] * direction[i] * inverse_step;
+    parallel_for(static_cast<long long>(direction.size()),
+                 [&](long long i) { result[i] = p_.mass[i] * direction[i] * inverse_step; });
     if (p_.basis.nodes == 1) return;
     const int P = p_.basis.points * p_.basis.points, m = p_.layout.block_size();
-    Vec points(static_cast<size_t>(P) * NVAR), fx(points.size()), fy(points.size());
-    for (int cell = 0; cell < p_.layout.cells; ++cell) {
-        p_.tensor.interpolate(direction.data() + static_cast<size_t>(cell) * m, points.data());
-        point_action(cell, points.data(), fx.data(), fy.data());
-        p_.tensor.subtract_weak(fx.data(), fy.data(), result.data() + static_cast<size_t>(cell) * m);
-    }
+    const size_t width = static_cast<size_t>(P) * NVAR;
+    parallel_for(p_.layout.cells, [&] { return Vec(3 * width); }, [&](long long c, Vec &w) {
+        const int cell = static_cast<int>(c);
+        double *points = w.data(), *fx = points + width, *fy = fx + width;
+        p_.tensor.interpolate(direction.data() + static_cast<size_t>(cell) * m, points);
+        point_action(cell, points, fx, fy);
+        p_.tensor.subtract_weak(fx, fy, result.data() + static_cast<size_t>(cell) * m);
+    });
 }
 // volume_jacobian.cpp:72-126 (explicit element volume block + M/tau)
 void VolumeJacobian::block(int cell, Mat &result) const {
@@ -952,10 +1008,11 @@ FaceJacobian::FaceJacobian(const Problem &p)
 // face_jacobian.cpp:16-37
 void FaceJacobian::refresh(const Vec &state) {
     const int n = p_.basis.nodes, m = p_.layout.block_size(), fw = p_.face_width();
-    Vec left(n * NVAR), right(n * NVAR);
-    for (int axis = 0; axis < NDIM; ++axis)
-        for (int cell = 0; cell < p_.layout.cells; ++cell) {
-            const int face = axis * p_.layout.cells + cell, side = 2 * axis + 1;
+    parallel_for(p_.face_count(), [&] { return std::make_pair(Vec(n * NVAR), Vec(n * NVAR)); },
+                 [&](long long f, std::pair<Vec, Vec> &lr) {
+            Vec &left = lr.first, &right = lr.second;
+            const int face = static_cast<int>(f), axis = face / p_.layout.cells, cell = face % p_.layout.cells;
+            const int side = 2 * axis + 1;
             p_.face_values(state.data() + static_cast<size_t>(cell) * m, side, left.data());
             p_.face_values(state.data() + static_cast<size_t>(p_.neighbor(cell, side)) * m, side - 1, right.data());
             for (int q = 0; q < n; ++q) {
@@ -975,7 +1032,7 @@ void FaceJacobian::refresh(const Vec &state) {
                 penalty(q, face) = weights(q, face) * s.a * s.b / (s.a + s.b);
                 for (int v = 0; v < NVAR; ++v) diagonal[static_cast<size_t>(face) * fw + v * n + q] = -weights(q, face) * (s.a + s.b);
             }
-        }
+        });
 }
 // face_jacobian.cpp:38-64 (normal_action on both sides, HLL blend + penalty)
 void FaceJacobian::hll_action(int face, const double *l, const double *r, double *flux) const {
@@ -1052,17 +1109,17 @@ void FaceJacobian::cell_c_many(int cell, const Mat &directions, int columns, Mat
 // dg_operator.cpp:10-21
 void DGOperator::apply(const Vec &direction, Vec &result) const {
     volume.apply(direction, result);
-    const int n = p_.basis.nodes, m = p_.layout.block_size();
-    Vec left(n * NVAR), right(n * NVAR), flux(n * NVAR);
-    for (int axis = 0; axis < NDIM; ++axis)
-        for (int cell = 0; cell < p_.layout.cells; ++cell) {
-            int side = 2 * axis + 1, next = p_.neighbor(cell, side), face = axis * p_.layout.cells + cell;
-            p_.face_values(direction.data() + static_cast<size_t>(cell) * m, side, left.data());
-            p_.face_values(direction.data() + static_cast<size_t>(next) * m, side - 1, right.data());
-            faces.hll_action(face, left.data(), right.data(), flux.data());
-            p_.lift(side, flux.data(), result.data() + static_cast<size_t>(cell) * m);
-            p_.lift(side - 1, flux.data(), result.data() + static_cast<size_t>(next) * m, -1);
-        }
+    const int n = p_.basis.nodes, m = p_.layout.block_size(), fw = p_.face_width();
+    Vec fluxes(static_cast<size_t>(p_.face_count()) * fw);
+    parallel_for(p_.face_count(), [&] { return std::make_pair(Vec(n * NVAR), Vec(n * NVAR)); },
+                 [&](long long f, std::pair<Vec, Vec> &lr) {
+        const int face = static_cast<int>(f), axis = face / p_.layout.cells, cell = face % p_.layout.cells;
+        const int side = 2 * axis + 1, next = p_.neighbor(cell, side);
+        p_.face_values(direction.data() + static_cast<size_t>(cell) * m, side, lr.first.data());
+        p_.face_values(direction.data() + static_cast<size_t>(next) * m, side - 1, lr.second.data());
+        faces.hll_action(face, lr.first.data(), lr.second.data(), fluxes.data() + static_cast<size_t>(face) * fw);
+    });
+    p_.lift_faces(fluxes, result);
 }
 // dg_operator.cpp:22-48
 void DGOperator::block(int row, int col, Mat &result) const {
@@ -1171,11 +1228,11 @@ void DenseLU::solve_multiple(const Mat &b, int cols, Mat &x) const {
 // block_jacobi.hpp:24-39
 void BlockJacobi::apply(const Vec &rhs, Vec &result) const {
     result.resize(rhs.size());
-    for (int i = 0; i < blocks_; ++i) {
+    parallel_for(blocks_, [&](long long i) {
         std::memcpy(result.data() + static_cast<size_t>(i) * width_, rhs.data() + static_cast<size_t>(i) * width_,
                     sizeof(double) * width_);
         lus_[i].solve_inplace(result.data() + static_cast<size_t>(i) * width_);
-    }
+    });
 }
 
 // ------------------------------------------------------------- block ILU(0)
@@ -1311,9 +1368,7 @@ GMRES::GMRES(int n, int restart, const Vec &weights)
     require(ok, "Invalid GMRES metric or dimensions");
 }
 double GMRES::dot(const Vec &a, const Vec &b) const {
-    Accumulator s;
-    for (int i = 0; i < n_; ++i) s.add(a[i] * weights_[i] * b[i]);
-    return s.result();
+    return compensated_sum(static_cast<size_t>(n_), [&](size_t i) { return a[i] * weights_[i] * b[i]; });
 }
 // gmres.cpp:17-86: restarted right-preconditioned GMRES, weighted CGS2, Givens
 LinearResult GMRES::solve(const Action &A, const Action &M, const Vec &rhs, Vec &x, int limit, double tolerance) {
@@ -1326,7 +1381,7 @@ LinearResult GMRES::solve(const Action &A, const Action &M, const Vec &rhs, Vec 
     while (iterations < limit) {
         const double beta = norm(residual_);
         if (beta <= tolerance * rhs_norm) return {iterations, beta / rhs_norm};
-        for (int i = 0; i < n_; ++i) basis_(i, 0) = residual_[i] / beta;
+        parallel_for(n_, [&](long long i) { basis_(static_cast<int>(i), 0) = residual_[i] / beta; });
         hessenberg_.zero();
         std::fill(rotated_rhs_.begin(), rotated_rhs_.end(), 0.0);
         rotated_rhs_[0] = beta;
@@ -1336,25 +1391,22 @@ LinearResult GMRES::solve(const Action &A, const Action &M, const Vec &rhs, Vec 
             M(column_, work_);
             A(work_, column_);
             for (int pass = 0; pass < 2; ++pass) {
-                for (int i = 0; i < n_; ++i) weighted[i] = weights_[i] * column_[i];
+                parallel_for(n_, [&](long long i) { weighted[i] = weights_[i] * column_[i]; });
                 for (int k = 0; k <= j; ++k) {
-                    Accumulator acc;
                     const double *v = basis_.col(k);
-                    for (int i = 0; i < n_; ++i) acc.add(v[i] * weighted[i]);
-                    const double s = acc.result();
+                    const double s = compensated_sum(static_cast<size_t>(n_), [&](size_t i) { return v[i] * weighted[i]; });
                     projection_[k] = s;
                     hessenberg_(k, j) += s;
                 }
-                for (int i = 0; i < n_; ++i) {
+                parallel_for(n_, [&](long long i) {
                     double s = 0;
-                    for (int k = 0; k <= j; ++k) s += basis_(i, k) * projection_[k];
+                    for (int k = 0; k <= j; ++k) s += basis_(static_cast<int>(i), k) * projection_[k];
                     column_[i] -= s;
-                }
+                });
             }
             const double next = norm(column_);
             hessenberg_(j + 1, j) = next;
-            if (next > 0)
-                for (int i = 0; i < n_; ++i) basis_(i, j + 1) = column_[i] / next;
+            if (next > 0) parallel_for(n_, [&](long long i) { basis_(static_cast<int>(i), j + 1) = column_[i] / next; });
             for (int i = 0; i < j; ++i) {
                 double a = hessenberg_(i, j), b = hessenberg_(i + 1, j);
                 hessenberg_(i, j) = cosine_[i] * a + sine_[i] * b;
@@ -1378,15 +1430,15 @@ LinearResult GMRES::solve(const Action &A, const Action &M, const Vec &rhs, Vec 
             for (int j = i + 1; j < count; ++j) value -= hessenberg_(i, j) * coefficients_[j];
             coefficients_[i] = value / hessenberg_(i, i);
         }
-        for (int i = 0; i < n_; ++i) {
+        parallel_for(n_, [&](long long i) {
             double s = 0;
-            for (int k = 0; k < count; ++k) s += basis_(i, k) * coefficients_[k];
+            for (int k = 0; k < count; ++k) s += basis_(static_cast<int>(i), k) * coefficients_[k];
             work_[i] = s;
-        }
+        });
         M(work_, column_);
-        for (int i = 0; i < n_; ++i) x[i] += column_[i];
+        parallel_for(n_, [&](long long i) { x[i] += column_[i]; });
         A(x, work_);
-        for (int i = 0; i < n_; ++i) residual_[i] = rhs[i] - work_[i];
+        parallel_for(n_, [&](long long i) { residual_[i] = rhs[i] - work_[i]; });
         const double relative = norm(residual_) / rhs_norm;
         require(std::isfinite(relative), "Nonfinite GMRES residual");
         if (relative <= tolerance) return {iterations, relative};
@@ -1592,14 +1644,10 @@ TimeIntegrator::TimeIntegrator(Problem &p, Method method) : p_(p), method_(metho
 }
 // time_integrator.cpp:42-47
 double TimeIntegrator::dual_norm(const Vec &x) const {
-    Accumulator s;
-    for (size_t i = 0; i < x.size(); ++i) s.add(x[i] * x[i] * p_.inverse_mass[i]);
-    return std::sqrt(s.result());
+    return std::sqrt(compensated_sum(x.size(), [&](size_t i) { return x[i] * x[i] * p_.inverse_mass[i]; }));
 }
 double TimeIntegrator::state_norm(const Vec &x) const {
-    Accumulator s;
-    for (size_t i = 0; i < x.size(); ++i) s.add(x[i] * x[i] * p_.mass[i]);
-    return std::sqrt(s.result());
+    return std::sqrt(compensated_sum(x.size(), [&](size_t i) { return x[i] * x[i] * p_.mass[i]; }));
 }
 
 namespace {
@@ -1759,7 +1807,7 @@ RunStatistics TimeIntegrator::advance(Vec &solution) {
                 double damping = 1.;
                 bool accepted = false;
                 for (int search = 0; search < 30; ++search) {
-                    for (size_t i = 0; i < N; ++i) trial[i] = state[i] + damping * direction[i];
+                    parallel_for(static_cast<long long>(N), [&](long long i) { trial[i] = state[i] + damping * direction[i]; });
                     if (p_.admissible(trial)) {
                         p_.dg_residual(trial, trial_residual);
                         if (dual_norm(trial_residual) <= (1 - 1e-4 * damping) * norm + allowance) {

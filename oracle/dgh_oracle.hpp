@@ -27,6 +27,42 @@ inline void require(bool condition, const char *message) {
     if (!condition) throw std::runtime_error(message);
 }
 
+// ---------------------------------------------------------------- threading
+// The CPU-baseline build of the oracle is compiled with -fopenmp; the checker
+// build is not, and then these loops run in index order. Both builds give
+// bit-identical results: threaded loops write disjoint entries (cells, faces,
+// blocks, vector entries); reductions keep fixed 256-term blocks, each summed
+// in order by one thread, with the block sums compensated in block order
+// (compensated_sum in dgh_oracle.cpp); face lifts are gathered per cell in the
+// order the serial face loop applies them (Problem::lift_faces). An exception
+// thrown for one index is rethrown after the loop (the lowest such index's
+// message when the loop runs in order).
+template <class MakeScratch, class Body>
+void parallel_for(long long count, MakeScratch make_scratch, Body body) {
+    bool failed = false;
+    std::string message;
+#pragma omp parallel
+    {
+        auto scratch = make_scratch();
+#pragma omp for schedule(static)
+        for (long long i = 0; i < count; ++i) {
+            try {
+                body(i, scratch);
+            } catch (const std::exception &e) {
+#pragma omp critical(oracle_parallel_error)
+                if (!failed) {
+                    failed = true;
+                    message = e.what();
+                }
+            }
+        }
+    }
+    if (failed) throw std::runtime_error(message);
+}
+template <class Body> void parallel_for(long long count, Body body) {
+    parallel_for(count, [] { return 0; }, [&](long long i, int &) { body(i); });
+}
+
 // Column-major dense matrix (types.hpp:13).
 struct Mat {
     int rows = 0, cols = 0;
@@ -133,6 +169,8 @@ class Problem {
     int face_node(int side, int tangent, int normal) const;
     void face_values(const double *nodal, int side, double *values) const;
     void lift(int side, const double *values, double *nodal, double scale = 1) const;
+    // r += the lifts of face values [face][face_width] into both cells of each face
+    void lift_faces(const Vec &face_values, Vec &r) const;
     void initial(Vec &u) const;
     void seeded(Vec &u, uint64_t seed) const;  // build addition: seeded admissible state
     double freeze(const Vec &u);
@@ -213,11 +251,10 @@ class BlockJacobi {  // block_jacobi.hpp:12-39
   public:
     BlockJacobi(int blocks, int width) : blocks_(blocks), width_(width), lus_(blocks) {}
     template <class Provider> void refresh(Provider &&provider) {
-        Mat block(width_, width_);
-        for (int i = 0; i < blocks_; ++i) {
-            provider(i, i, block);
+        parallel_for(blocks_, [&] { return Mat(width_, width_); }, [&](long long i, Mat &block) {
+            provider(static_cast<int>(i), static_cast<int>(i), block);
             require(lus_[i].factor(block), "Singular block Jacobi factor");
-        }
+        });
     }
     void apply(const Vec &rhs, Vec &result) const;
     const DenseLU &lu(int i) const { return lus_[i]; }

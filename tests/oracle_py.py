@@ -18,6 +18,15 @@ _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
 
 
+def select_library(path):
+    """Use another build of the oracle (the threaded one for the CPU baseline);
+    call before the first use in the process."""
+    global _LIB_PATH
+    if _lib is not None and os.path.abspath(path) != os.path.abspath(_LIB_PATH):
+        raise RuntimeError("the oracle library is already loaded in this process")
+    _LIB_PATH = os.path.abspath(path)
+
+
 def lib():
     global _lib
     if _lib is None:
