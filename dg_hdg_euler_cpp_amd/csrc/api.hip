@@ -99,6 +99,7 @@ struct dgh_ctx {
     DevBuf<float> factors_f32, ibasis, iw, it, is, ires, ib, zbasis;
     DevBuf<unsigned short> factors_bf16;  // 2-byte factors (BF16 or FP16) in the pair layout
     DevBuf<float> factor_scale;           // FP16: 2^-e per block (U' = sigma U)
+    DevBuf<float> factor_rdiag;           // 2-byte factors: FP32 1 / (sigma U_ii) per row
     DevBuf<int> rowsrc32;
     DevBuf<double> check_dir;
     bool bj32_ready = false;
@@ -195,6 +196,7 @@ struct dgh_ctx {
         for (auto *b : fbufs) b->release(bytes);
         factors_bf16.release(bytes);
         factor_scale.release(bytes);
+        factor_rdiag.release(bytes);
         rowsrc32.release(bytes);
         face_rowsrc.release(bytes);
         factors32.release(bytes);
@@ -814,10 +816,13 @@ struct dgh_ctx {
         if (two_byte_factors()) {
             factors_bf16.alloc(static_cast<size_t>(cells) * pair_factor_stride(m), bytes);
             factor_scale.alloc(cells, bytes);
+            factor_rdiag.alloc(static_cast<size_t>(cells) * m, bytes);
+            set_flag(2, 0);
             timed("pack", [&] {
-                HIPCHECK(launch_to_pairs(m, factors_f32.p, factors_bf16.p, factor_scale.p, cells,
-                                         cfg.precision == DGH_PRECISION_MIXED_FP16, stream));
+                HIPCHECK(launch_to_pairs(m, factors_f32.p, factors_bf16.p, factor_scale.p, factor_rdiag.p, cells,
+                                         cfg.precision == DGH_PRECISION_MIXED_FP16, flags.p + 2, stream));
             });
+            require(read_flag(2) == 0, "Singular block Jacobi factor");
         } else {
             pack_blocks(m, factors_f32.p, cells);
         }
@@ -827,8 +832,8 @@ struct dgh_ctx {
         require(bj32_ready, "Refresh the block Jacobi preconditioner first");
         timed("bj_apply", [&] {
             if (two_byte_factors())
-                HIPCHECK(launch_bj_apply_pairs(m, factors_bf16.p, factor_scale.p, rowsrc32.p, r, z, cells,
-                                               cfg.precision == DGH_PRECISION_MIXED_FP16, stream));
+                HIPCHECK(launch_bj_apply_pairs(m, factors_bf16.p, factor_scale.p, factor_rdiag.p, rowsrc32.p, r, z,
+                                               cells, cfg.precision == DGH_PRECISION_MIXED_FP16, stream));
             else
                 HIPCHECK(launch_bj_apply(m, factors_f32.p, rowsrc32.p, r, z, cells, stream));
         });
@@ -1042,6 +1047,123 @@ struct dgh_ctx {
         throw std::runtime_error("GMRES iteration limit reached");
     }
 
+    // One damped Newton update of the current stage (time_integrator.cpp:104-179):
+    // state.p holds the iterate and residual.p its residual, whose dual norm is
+    // `norm`; tau, old and the frozen speeds are the stage's, old_norm = ||old||_M.
+    // Solves for the direction (DG or condensed HDG; FP64, or mixed precision
+    // inside FP64 FGMRES), checks the linear residual against the reference's
+    // target, and line-searches along the direction; the accepted iterate is
+    // left in state.p and its residual in residual.p.
+    struct UpdateResult {
+        Linear linear;
+        int outer;
+        double precision_error, error_norm, damping;
+    };
+    UpdateResult newton_update(double norm, double old_norm) {
+        const auto &c = cfg;
+        const double allowance = 128 * std::numeric_limits<double>::epsilon() *
+                                 ((state_norm(state.p) + old_norm) / tau + norm);
+        const double target = std::max(10 * c.linear_tolerance * norm, allowance);
+        Linear linear{0, 0};
+        int outer = 0;
+        double precision_error = -1;
+        if (c.method == DGH_METHOD_HDG) {
+            // time_integrator.cpp:164-191: projected condensation, face
+            // block Jacobi, Schur GMRES with up to four tolerance cuts,
+            // acceptance on the recovered DG residual
+            hdg_refresh_projected(state.p, residual.p);
+            hdg_bj_refresh();
+            trace_dir.alloc(static_cast<size_t>(T), bytes);
+            hlambda.alloc(static_cast<size_t>(T), bytes);
+            double tolerance = c.linear_tolerance;
+            bool valid = false;
+            for (int attempt = 0; attempt < 4; ++attempt) {
+                const Linear current = hdg_gmres(hrhs.p, trace_dir.p, c.max_linear, tolerance);
+                linear.iterations += current.iterations;
+                linear.relative = current.relative;
+                hdg_recover(trace_dir.p, direction.p);
+                axpby(-1.0, kres.p, 0.0, nullptr, hlambda.p, T);
+                hdg_recovered_residual(hlambda.p, equation_error.p);
+                valid = dual_norm(equation_error.p) <= target;
+                if (valid) break;
+                tolerance *= .01;
+            }
+            require(valid, "Recovered DG residual criterion failed after Schur refinement");
+        } else {
+            dg_refresh(state.p);
+            axpby(-1.0, residual.p, 0.0, nullptr, rhs.p, D);
+            if (c.precision == DGH_PRECISION_FP64) {
+                precond_refresh();
+                linear = gmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance);
+            } else {
+                precond_refresh_mixed();
+                linear = fgmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance, outer);
+            }
+            axpby(-1.0, kres.p, 0.0, nullptr, equation_error.p, D);
+            if (c.precision != DGH_PRECISION_FP64 && c.check_update) {
+                // The same Newton system through the FP64 path. The two
+                // factor pools are never resident together (at 512^2, k=8
+                // they are 221 GB + 111 GB): the FP32/BF16 factors are
+                // released first and the FP64 ones after the check.
+                sync();
+                factors_f32.release(bytes);
+                factors_bf16.release(bytes);
+                factor_scale.release(bytes);
+                factor_rdiag.release(bytes);
+                ilu_Y32.release(bytes);
+                bj32_ready = false;
+                ilu32_ready = false;
+                check_dir.alloc(D, bytes);
+                precond_refresh();
+                gmres(rhs.p, check_dir.p, c.max_linear, c.linear_tolerance);
+                axpby(1.0, direction.p, -1.0, check_dir.p, kwork.p, D);
+                precision_error = state_norm(kwork.p) / state_norm(check_dir.p);
+                factors.release(bytes);
+                ilu_Y.release(bytes);
+                bj_ready = false;
+                ilu_ready = false;
+            }
+        }
+        const double error_norm = dual_norm(equation_error.p);
+        require(error_norm <= target, "DG Newton linear residual too large");
+        if (c.method == DGH_METHOD_HDG && c.check_update) {
+            // time_integrator.cpp:194-203: independent DG check of the recovered update
+            indep.alloc(D, bytes);
+            dg_apply(direction.p, indep.p);
+            axpby(1.0, indep.p, 1.0, residual.p, indep.p, D);
+            require(dual_norm(indep.p) <= std::max(4 * target, 1e-12),
+                    "Independent recovered DG equation check failed");
+            axpby(1.0, indep.p, -1.0, equation_error.p, indep.p, D);
+            require(dual_norm(indep.p) <= std::max(4 * allowance, 1e-12), "Recovered residual identity failed");
+        }
+        double damping = 1.;
+        bool accepted = false;
+        for (int search = 0; search < 30; ++search) {
+            axpby(1.0, state.p, damping, direction.p, trial.p, D);
+            if (admissible(trial.p)) {
+                residual_into(trial.p, trial_residual.p);
+                if (dual_norm(trial_residual.p) <= (1 - 1e-4 * damping) * norm + allowance) {
+                    accepted = true;
+                    break;
+                }
+            }
+            damping *= .5;
+        }
+        require(accepted, "Newton line search failed");
+        std::swap(state, trial);
+        std::swap(residual, trial_residual);
+        return {linear, outer, precision_error, error_norm, damping};
+    }
+    void newton_buffers() {
+        state.alloc(D, bytes);
+        trial.alloc(D, bytes);
+        residual.alloc(D, bytes);
+        trial_residual.alloc(D, bytes);
+        direction.alloc(D, bytes);
+        rhs.alloc(D, bytes);
+        equation_error.alloc(D, bytes);
+    }
+
     // ---- TimeIntegrator::advance (time_integrator.cpp:48-213), DG method
     void advance(double *solution, dgh_run_stats &stats, std::vector<dgh_update_stats> &updates) {
         const auto &c = cfg;
@@ -1061,13 +1183,7 @@ struct dgh_ctx {
         std::memset(&stats, 0, sizeof(stats));
         freeze(solution);
         const auto initial_integral = integral(solution);
-        state.alloc(D, bytes);
-        trial.alloc(D, bytes);
-        residual.alloc(D, bytes);
-        trial_residual.alloc(D, bytes);
-        direction.alloc(D, bytes);
-        rhs.alloc(D, bytes);
-        equation_error.alloc(D, bytes);
+        newton_buffers();
         step_state.alloc(D, bytes);
         stage_base.alloc(D, bytes);
         if (stages > 1) slopes.alloc(static_cast<size_t>(4) * D, bytes);
@@ -1128,96 +1244,11 @@ struct dgh_ctx {
                         break;
                     }
                     if (it == limit) break;
-                    const double allowance = 128 * std::numeric_limits<double>::epsilon() *
-                                             ((state_norm(state.p) + old_norm) / tau + norm);
-                    const double target = std::max(10 * c.linear_tolerance * norm, allowance);
-                    Linear linear{0, 0};
-                    int outer = 0;
-                    double precision_error = -1;
-                    if (c.method == DGH_METHOD_HDG) {
-                        // time_integrator.cpp:164-191: projected condensation, face
-                        // block Jacobi, Schur GMRES with up to four tolerance cuts,
-                        // acceptance on the recovered DG residual
-                        hdg_refresh_projected(state.p, residual.p);
-                        hdg_bj_refresh();
-                        trace_dir.alloc(static_cast<size_t>(T), bytes);
-                        hlambda.alloc(static_cast<size_t>(T), bytes);
-                        double tolerance = c.linear_tolerance;
-                        bool valid = false;
-                        for (int attempt = 0; attempt < 4; ++attempt) {
-                            const Linear current = hdg_gmres(hrhs.p, trace_dir.p, c.max_linear, tolerance);
-                            linear.iterations += current.iterations;
-                            linear.relative = current.relative;
-                            hdg_recover(trace_dir.p, direction.p);
-                            axpby(-1.0, kres.p, 0.0, nullptr, hlambda.p, T);
-                            hdg_recovered_residual(hlambda.p, equation_error.p);
-                            valid = dual_norm(equation_error.p) <= target;
-                            if (valid) break;
-                            tolerance *= .01;
-                        }
-                        require(valid, "Recovered DG residual criterion failed after Schur refinement");
-                    } else {
-                    dg_refresh(state.p);
-                    axpby(-1.0, residual.p, 0.0, nullptr, rhs.p, D);
-                    if (c.precision == DGH_PRECISION_FP64) {
-                        precond_refresh();
-                        linear = gmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance);
-                    } else {
-                        precond_refresh_mixed();
-                        linear = fgmres(rhs.p, direction.p, c.max_linear, c.linear_tolerance, outer);
-                    }
-                    axpby(-1.0, kres.p, 0.0, nullptr, equation_error.p, D);
-                    if (c.precision != DGH_PRECISION_FP64 && c.check_update) {
-                        // The same Newton system through the FP64 path. The two
-                        // factor pools are never resident together (at 512^2, k=8
-                        // they are 221 GB + 111 GB): the FP32/BF16 factors are
-                        // released first and the FP64 ones after the check.
-                        sync();
-                        factors_f32.release(bytes);
-                        factors_bf16.release(bytes);
-        factor_scale.release(bytes);
-                        ilu_Y32.release(bytes);
-                        bj32_ready = false;
-                        ilu32_ready = false;
-                        check_dir.alloc(D, bytes);
-                        precond_refresh();
-                        gmres(rhs.p, check_dir.p, c.max_linear, c.linear_tolerance);
-                        axpby(1.0, direction.p, -1.0, check_dir.p, kwork.p, D);
-                        precision_error = state_norm(kwork.p) / state_norm(check_dir.p);
-                        factors.release(bytes);
-                        ilu_Y.release(bytes);
-                        bj_ready = false;
-                        ilu_ready = false;
-                    }
-                    }
-                    const double error_norm = dual_norm(equation_error.p);
-                    require(error_norm <= target, "DG Newton linear residual too large");
-                    if (c.method == DGH_METHOD_HDG && c.check_update) {
-                        // time_integrator.cpp:194-203: independent DG check of the recovered update
-                        indep.alloc(D, bytes);
-                        dg_apply(direction.p, indep.p);
-                        axpby(1.0, indep.p, 1.0, residual.p, indep.p, D);
-                        require(dual_norm(indep.p) <= std::max(4 * target, 1e-12),
-                                "Independent recovered DG equation check failed");
-                        axpby(1.0, indep.p, -1.0, equation_error.p, indep.p, D);
-                        require(dual_norm(indep.p) <= std::max(4 * allowance, 1e-12), "Recovered residual identity failed");
-                    }
-                    double damping = 1.;
-                    bool accepted = false;
-                    for (int search = 0; search < 30; ++search) {
-                        axpby(1.0, state.p, damping, direction.p, trial.p, D);
-                        if (admissible(trial.p)) {
-                            residual_into(trial.p, trial_residual.p);
-                            if (dual_norm(trial_residual.p) <= (1 - 1e-4 * damping) * norm + allowance) {
-                                accepted = true;
-                                break;
-                            }
-                        }
-                        damping *= .5;
-                    }
-                    require(accepted, "Newton line search failed");
-                    std::swap(state, trial);
-                    std::swap(residual, trial_residual);
+                    const UpdateResult up = newton_update(norm, old_norm);
+                    const Linear &linear = up.linear;
+                    const int outer = up.outer;
+                    const double precision_error = up.precision_error, error_norm = up.error_norm,
+                                 damping = up.damping;
                     residual_ready = true;
                     sync();
                     const double duration = elapsed(update_start);
@@ -1703,6 +1734,36 @@ int dgh_advance(dgh_ctx *ctx, double *u, dgh_run_stats *run, dgh_update_stats *u
         if (updates)
             for (int k = 0; k < capacity && k < static_cast<int>(list.size()); ++k) updates[k] = list[k];
         if (count) *count = static_cast<int>(list.size());
+    });
+}
+int dgh_newton_update(dgh_ctx *ctx, double *u, dgh_update_stats *out) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        const auto &c = ctx->cfg;
+        require(c.method != DGH_METHOD_HDG || c.precision == DGH_PRECISION_FP64,
+                "Mixed precision is available for the DG method only");
+        require(ctx->tau > 0, "Set the time step before a Newton update");
+        ctx->newton_buffers();
+        const auto start = Clock::now();
+        ctx->copy(ctx->state.p, u, ctx->D);
+        ctx->residual_into(ctx->state.p, ctx->residual.p);
+        const double norm = ctx->dual_norm(ctx->residual.p), rms = norm / std::sqrt(ctx->geo.lx * ctx->geo.ly);
+        const auto up = ctx->newton_update(norm, ctx->state_norm(ctx->old.p));
+        ctx->copy(u, ctx->state.p, ctx->D);
+        ctx->sync();
+        if (out) {
+            dgh_update_stats s{};
+            s.krylov_iterations = s.inner_iterations = up.linear.iterations;
+            s.outer_iterations = up.outer;
+            s.dt = ctx->tau;
+            s.residual_physical_rms = rms;
+            s.damping = up.damping;
+            s.linear_relative_residual = up.linear.relative;
+            s.dg_linear_relative_residual = norm > 0 ? up.error_norm / norm : 0.;
+            s.iteration_s = elapsed(start);
+            s.precision_error = up.precision_error;
+            *out = s;
+        }
     });
 }
 int dgh_bj_refresh_transposed(dgh_ctx *ctx) {

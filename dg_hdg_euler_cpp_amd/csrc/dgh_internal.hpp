@@ -150,10 +150,10 @@ hipError_t launch_convert(const float *src, double *dst, long long count, hipStr
 // 2-byte factors in the pair layout (BF16, or FP16 with a per-block scale of U),
 // converted from the square column-major FP32 LU factors
 long long pair_factor_stride(int m);
-hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, int nblocks, int fp16,
-                           hipStream_t s);
-hipError_t launch_bj_apply_pairs(int m, const void *factors, const float *sigma, const int *rowsrc, const float *r,
-                                 float *z, int nblocks, int fp16, hipStream_t s);
+hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, float *rdiag, int nblocks, int fp16,
+                           int *flag, hipStream_t s);
+hipError_t launch_bj_apply_pairs(int m, const void *factors, const float *sigma, const float *rdiag,
+                                 const int *rowsrc, const float *r, float *z, int nblocks, int fp16, hipStream_t s);
 
 // GMRES building blocks over vectors of length `count` with the DG metric
 // inv_mass(node) (weights == nullptr) or a per-entry weight vector; dot

@@ -802,21 +802,29 @@ __device__ __forceinline__ float half_value(unsigned short u, bool fp16) {
 // unit L strictly below the diagonal, U on and above it) into the pair layout
 // in 2 bytes. FP16 scales U by sigma = 2^-e, e = floor(log2 max|U|), so that
 // the largest entry lies in [1, 2) (L is unscaled: |L_ij| <= 1 after partial
-// pivoting); BF16 keeps sigma = 1.
+// pivoting); BF16 keeps sigma = 1. The pivots are not taken from the 2-byte
+// copy: rdiag[j] = 1 / (sigma U_jj) in FP32, so that a small pivot keeps its
+// FP32 value instead of becoming a subnormal or zero 2-byte number; a
+// reciprocal that is not finite and nonzero raises bit 16 of *flag.
+// Each wave converts whole columns: every pair (4 bytes) of the layout is
+// written once, pads included, and every source entry is read once (the scale
+// pass reads the upper triangles first).
 __global__ __launch_bounds__(BLOCK) void k_to_pairs(const float *__restrict__ square, unsigned short *__restrict__ out,
-                                                    float *__restrict__ sigma, int M, int fp16) {
+                                                    float *__restrict__ sigma, float *__restrict__ rdiag, int M,
+                                                    int fp16, int *flag) {
     __shared__ float red[BLOCK / 64];
     __shared__ float scale;
-    const int mm = M * M;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const long long stride = pair_stride(M);
-    const float *src = square + blockIdx.x * static_cast<long long>(mm);
+    const float *src = square + blockIdx.x * static_cast<long long>(M) * M;
     unsigned short *dst = out + blockIdx.x * stride;
+    unsigned *dst32 = reinterpret_cast<unsigned *>(dst);
     float mx = 0;
     if (fp16)
-        for (int t = threadIdx.x; t < mm; t += BLOCK)
-            if (t % M <= t / M) mx = fmaxf(mx, fabsf(src[t]));
+        for (int c = wave; c < M; c += BLOCK / 64)
+            for (int r = lane; r <= c; r += 64) mx = fmaxf(mx, fabsf(src[static_cast<long long>(c) * M + r]));
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = mx;
+    if (lane == 0) red[wave] = mx;
     __syncthreads();
     if (threadIdx.x == 0) {
         float m = 0;
@@ -824,15 +832,37 @@ __global__ __launch_bounds__(BLOCK) void k_to_pairs(const float *__restrict__ sq
         scale = (fp16 && m > 0) ? ldexpf(1.0f, -ilogbf(m)) : 1.0f;
         sigma[blockIdx.x] = scale;
     }
-    for (long long t = threadIdx.x; t < stride; t += BLOCK) dst[t] = 0;
     __syncthreads();
     const float sc = scale;
-    for (int t = threadIdx.x; t < mm; t += BLOCK) {
-        const int c = t / M, row = t % M;
-        const float v = src[t];
-        if (row > c) dst[pair_lower(M, c) + (row - 2 * ((c + 1) / 2))] = half_bits(v, fp16);
-        else dst[pair_upper(M, c) + row] = half_bits(v * sc, fp16);
+    bool bad = false;
+    for (int j = threadIdx.x; j < M; j += BLOCK) {
+        const float rd = 1.0f / (sc * src[static_cast<long long>(j) * M + j]);
+        rdiag[blockIdx.x * static_cast<long long>(M) + j] = rd;
+        bad = bad || !(isfinite(rd) && rd != 0.f);
     }
+    if (bad) atomicOr(flag, 16);
+    for (int c = wave; c < M; c += BLOCK / 64) {
+        const float *col = src + static_cast<long long>(c) * M;
+        // lower part: rows s0 .. M-1 in pairs; row c itself (even c) is a zero
+        const int s0 = 2 * ((c + 1) / 2);
+        unsigned *lo = dst32 + pair_lower(M, c) / 2;
+        for (int p = lane; p < (M - s0) / 2; p += 64) {
+            const int row = s0 + 2 * p;
+            const unsigned a = row > c ? half_bits(col[row], fp16) : 0u;
+            const unsigned b = half_bits(col[row + 1], fp16);
+            lo[p] = a | (b << 16);
+        }
+        // upper part: rows 0 .. c scaled by sigma, padded with a zero to even length
+        unsigned *up = dst32 + pair_upper(M, c) / 2;
+        for (int p = lane; p < (c + 2) / 2; p += 64) {
+            const int row = 2 * p;
+            const unsigned a = half_bits(col[row] * sc, fp16);
+            const unsigned b = row + 1 <= c ? half_bits(col[row + 1] * sc, fp16) : 0u;
+            up[p] = a | (b << 16);
+        }
+    }
+    for (long long t = pair_lower(M, M) + pair_cumulative_upper(M - 1) + threadIdx.x; t < stride; t += BLOCK)
+        dst[t] = 0;
 }
 
 // Block Jacobi apply on the pair layout: lane l owns rows 128 a + 2 l and
@@ -843,6 +873,7 @@ __global__ __launch_bounds__(BLOCK) void k_to_pairs(const float *__restrict__ sq
 template <int M, bool FP16, int U>
 __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *__restrict__ factors,
                                                           const float *__restrict__ sigma,
+                                                          const float *__restrict__ rdiag,
                                                           const int *__restrict__ rowsrc, const float *__restrict__ r,
                                                           float *__restrict__ z, int nblocks) {
     static_assert(M % 2 == 0 && 128 % U == 0 && U % 2 == 0, "pair layout");
@@ -965,6 +996,15 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
 #pragma unroll
         for (int a = 0; a < NA; ++a) x[a][0] *= sc, x[a][1] *= sc;
     }
+    // FP32 reciprocal pivots of the lane's rows (1 / (sigma U_ii))
+    float rd[NA][2];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int row = 128 * a + 2 * lane + e;
+            rd[a][e] = row < M ? rdiag[blk * M + row] : 0.f;
+        }
 #pragma unroll
     for (int u = 0; u < U; ++u) load_upper(buf[u], M - 1 - u, NA - 1);
 #pragma unroll
@@ -980,11 +1020,9 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
                     const int j = ja * 128 + jl;
                     unsigned(&cur)[NA] = buf[(t0 + u) % U];
                     // jend is even, so the parity of jl is that of u + 1
-                    const unsigned dw = cur[ja];
-                    const float dv = half_value((u & 1) ? lo16(dw) : hi16(dw), FP16);
-                    const float diag = readlane_d(dv, jl >> 1);
-                    const float xj = readlane_d(x[ja][(u & 1) ? 0 : 1], jl >> 1) / diag;
-                    if (lane == (jl >> 1)) x[ja][(u & 1) ? 0 : 1] = xj;
+                    const int e = (u & 1) ? 0 : 1;
+                    const float xj = readlane_d(x[ja][e] * rd[ja][e], jl >> 1);
+                    if (lane == (jl >> 1)) x[ja][e] = xj;
 #pragma unroll
                     for (int a = 0; a <= ja; ++a) {
                         const int r0 = 128 * a + 2 * lane;
@@ -1461,9 +1499,10 @@ hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const
     })
 }
 long long pair_factor_stride(int m) { return pair_stride(m); }
-hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, int nblocks, int fp16,
-                           hipStream_t s) {
-    k_to_pairs<<<nblocks, BLOCK, 0, s>>>(square, static_cast<unsigned short *>(out), sigma, m, fp16);
+hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, float *rdiag, int nblocks, int fp16,
+                           int *flag, hipStream_t s) {
+    if (m % 2 != 0) return hipErrorInvalidValue;
+    k_to_pairs<<<nblocks, BLOCK, 0, s>>>(square, static_cast<unsigned short *>(out), sigma, rdiag, m, fp16, flag);
     return hipGetLastError();
 }
 // columns in flight per wave (measured at 256^2, k=8: FP16 2.67 / 2.77 / 3.18 ms
@@ -1474,14 +1513,14 @@ hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, 
 #ifndef DGH_BJ_RING_BF16
 #define DGH_BJ_RING_BF16 8
 #endif
-hipError_t launch_bj_apply_pairs(int m, const void *factors, const float *sigma, const int *rowsrc, const float *r,
-                                 float *z, int nblocks, int fp16, hipStream_t s) {
+hipError_t launch_bj_apply_pairs(int m, const void *factors, const float *sigma, const float *rdiag,
+                                 const int *rowsrc, const float *r, float *z, int nblocks, int fp16, hipStream_t s) {
     const auto *f = static_cast<const unsigned short *>(factors);
     DGH_DISPATCH_M(m, {
         if (fp16)
-            k_bj_apply_pairs<M, true, DGH_BJ_RING_FP16><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, sigma, rowsrc, r, z, nblocks);
+            k_bj_apply_pairs<M, true, DGH_BJ_RING_FP16><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, sigma, rdiag, rowsrc, r, z, nblocks);
         else
-            k_bj_apply_pairs<M, false, DGH_BJ_RING_BF16><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, sigma, rowsrc, r, z, nblocks);
+            k_bj_apply_pairs<M, false, DGH_BJ_RING_BF16><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, sigma, rdiag, rowsrc, r, z, nblocks);
     })
 }
 // In place, `chunk` blocks at a time through `scratch` (chunk * m*m entries):

@@ -354,6 +354,13 @@ class DeviceContext:
     def advance(self, u, capacity=100000):
         return self._run("dgh_advance", u, capacity)
 
+    def newton_update(self, u):
+        """One damped Newton update of the current stage (speeds frozen, step
+        set); u is overwritten with the accepted iterate."""
+        st = UpdateStats()
+        self._call("dgh_newton_update", self._v(u), C.byref(st))
+        return {f: getattr(st, f) for f, _ in UpdateStats._fields_}
+
     # ---- discrete adjoint
     def bj_refresh_transposed(self):
         self._call("dgh_bj_refresh_transposed")

@@ -230,6 +230,17 @@ int dgh_hdg_gmres(dgh_ctx *ctx, const double *rhs, double *x, int max_iterations
 int dgh_advance(dgh_ctx *ctx, double *u, dgh_run_stats *run, dgh_update_stats *updates, int capacity,
                 int *count);
 
+/* One damped Newton update of the current stage, the body of the Newton loop
+ * of TimeIntegrator::advance (time_integrator.cpp:92-179): with the speeds
+ * frozen (dgh_freeze) and the stage's step size and base state set
+ * (dgh_set_step), evaluates r(u), refreshes the Jacobian and preconditioner,
+ * solves for the direction to the reference's linear target (DG or condensed
+ * HDG, in the configured precision), line-searches with the admissibility
+ * check and overwrites u with the accepted iterate. out (may be NULL) gets
+ * the RMS of r(u) before the update, the Krylov count, damping, linear
+ * residuals and the update's wall time; step/stage/time fields are 0. */
+int dgh_newton_update(dgh_ctx *ctx, double *u, dgh_update_stats *out);
+
 /* ---- discrete adjoint (no reference counterpart) -------------------------- */
 /* The reference only linearises forward (Jv); these entry points are the
  * reverse of the same operators, built on dgh_dg_apply_transpose.

@@ -32,19 +32,25 @@ def mass_vector(orc):
 
 
 @pytest.mark.parametrize("precision,bound", [("mixed", 1e-5), ("mixed-bf16", 6e-2), ("mixed-fp16", 1e-2)])
-@pytest.mark.parametrize("k,nx", [(2, 8), (8, 12)])
-def test_fp32_pieces_and_fgmres(precision, bound, k, nx, torch_cuda):
+@pytest.mark.parametrize("k,nx,state,cfl", [(2, 8, "diagonal", 5.0), (8, 12, "diagonal", 5.0),
+                                            (8, 8, "seeded", 0.1), (4, 10, "seeded", 0.5)])
+def test_fp32_pieces_and_fgmres(precision, bound, k, nx, state, cfl, torch_cuda):
+    """FP32 Jv, the FP32 / 2-byte factor apply (backward error of every sampled
+    block solve at the storage precision) and FP64 FGMRES over FP32 inner
+    solves, on the smooth projected state and on seeded (node-level random)
+    states, whose blocks carry pivots over a wide range of magnitudes."""
     from dg_hdg_euler_cpp_amd import DeviceContext
 
-    args = ["--nx", str(nx), "--ny", str(nx), "--k", str(k), "--cfl", "5", "--integrator", "be"]
+    args = ["--nx", str(nx), "--ny", str(nx), "--k", str(k), "--cfl", str(cfl), "--integrator", "be",
+            "--testcase", state]
     print(f"\n[case] {precision} {args}", flush=True)
     orc = O.Oracle(args)
     dev = DeviceContext(args + ["--precision", precision, "--restart", "20"])
-    u_o = orc.initial()
+    u_o = orc.initial() if state != "seeded" else orc.seeded(2025)
     u = torch_cuda.tensor(u_o, device="cuda")
     rate = orc.freeze(u_o)
     dev.freeze(u)
-    tau = 5.0 / rate
+    tau = cfl / rate
     orc.set_step(tau, u_o)
     dev.set_step(tau, u)
     orc.dg_refresh(u_o)
@@ -65,6 +71,8 @@ def test_fp32_pieces_and_fgmres(precision, bound, k, nx, torch_cuda):
         zb = z32[row * m:(row + 1) * m].astype(np.float64)
         res = B @ zb - r[row * m:(row + 1) * m].astype(np.float32).astype(np.float64)
         worst = max(worst, np.linalg.norm(res) / (np.linalg.norm(B) * np.linalg.norm(zb)))
+    print(f"worst block backward error {worst:.3e} (bound {bound:.0e})", flush=True)
+    assert np.all(np.isfinite(z32))
     assert worst < bound, worst
     # FP64 FGMRES with FP32 inner solves reaches the reference's ltol on the
     # oracle's own FP64 system
@@ -125,12 +133,14 @@ def test_mixed_update_error_vs_fp64_256(precision, torch_cuda):
         assert 0 <= got["precision_error"] <= 1e-9, got["precision_error"]
 
 
-def test_mixed_512_config(torch_cuda):
+@pytest.mark.parametrize("precision", ["mixed", "mixed-fp16"])
+def test_mixed_512_config(precision, torch_cuda):
     """512^2, k=8 (the mixed-precision configuration): two BE Newton updates with
-    FP32 factors and the FP32 inner GMRES(30) inside FP64 FGMRES(20); the RMS
-    sequence of the reference's 512^2 run (k8_512_be) and, per update, the
-    direction's error against the FP64 path (FP32 factors released before the
-    FP64 factors are built: 111 GB, then 221 GB)."""
+    FP32 factors, stored in FP32 or in FP16 with a per-block scale, and the FP32
+    inner GMRES(30) inside FP64 FGMRES(20); the RMS sequence of the reference's
+    512^2 run (k8_512_be) and, per update, the direction's error against the
+    FP64 path (the 2-byte and FP32 factors are released before the FP64 factors
+    are built: 111 GB, then 221 GB)."""
     from dg_hdg_euler_cpp_amd import DeviceContext
 
     run = GOLD["runs"]["k8_512_be"]
@@ -138,8 +148,8 @@ def test_mixed_512_config(torch_cuda):
     args = ["--nx", "512", "--ny", "512", "--k", "8", "--method", "dg", "--integrator", "be", "--steps", "1",
             "--newton-steps", "2", "--cfl", "5", "--ntol", "1e-12", "--ltol", "1e-12", "--restart", "20",
             "--max-linear", "1500", "--safety", "1.1", "--timing-only"]
-    print(f"\n[case] 512 mixed {args}", flush=True)
-    with DeviceContext(args + ["--precision", "mixed", "--check-update"]) as dev:
+    print(f"\n[case] 512 {precision} {args}", flush=True)
+    with DeviceContext(args + ["--precision", precision, "--check-update"]) as dev:
         u = dev.initial()
         st = dev.advance(u)
     for it, got in enumerate(st["updates"]):
