@@ -6,6 +6,7 @@
 #include <hip/hip_fp16.h>
 
 #include <cfloat>
+#include <cstdlib>
 #include <cstdint>
 #include <type_traits>
 
@@ -342,103 +343,168 @@ __device__ __forceinline__ float readlane_d(float v, int lane) {
 //
 // Right-looking with NB-column panels. Almost every row of these blocks is
 // pivoted, so row interchanges are never applied as strided row swaps:
-//  * the panel (rows k0..M-1) is factorised in LDS;
+//  * the panel (rows k0..M-1) is factorised in LDS (lu_panel);
 //  * the panel's interchanges, the U12 triangular solve and the trailing update
-//    are one pass over each trailing column: a wavefront gathers the column's
-//    rows through the panel's row permutation, solves the NB-row U12 segment
-//    in registers (readlane broadcasts) and writes the updated column back;
+//    are one pass over each trailing column: the column's rows are gathered
+//    through the panel's row permutation, the NB-row U12 segment is solved in
+//    registers and the updated column is written back. k_lu does this one
+//    column per wavefront with vector FMAs and L21 read from LDS; k_lu_mfma
+//    does it for 16-column tiles with the A22 -= L21 U12 product on MFMA;
 //  * interchanges of later panels are applied to the finished L columns in one
-//    final gather pass.
+//    final gather pass (lu_later_interchanges).
+template <int M, int NB, typename T, int LT> struct LuShared {
+    T panel[NB * M];  // [c][r], rows k0..M-1 of the current panel
+    int perm[M];      // original row held by each logical row
+    int piv[M];       // row interchanged with row k at step k
+    int sigma[M];     // gather map of the current panel's interchanges
+    int tmp[M];
+    double red_val[LT / 64];
+    int red_idx[LT / 64];
+    int bad;
+};
+
+template <int M, int NB, typename T, int LT> __device__ void lu_begin(LuShared<M, NB, T, LT> &S) {
+    for (int r = threadIdx.x; r < M; r += LT) S.perm[r] = r;
+    if (threadIdx.x == 0) S.bad = 0;
+    __syncthreads();
+}
+
+// Panel columns k0..k0+nb-1 (rows k0..M-1, already in the current row order):
+// load into LDS, unblocked factorisation with partial pivoting, write back,
+// and the gather map sigma of the panel's interchanges. Ends synchronised.
+template <int M, int NB, typename T, int LT>
+__device__ void lu_panel(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int k0, int nb) {
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64, rows = M - k0;
+    const double tiny = sizeof(T) == 8 ? DBL_MIN : FLT_MIN;
+    T *panel = S.panel;
+    for (int idx = threadIdx.x; idx < nb * rows; idx += LT) {
+        const int c = idx / rows, r = k0 + idx % rows;
+        panel[c * M + r] = A[static_cast<long long>(k0 + c) * M + r];
+    }
+    __syncthreads();
+    for (int c = 0; c < nb; ++c) {
+        const int col = k0 + c;
+        double best = -1.0;
+        int bidx = M;
+        for (int r = col + threadIdx.x; r < M; r += LT) {
+            const double a = fabs(static_cast<double>(panel[c * M + r]));
+            if (a > best || (a == best && r < bidx)) { best = a; bidx = r; }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off, 64);
+            const int oi = __shfl_xor(bidx, off, 64);
+            if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
+        }
+        if (lane == 0) { S.red_val[wave] = best; S.red_idx[wave] = bidx; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double b = S.red_val[0];
+            int bi = S.red_idx[0];
+            for (int w = 1; w < LT / 64; ++w)
+                if (S.red_val[w] > b || (S.red_val[w] == b && S.red_idx[w] < bi)) { b = S.red_val[w]; bi = S.red_idx[w]; }
+            if (bi >= M) bi = col;  // all-NaN column: keep the diagonal, flagged below
+            S.piv[col] = bi;
+            if (bi != col) { const int t = S.perm[col]; S.perm[col] = S.perm[bi]; S.perm[bi] = t; }
+        }
+        __syncthreads();
+        const int p = S.piv[col];
+        if (p != col)
+            for (int cc = threadIdx.x; cc < nb; cc += LT) {
+                const T t = panel[cc * M + col];
+                panel[cc * M + col] = panel[cc * M + p];
+                panel[cc * M + p] = t;
+            }
+        __syncthreads();
+        const T pivot = panel[c * M + col];
+        if (pivot != T(0))
+            for (int r = col + 1 + threadIdx.x; r < M; r += LT) panel[c * M + r] /= pivot;
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < (nb - c - 1) * (M - col - 1); idx += LT) {
+            const int cc = c + 1 + idx / (M - col - 1), r = col + 1 + idx % (M - col - 1);
+            panel[cc * M + r] -= panel[c * M + r] * panel[cc * M + col];
+        }
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < nb * rows; idx += LT) {
+        const int c = idx / rows, r = k0 + idx % rows;
+        const T val = panel[c * M + r];
+        if (!isfinite(static_cast<double>(val)) || (r == k0 + c && !(fabs(static_cast<double>(val)) > tiny)))
+            S.bad = 1;
+        A[static_cast<long long>(k0 + c) * M + r] = val;
+    }
+    if (threadIdx.x == 0) {
+        for (int r = k0; r < M; ++r) S.sigma[r] = r;
+        for (int c = 0; c < nb; ++c) {
+            const int a = k0 + c, b = S.piv[a];
+            const int t = S.sigma[a]; S.sigma[a] = S.sigma[b]; S.sigma[b] = t;
+        }
+    }
+    __syncthreads();
+}
+
+// Interchanges of later panels applied to the finished L columns (tau = gather
+// map of all interchanges after the panel, built backwards), then rowsrc and
+// the singularity flag.
+template <int M, int NB, typename T, int LT>
+__device__ void lu_finish(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int *__restrict__ rowsrc, int *flag) {
+    constexpr int NA = (M + 63) / 64;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    int *sigma = S.sigma, *tmp = S.tmp;
+    for (int r = threadIdx.x; r < M; r += LT) sigma[r] = r;
+    __syncthreads();
+    const int npanels = (M + NB - 1) / NB;
+    for (int q = npanels - 1; q >= 0; --q) {
+        const int k0 = q * NB, nb = min(NB, M - k0), lo = k0 + nb;
+        if (q < npanels - 1)
+            for (int c = k0 + wave; c < k0 + nb; c += LT / 64) {
+                T *colp = A + static_cast<long long>(c) * M;
+                T x[NA];
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                    const int row = lo + a * 64 + lane;
+                    x[a] = row < M ? colp[sigma[row]] : T(0);
+                }
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                    const int row = lo + a * 64 + lane;
+                    if (row < M) colp[row] = x[a];
+                }
+            }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int r = k0; r < M; ++r) tmp[r] = r;
+            for (int c = 0; c < nb; ++c) {
+                const int a = k0 + c, b = S.piv[a];
+                const int t = tmp[a]; tmp[a] = tmp[b]; tmp[b] = t;
+            }
+        }
+        __syncthreads();
+        for (int r = k0 + threadIdx.x; r < M; r += LT) {
+            const int src = sigma[r];
+            sigma[r] = tmp[src];
+        }
+        __syncthreads();
+    }
+    for (int r = threadIdx.x; r < M; r += LT) rowsrc[static_cast<long long>(blockIdx.x) * M + r] = S.perm[r];
+    if (threadIdx.x == 0 && S.bad) atomicOr(flag, 8);
+}
+
 template <int M, int NB, typename T, int LT>
 __global__ __launch_bounds__(LT) void k_lu(T *__restrict__ blocks, int *__restrict__ rowsrc, int nblocks,
                                               int *flag) {
     constexpr int NA = (M + 63) / 64;  // row chunks per lane
-    __shared__ T panel[NB * M];         // [c][r], rows k0..M-1 of the current panel
-    __shared__ int perm[M];             // original row held by each logical row
-    __shared__ int piv[M];              // row interchanged with row k at step k
-    __shared__ int sigma[M];            // gather map of the current panel's interchanges
-    __shared__ int tmp[M];
-    __shared__ double red_val[LT / 64];
-    __shared__ int red_idx[LT / 64];
-    __shared__ int bad;
-    const int blk = blockIdx.x;
-    T *A = blocks + static_cast<long long>(blk) * M * M;
+    __shared__ LuShared<M, NB, T, LT> S;
+    T *A = blocks + static_cast<long long>(blockIdx.x) * M * M;
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    const double tiny = sizeof(T) == 8 ? DBL_MIN : FLT_MIN;
-    for (int r = threadIdx.x; r < M; r += LT) perm[r] = r;
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
+    const T *panel = S.panel;
+    const int *sigma = S.sigma;
+    lu_begin(S);
     for (int k0 = 0; k0 < M; k0 += NB) {
-        const int nb = min(NB, M - k0), rows = M - k0;
-        // (a) panel columns k0..k0+nb-1, rows k0..M-1 (already in the current row order)
-        for (int idx = threadIdx.x; idx < nb * rows; idx += LT) {
-            const int c = idx / rows, r = k0 + idx % rows;
-            panel[c * M + r] = A[static_cast<long long>(k0 + c) * M + r];
-        }
-        __syncthreads();
-        // (b) unblocked factorisation of the panel
-        for (int c = 0; c < nb; ++c) {
-            const int col = k0 + c;
-            double best = -1.0;
-            int bidx = M;
-            for (int r = col + threadIdx.x; r < M; r += LT) {
-                const double a = fabs(static_cast<double>(panel[c * M + r]));
-                if (a > best || (a == best && r < bidx)) { best = a; bidx = r; }
-            }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ob = __shfl_xor(best, off, 64);
-                const int oi = __shfl_xor(bidx, off, 64);
-                if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
-            }
-            if (lane == 0) { red_val[wave] = best; red_idx[wave] = bidx; }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double b = red_val[0];
-                int bi = red_idx[0];
-                for (int w = 1; w < LT / 64; ++w)
-                    if (red_val[w] > b || (red_val[w] == b && red_idx[w] < bi)) { b = red_val[w]; bi = red_idx[w]; }
-                if (bi >= M) bi = col;  // all-NaN column: keep the diagonal, flagged below
-                piv[col] = bi;
-                if (bi != col) { const int t = perm[col]; perm[col] = perm[bi]; perm[bi] = t; }
-            }
-            __syncthreads();
-            const int p = piv[col];
-            if (p != col)
-                for (int cc = threadIdx.x; cc < nb; cc += LT) {
-                    const T t = panel[cc * M + col];
-                    panel[cc * M + col] = panel[cc * M + p];
-                    panel[cc * M + p] = t;
-                }
-            __syncthreads();
-            const T pivot = panel[c * M + col];
-            if (pivot != T(0))
-                for (int r = col + 1 + threadIdx.x; r < M; r += LT) panel[c * M + r] /= pivot;
-            __syncthreads();
-            for (int idx = threadIdx.x; idx < (nb - c - 1) * (M - col - 1); idx += LT) {
-                const int cc = c + 1 + idx / (M - col - 1), r = col + 1 + idx % (M - col - 1);
-                panel[cc * M + r] -= panel[c * M + r] * panel[cc * M + col];
-            }
-            __syncthreads();
-        }
-        // (c) panel write-back; (d) gather map of the panel's interchanges
-        for (int idx = threadIdx.x; idx < nb * rows; idx += LT) {
-            const int c = idx / rows, r = k0 + idx % rows;
-            const T val = panel[c * M + r];
-            if (!isfinite(static_cast<double>(val)) || (r == k0 + c && !(fabs(static_cast<double>(val)) > tiny)))
-                bad = 1;
-            A[static_cast<long long>(k0 + c) * M + r] = val;
-        }
-        if (threadIdx.x == 0) {
-            for (int r = k0; r < M; ++r) sigma[r] = r;
-            for (int c = 0; c < nb; ++c) {
-                const int a = k0 + c, b = piv[a];
-                const int t = sigma[a]; sigma[a] = sigma[b]; sigma[b] = t;
-            }
-        }
-        __syncthreads();
-        // (e) per trailing column j: gather rows through sigma, U12 segment by
-        //     forward substitution with the unit L11, A22 -= L21 U12, write back.
+        const int nb = min(NB, M - k0);
+        lu_panel(A, S, k0, nb);
+        // per trailing column j: gather rows through sigma, U12 segment by
+        // forward substitution with the unit L11, A22 -= L21 U12, write back.
         for (int j = k0 + nb + wave; j < M; j += LT / 64) {
             T *colp = A + static_cast<long long>(j) * M;
             T x[NA];
@@ -467,7 +533,7 @@ __global__ __launch_bounds__(LT) void k_lu(T *__restrict__ blocks, int *__restri
                     x[a] = acc;
                 }
             }
-            if (lane < nb && !isfinite(static_cast<double>(x[0]))) bad = 1;
+            if (lane < nb && !isfinite(static_cast<double>(x[0]))) S.bad = 1;
 #pragma unroll
             for (int a = 0; a < NA; ++a) {
                 const int row = k0 + a * 64 + lane;
@@ -476,45 +542,190 @@ __global__ __launch_bounds__(LT) void k_lu(T *__restrict__ blocks, int *__restri
         }
         __syncthreads();
     }
-    // (f) apply the interchanges of later panels to the finished L columns:
-    //     tau = gather map of all interchanges after the panel, built backwards.
-    for (int r = threadIdx.x; r < M; r += LT) sigma[r] = r;
-    __syncthreads();
-    const int npanels = (M + NB - 1) / NB;
-    for (int q = npanels - 1; q >= 0; --q) {
-        const int k0 = q * NB, nb = min(NB, M - k0), lo = k0 + nb;
-        if (q < npanels - 1)
-            for (int c = k0 + wave; c < k0 + nb; c += LT / 64) {
-                T *colp = A + static_cast<long long>(c) * M;
-                T x[NA];
-#pragma unroll
-                for (int a = 0; a < NA; ++a) {
-                    const int row = lo + a * 64 + lane;
-                    x[a] = row < M ? colp[sigma[row]] : T(0);
+    lu_finish(A, S, rowsrc, flag);
+}
+
+// MFMA 16x16x4 tiles (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32): lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16]; the 16 x 16 accumulator
+// holds column l % 16 in every lane, rows l / 16 + 4 r (FP64) or 4 (l / 16) + r
+// (FP32) in register r.
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+template <typename T> struct Mfma16;
+template <> struct Mfma16<double> {
+    using acc = mfma_d4;
+    static __device__ __forceinline__ acc mac(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return lane / 16 + 4 * r; }
+};
+template <> struct Mfma16<float> {
+    using acc = mfma_f4;
+    static __device__ __forceinline__ acc mac(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane / 16) + r; }
+};
+
+// k_lu with the trailing update on MFMA: each wavefront takes 16-column tiles
+// of the trailing matrix; it solves U12 = L11^-1 A12 by forward substitution
+// in the MFMA B layout (the value of row k is shuffled within each column's
+// four lanes), writes U12, and updates the 16-row tiles of A22 with
+// A22 -= L21 U12: NB / 4 MFMAs per tile, L21 read from the LDS panel. No write
+// may overtake a read of a row the panel's interchanges moved:
+//  * RESIDENT: the wavefront gathers its whole tile (U12 rows as B operands,
+//    every A22 tile as an accumulator) through sigma into registers before any
+//    write, which keeps (M - NB) / 4 accumulator values per lane;
+//  * otherwise the source entries of the at most 2 nb moved rows are first
+//    copied to LDS (slot[r], srcrow[q]), every other row is read in place, and
+//    the A22 tiles are streamed one after the other (the next tile's loads
+//    issued before the current tile's MFMAs).
+template <int M, int NB, typename T, int LT, bool RESIDENT>
+__global__ __launch_bounds__(LT) void k_lu_mfma(T *__restrict__ blocks, int *__restrict__ rowsrc, int nblocks,
+                                                int *flag) {
+    static_assert(NB % 4 == 0, "MFMA k-slices of 4 rows");
+    using MF = Mfma16<T>;
+    using Acc = typename MF::acc;
+    constexpr int KS = NB / 4;                   // k-slices of the panel
+    constexpr int TT = (M - NB + 15) / 16;       // A22 tiles below a full panel
+    constexpr int WAVES = LT / 64;
+    constexpr int SLOTS = 2 * NB;  // rows moved by one panel's interchanges
+    __shared__ LuShared<M, NB, T, LT> S;
+    __shared__ T saved[WAVES][RESIDENT ? 1 : SLOTS * 16];  // [wave][slot][column in tile]
+    __shared__ int srcrow[SLOTS];
+    __shared__ int nslots;
+    T *A = blocks + static_cast<long long>(blockIdx.x) * M * M;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int cl = lane % 16, kl = lane / 16;  // column in the tile; k (B) / row quarter (C)
+    const T *panel = S.panel;
+    const int *sigma = S.sigma;
+    int *slot = S.tmp;  // rows >= k0: slot of a moved row, -1 otherwise
+    lu_begin(S);
+    for (int k0 = 0; k0 < M; k0 += NB) {
+        const int nb = min(NB, M - k0);
+        lu_panel(A, S, k0, nb);
+        const int first = k0 + nb, tiles = (M - first + 15) / 16;
+        if (tiles == 0) continue;
+        if constexpr (!RESIDENT) {
+            // slots of the moved rows in row order, by one wavefront with
+            // ballots (hipcc 7.2 lowers the serial form of this loop, a uniform
+            // select in a loop remainder, to s_cselect on a stale SCC)
+            if (wave == 0) {
+                int base = 0;
+                for (int r0 = k0; r0 < M; r0 += 64) {
+                    const int r = r0 + lane;
+                    const int src = r < M ? sigma[r] : r;
+                    const bool moved = src != r;
+                    const unsigned long long mask = __ballot(moved);
+                    const int at = base + __popcll(mask & ((1ull << lane) - 1ull));
+                    if (r < M) slot[r] = moved ? at : -1;
+                    if (moved) srcrow[at] = src;
+                    base += __popcll(mask);
                 }
-#pragma unroll
-                for (int a = 0; a < NA; ++a) {
-                    const int row = lo + a * 64 + lane;
-                    if (row < M) colp[row] = x[a];
-                }
+                if (lane == 0) nslots = base;
             }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int r = k0; r < M; ++r) tmp[r] = r;
-            for (int c = 0; c < nb; ++c) {
-                const int a = k0 + c, b = piv[a];
-                const int t = tmp[a]; tmp[a] = tmp[b]; tmp[b] = t;
-            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (int r = k0 + threadIdx.x; r < M; r += LT) {
-            const int src = sigma[r];
-            sigma[r] = tmp[src];
+        const int ns = RESIDENT ? 0 : nslots;
+        for (int ct = wave; ct < tiles; ct += WAVES) {
+            const int col = first + 16 * ct + cl;
+            const bool colok = col < M;
+            T *colp = A + static_cast<long long>(colok ? col : 0) * M;
+            if constexpr (!RESIDENT) {
+                for (int q = kl; q < ns; q += 4) saved[wave][q * 16 + cl] = colok ? colp[srcrow[q]] : T(0);
+                asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+            }
+            // entry of logical row r after the panel's interchanges (streamed
+            // form: both loads are issued; the moved rows take the saved value)
+            auto value = [&](int r) -> T {
+                if constexpr (RESIDENT) {
+                    return colp[sigma[r]];
+                } else {
+                    const int q = slot[r];
+                    const T kept = saved[wave][(q >= 0 ? q : 0) * 16 + cl];
+                    const T here = colp[r];
+                    return q >= 0 ? kept : here;
+                }
+            };
+            T u[KS];
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int k = 4 * s + kl;
+                u[s] = (colok && k < nb) ? value(k0 + k) : T(0);
+            }
+            auto load_tile = [&](int r0, Acc &c) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = r0 + MF::row(lane, r);
+                    c[r] = (colok && row < M) ? value(row) : T(0);
+                }
+            };
+            auto update_tile = [&](int r0, Acc &c) {
+                const int arow = r0 + cl;
+#pragma unroll
+                for (int s = 0; s < KS; ++s) {
+                    const T a = (arow < M && 4 * s + kl < nb) ? -panel[(4 * s + kl) * M + arow] : T(0);
+                    c = MF::mac(a, u[s], c);
+                }
+            };
+            auto store_tile = [&](int r0, const Acc &c) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = r0 + MF::row(lane, r);
+                    if (colok && row < M) colp[row] = c[r];
+                }
+            };
+            Acc res[RESIDENT ? TT : 1];
+            if constexpr (RESIDENT) {
+#pragma unroll
+                for (int t = 0; t < TT; ++t) load_tile(first + 16 * t, res[t]);
+            }
+            // U12: x_i -= L_ik x_k for k < i < nb, row k taken from lane cl + 16 (k % 4)
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                if (k < nb) {
+                    const T xk = __shfl(u[k / 4], cl + 16 * (k % 4), 64);
+#pragma unroll
+                    for (int s = k / 4; s < KS; ++s) {
+                        const int i = 4 * s + kl;
+                        if (i > k && i < nb) u[s] -= panel[k * M + k0 + i] * xk;
+                    }
+                }
+            }
+            bool bad = false;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int k = 4 * s + kl;
+                if (colok && k < nb) {
+                    colp[k0 + k] = u[s];
+                    bad = bad || !isfinite(static_cast<double>(u[s]));
+                }
+            }
+            if (bad) S.bad = 1;
+            if constexpr (RESIDENT) {
+#pragma unroll
+                for (int t = 0; t < TT; ++t)
+                    if (first + 16 * t < M) {
+                        update_tile(first + 16 * t, res[t]);
+                        store_tile(first + 16 * t, res[t]);
+                    }
+            } else {
+                Acc cur;
+                load_tile(first, cur);
+                for (int r0 = first; r0 < M; r0 += 16) {
+                    Acc next;
+                    if (r0 + 16 < M) load_tile(r0 + 16, next);
+                    update_tile(r0, cur);
+                    store_tile(r0, cur);
+                    cur = next;
+                }
+                // the next tile of this wave overwrites `saved` only after these reads
+                asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            }
         }
         __syncthreads();
     }
-    for (int r = threadIdx.x; r < M; r += LT) rowsrc[static_cast<long long>(blk) * M + r] = perm[r];
-    if (threadIdx.x == 0 && bad) atomicOr(flag, 8);
+    lu_finish(A, S, rowsrc, flag);
 }
 
 
@@ -1464,12 +1675,48 @@ hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache
 #ifndef DGH_LU_THREADS32
 #define DGH_LU_THREADS32 256
 #endif
+// k_lu_mfma (m = 324 only) is selected with DGH_LU_MFMA=1 in the environment.
+// Measured at 256^2, k = 8, ms per refresh, resident trailing tile:
+//   FP32 (256 threads): NB = 12: 177.5; 16: 167.7; 24: 153.7; 32: 158.5 (k_lu: 149.8)
+//   FP64 (256 threads): NB = 12: 307.6; 16: 276.8; 24: 439.7           (k_lu: 223.7)
+// Its trailing update is faster than k_lu's (FP32 NB = 24: 82 ms against
+// 108 ms) but the unblocked panel grows with NB (FP32 panels alone: NB = 12:
+// 42 ms, NB = 24: 71 ms), so k_lu stays the default.
+#ifndef DGH_LU_MFMA_NB64
+#define DGH_LU_MFMA_NB64 16
+#endif
+#ifndef DGH_LU_MFMA_NB32
+#define DGH_LU_MFMA_NB32 24
+#endif
+#ifndef DGH_LU_MFMA_THREADS64
+#define DGH_LU_MFMA_THREADS64 256
+#endif
+#ifndef DGH_LU_MFMA_THREADS32
+#define DGH_LU_MFMA_THREADS32 256
+#endif
+static bool lu_use_mfma() {
+    static const bool v = [] {
+        const char *e = std::getenv("DGH_LU_MFMA");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
 hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s) {
+    if (m == 324 && lu_use_mfma()) {
+        k_lu_mfma<324, DGH_LU_MFMA_NB64, double, DGH_LU_MFMA_THREADS64, true>
+            <<<nblocks, DGH_LU_MFMA_THREADS64, 0, s>>>(blocks, rowsrc, nblocks, flag);
+        return hipGetLastError();
+    }
     DGH_DISPATCH_M(m, {
         k_lu<M, DGH_LU_NB64, double, DGH_LU_THREADS64><<<nblocks, DGH_LU_THREADS64, 0, s>>>(blocks, rowsrc, nblocks, flag);
     })
 }
 hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s) {
+    if (m == 324 && lu_use_mfma()) {
+        k_lu_mfma<324, DGH_LU_MFMA_NB32, float, DGH_LU_MFMA_THREADS32, true>
+            <<<nblocks, DGH_LU_MFMA_THREADS32, 0, s>>>(blocks, rowsrc, nblocks, flag);
+        return hipGetLastError();
+    }
     DGH_DISPATCH_M(m, {
         k_lu<M, DGH_LU_NB32, float, DGH_LU_THREADS32><<<nblocks, DGH_LU_THREADS32, 0, s>>>(blocks, rowsrc, nblocks, flag);
     })

@@ -37,6 +37,7 @@ constexpr int MAX_RESTART = 31;
 struct Line {
     int n, Q, eq, cells;
     double h, gamma, safety;
+    double amplitude, entropy;  // initial Riemann invariant amplitude, p = entropy rho^gamma
     double wd[MAXQ * MAXN];  // weighted derivative WD(q, i) at [q + Q*i]
     double vq[MAXQ * MAXN];  // basis values V(q, k) at [q + Q*k] (identity when Q = n)
     double w[MAXN], left[MAXN], right[MAXN], mass[MAXN];
@@ -80,20 +81,24 @@ __device__ bool einfeldt1(const double l[3], const double r[3], double g, double
     b = safety * fmax(fmax(0.0, ur + cr), v + c);
     return true;
 }
-// simple_wave.cpp:3-25: the simple wave through its characteristic foot
-__device__ void simple_wave(double x, double t, double g, double out[3]) {
-    const double pi = 3.14159265358979323846, a = (3 - g) / 2, b = (g + 1) / 4;
-    double lo = x - t * (a + b), hi = x - t * (a - b), xi = x - t * a;
+// simple_wave.cpp:3-25: the simple wave through its characteristic foot, with
+// J(xi) = A sin(pi xi) the invariant u - 2c/(gamma-1) at t = 0 and
+// p = K rho^gamma: the reference's wave is A = 1, K = 1; the `density` case
+// (no reference counterpart) A = -0.4, K = 1/gamma, i.e. rho = 1 + 0.2 sin(pi x)
+// and u = 2 - rho at t = 0 for gamma = 3.
+__device__ void simple_wave(double x, double t, double g, double A, double K, double out[3]) {
+    const double pi = 3.14159265358979323846, a = (3 - g) / 2, b = (g + 1) / 4, bs = b * fabs(A);
+    double lo = x - t * (a + bs), hi = x - t * (a - bs), xi = x - t * a;
     for (int i = 0; i < 80; ++i) {
-        const double f = xi + t * (a + b * sin(pi * xi)) - x;
+        const double f = xi + t * (a + b * (A * sin(pi * xi))) - x;
         if (fabs(f) < 2e-15) break;
         if (f > 0) hi = xi;
         else lo = xi;
-        const double z = xi - f / (1 + t * b * pi * cos(pi * xi));
+        const double z = xi - f / (1 + t * (b * A) * pi * cos(pi * xi));
         xi = z > lo && z < hi ? z : .5 * (lo + hi);
     }
-    const double jm = sin(pi * xi), v = .5 * (2 + jm), c = (g - 1) * (2 - jm) / 4;
-    const double rho = pow(c * c / g, 1 / (g - 1)), p = pow(rho, g);
+    const double jm = A * sin(pi * xi), v = .5 * (2 + jm), c = (g - 1) * (2 - jm) / 4;
+    const double rho = pow(c * c / (g * K), 1 / (g - 1)), p = K * pow(rho, g);
     out[0] = rho;
     out[1] = rho * v;
     out[2] = p / (g - 1) + .5 * rho * v * v;
@@ -146,7 +151,7 @@ __global__ __launch_bounds__(CELL_BLOCK) void k1_initial(double *u, int *flag, c
     for (int i = 0; i < N; ++i) nodal[i][0] = nodal[i][1] = nodal[i][2] = 0;
     for (int q = 0; q < L.eq; ++q) {
         double s[3];
-        simple_wave(-1 + L.h * (e + .5) + .5 * L.h * L.ex[q], 0.0, g, s);
+        simple_wave(-1 + L.h * (e + .5) + .5 * L.h * L.ex[q], 0.0, g, L.amplitude, L.entropy, s);
 #pragma unroll
         for (int i = 0; i < N; ++i)
 #pragma unroll
@@ -814,7 +819,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k1_integrals(const double *u, d
                 double z[3] = {0, 0, 0}, x[3];
                 for (int v = 0; v < 3; ++v)
                     for (int i = 0; i < N; ++i) z[v] += L.ev[q + L.eq * i] * c[v * N + i];
-                simple_wave(-1 + L.h * (e + .5) + .5 * L.h * L.ex[q], t, L.gamma, x);
+                simple_wave(-1 + L.h * (e + .5) + .5 * L.h * L.ex[q], t, L.gamma, L.amplitude, L.entropy, x);
                 for (int v = 0; v < 3; ++v) part[v] += .5 * L.h * L.ew[q] * (z[v] - x[v]) * (z[v] - x[v]);
             }
         }
@@ -1099,6 +1104,8 @@ struct dgh1_ctx {
         hline.h = h;
         hline.gamma = 3.0;  // the simple wave's gas (simple_wave.hpp:6)
         hline.safety = c.safety;
+        hline.amplitude = c.testcase == 1 ? -0.4 : 1.0;
+        hline.entropy = c.testcase == 1 ? 1 / hline.gamma : 1.0;
         for (int q = 0; q < Q; ++q)
             for (int i = 0; i < n; ++i) {
                 hline.wd[q + Q * i] = Q > n ? b.dq[q + Q * i] : b.wd[q + n * i];
@@ -1377,11 +1384,13 @@ struct dgh1_ctx {
     }
 
     // ---- TimeIntegrator::advance (time_integrator.cpp:45-207), method dg or hdg
+    // simple_wave.hpp:8: characteristics of the initial invariant cross at 1 / (b |A| pi)
+    double shock_time() const { return 4 / ((hline.gamma + 1) * std::acos(-1.0) * std::fabs(hline.amplitude)); }
     void advance(double *solution, dgh_run_stats &stats, std::vector<dgh_update_stats> &updates) {
         const dgh1_config &c = cfg;
         const bool hdg = c.method == DGH_METHOD_HDG;  // dg and both run DG
         require1(admissible(solution), "Invalid initial solution");
-        const double shock = 4 / ((3.0 + 1) * std::acos(-1.0));
+        const double shock = shock_time();
         require1(c.steps > 0 || c.final_time < shock, "Final time reaches simple-wave shock");
         constexpr double alpha = .24169426078821, beta = alpha / 4., eta = .12915286960590;
         constexpr double A[4][4] = {{alpha, 0, 0, 0}, {-alpha, alpha, 0, 0}, {0, 1 - alpha, alpha, 0},
@@ -1625,6 +1634,10 @@ int dgh1_parse_config(int argc, const char *const *argv, dgh1_config *out) {
                 require1(v == "block-jacobi", "The 1D device path preconditions with block Jacobi");
             } else if (o == "--device") {
                 (void)value();
+            } else if (o == "--testcase") {
+                const std::string v = value();
+                require1(v == "simple" || v == "density", "Test case must be simple or density");
+                c.testcase = v == "density" ? 1 : 0;
             } else
                 throw std::runtime_error("Unknown option: " + o);
         }
@@ -1729,7 +1742,7 @@ int dgh1_gmres(dgh1_ctx *ctx, const double *rhs, double *x, int max_iterations, 
 }
 int dgh1_error(dgh1_ctx *ctx, const double *u, double t, double *error3) {
     return run1(ctx, [&] {
-        require1(t < 4 / ((3.0 + 1) * std::acos(-1.0)), "Simple wave reaches shock time");
+        require1(t < ctx->shock_time(), "Simple wave reaches shock time");
         ctx->integrals(u, t, 1, error3);
     });
 }

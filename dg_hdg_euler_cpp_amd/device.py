@@ -419,7 +419,8 @@ class Dgh1Config(C.Structure):
                 ("max_linear", C.c_int), ("max_newton", C.c_int), ("steps", C.c_int), ("newton_steps", C.c_int),
                 ("warmup_updates", C.c_int), ("cfl", C.c_double), ("final_time", C.c_double),
                 ("safety", C.c_double), ("nonlinear_tolerance", C.c_double), ("linear_tolerance", C.c_double),
-                ("integrator", C.c_int), ("method", C.c_int), ("timing_only", C.c_int), ("prefix", C.c_char * 512)]
+                ("integrator", C.c_int), ("method", C.c_int), ("timing_only", C.c_int), ("prefix", C.c_char * 512),
+                ("testcase", C.c_int)]
 
 
 def _lib1():

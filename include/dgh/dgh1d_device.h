@@ -42,6 +42,11 @@ typedef struct dgh1_config {
     int method;     /* enum dgh_method; a context runs dg or hdg */
     int timing_only;
     char prefix[512];
+    /* initial state (no reference counterpart beside the default): 0 the
+     * reference's simple wave (simple_wave.cpp:3-25); 1 `--testcase density`,
+     * rho = 1 + 0.2 sin(pi x), u = 2 - rho, p = rho^3 / 3 at t = 0, the
+     * simple wave with Riemann invariant u + c = 2 */
+    int testcase;
 } dgh1_config;
 
 /* parse() (dg_hdg_1d/src/config.cpp:24-116): flags to a validated config.

@@ -64,24 +64,32 @@ Speed einfeldt(const State &l, const State &r, const Gas &gas, double safety) {
 }
 
 // simple_wave.cpp:3-25: the gamma = 3 simple wave through its characteristic
-// foot xi (safeguarded Newton on xi + t (a + b sin(pi xi)) = x).
+// foot xi (safeguarded Newton on xi + t (a + b J(xi)) = x), J(xi) the
+// Riemann invariant u - 2c/(gamma-1) at t = 0 and u + 2c/(gamma-1) = 2.
+// The reference's wave has J = sin(pi xi) and p = rho^gamma (amplitude 1,
+// entropy 1). The `density` case (no reference counterpart) has
+// J = -0.4 sin(pi xi) and p = rho^gamma / gamma, so that for gamma = 3
+// rho = c = 1 + 0.2 sin(pi x) and u = 2 - rho at t = 0: density, velocity and
+// pressure all positive; it steepens at t = 1 / (b |amplitude| pi).
 struct SimpleWave {
     Gas gas;
-    double shock_time() const { return 4 / ((gas.gamma + 1) * std::acos(-1.0)); }  // simple_wave.hpp:8
+    double amplitude = 1.0, entropy = 1.0;
+    double shock_time() const { return 4 / ((gas.gamma + 1) * std::acos(-1.0) * std::abs(amplitude)); }  // simple_wave.hpp:8
     State exact(double x, double t) const {
         require(t < shock_time(), "Simple wave reaches shock time");
         const double g = gas.gamma, pi = std::acos(-1.0), a = (3 - g) / 2, b = (g + 1) / 4;
-        double lo = x - t * (a + b), hi = x - t * (a - b), xi = x - t * a;
+        const double bs = b * std::abs(amplitude);
+        double lo = x - t * (a + bs), hi = x - t * (a - bs), xi = x - t * a;
         for (int i = 0; i < 80; ++i) {
-            const double f = xi + t * (a + b * std::sin(pi * xi)) - x;
+            const double f = xi + t * (a + b * (amplitude * std::sin(pi * xi))) - x;
             if (std::abs(f) < 2e-15) break;
             if (f > 0) hi = xi;
             else lo = xi;
-            const double z = xi - f / (1 + t * b * pi * std::cos(pi * xi));
+            const double z = xi - f / (1 + t * (b * amplitude) * pi * std::cos(pi * xi));
             xi = z > lo && z < hi ? z : .5 * (lo + hi);
         }
-        const double jm = std::sin(pi * xi), v = .5 * (2 + jm), c = (g - 1) * (2 - jm) / 4;
-        const double rho = std::pow(c * c / g, 1 / (g - 1)), p = std::pow(rho, g);
+        const double jm = amplitude * std::sin(pi * xi), v = .5 * (2 + jm), c = (g - 1) * (2 - jm) / 4;
+        const double rho = std::pow(c * c / (g * entropy), 1 / (g - 1)), p = entropy * std::pow(rho, g);
         return {rho, rho * v, p / (g - 1) + .5 * rho * v * v};
     }
 };
@@ -94,6 +102,7 @@ struct Config {
     double cfl = 2.0, final_time = 0.2, safety = 1.1;
     double nonlinear_tolerance = 1e-12, linear_tolerance = 1e-12;
     std::string integrator = "ssp-dirk3";
+    std::string testcase = "simple";  // build addition: `density` (rho = 1 + 0.2 sin(pi x) at t = 0)
 };
 
 // config.cpp:23-108 (the subset of flags the DG Newton-Krylov path reads)
@@ -115,6 +124,7 @@ Config parse(int argc, const char *const *argv) {
         else if (o == "--T") { c.final_time = real(); time_seen = true; }
         else if (o == "--steps") { c.steps = integer(); steps_seen = true; }
         else if (o == "--integrator") c.integrator = value();
+        else if (o == "--testcase") c.testcase = value();
         else if (o == "--ntol") c.nonlinear_tolerance = real();
         else if (o == "--ltol") c.linear_tolerance = real();
         else if (o == "--restart") c.restart = integer();
@@ -132,6 +142,7 @@ Config parse(int argc, const char *const *argv) {
     require(c.cells >= 3 && c.degree >= 0 && c.degree <= 8 && c.quadrature >= 0, "Invalid mesh or degree");
     require(c.quadrature == 0 || c.quadrature >= c.degree + 1, "Quadrature must be >= degree+1");
     require(c.integrator == "be" || c.integrator == "ssp-dirk3", "Invalid integrator");
+    require(c.testcase == "simple" || c.testcase == "density", "Test case must be simple or density");
     return c;
 }
 
@@ -150,6 +161,10 @@ class Problem {
         : c(cfg), basis(cfg.degree, cfg.quadrature), cells(cfg.cells), nodes(cfg.degree + 1),
           m(NV * (cfg.degree + 1)), h(2.0 / cfg.cells), old(size()), mass(size()), inverse_mass(size()),
           speeds(cfg.cells) {
+        if (cfg.testcase == "density") {
+            example.amplitude = -0.4;
+            example.entropy = 1 / example.gas.gamma;
+        }
         for (int e = 0; e < cells; ++e)
             for (int v = 0; v < NV; ++v)
                 for (int i = 0; i < nodes; ++i) mass[index(e, v, i)] = .5 * h * basis.w[i];

@@ -95,7 +95,8 @@ def test_1d_operators_match_oracle(name, args, torch_cuda):
 
 RUNS = [("be-k2-16", ["--cells", "16", "--k", "2", "--integrator", "be", "--T", "0.05"]),
         ("dirk3-k3-q5-12", ["--cells", "12", "--k", "3", "--quad", "5", "--T", "0.05"]),
-        ("dirk3-k5-16-steps", ["--cells", "16", "--k", "5", "--steps", "2"])]
+        ("dirk3-k5-16-steps", ["--cells", "16", "--k", "5", "--steps", "2"]),
+        ("density-dirk3-k2-32", ["--cells", "32", "--k", "2", "--T", "0.2", "--testcase", "density"])]
 
 
 @pytest.mark.parametrize("name,args", RUNS, ids=[r[0] for r in RUNS])
@@ -198,7 +199,8 @@ def test_1d_hdg_operators_match_oracle(name, args, torch_cuda):
 
 
 HDG_RUNS = [("be-k2-16", ["--cells", "16", "--k", "2", "--integrator", "be", "--T", "0.05"]),
-            ("dirk3-k3-q5-12", ["--cells", "12", "--k", "3", "--quad", "5", "--T", "0.05"])]
+            ("dirk3-k3-q5-12", ["--cells", "12", "--k", "3", "--quad", "5", "--T", "0.05"]),
+            ("density-dirk3-k2-32", ["--cells", "32", "--k", "2", "--T", "0.2", "--testcase", "density"])]
 
 
 @pytest.mark.parametrize("name,args", HDG_RUNS, ids=[r[0] for r in HDG_RUNS])
@@ -245,3 +247,33 @@ def test_1d_hdg_convergence_protocol_matches_reference(cells, torch_cuda):
     assert st["conservation_drift"] <= 1e-14
     for e, key in zip(err, ["rho_L2", "momentum_L2", "energy_L2"]):
         assert abs(e - float(ref[key])) <= 2e-9 * float(ref[key]), (key, e, ref[key])
+
+
+def test_1d_density_protocol(torch_cuda):
+    """The stated 1D initial condition rho = 1 + 0.2 sin(pi x) (`--testcase
+    density`, pinned on the oracle by tests/test_oracle_1d_density.py) through
+    the 1D protocol at 64 cells (k=2, DIRK3, CFL 2, T 0.2): DG and condensed
+    HDG on the device against the oracle, and DG = HDG."""
+    from dg_hdg_euler_cpp_amd import DeviceContext1D
+
+    args = ["--cells", "64", "--k", "2", "--integrator", "ssp-dirk3", "--cfl", "2", "--T", "0.2", "--ntol", "1e-12",
+            "--ltol", "1e-12", "--restart", "20", "--max-linear", "1500", "--max-newton", "30", "--safety", "1.1",
+            "--testcase", "density"]
+    finals = {}
+    for method in ("dg", "hdg"):
+        print(f"\n[case] 1d density {method}", flush=True)
+        orc = O.Oracle1D(args + ["--method", method])
+        u0_o = orc.initial()
+        uN_o, st_o = orc.advance(u0_o, method=method)
+        with DeviceContext1D(args + ["--method", method]) as dev:
+            u = dev.initial()
+            assert rel(u.cpu().numpy(), u0_o) <= 1e-13
+            st = dev.advance(u)
+            uN = u.cpu().numpy()
+            err = dev.error_l2(u, st["final_time"])
+        assert st["steps"] == st_o["steps"] and st["newton_iterations"] == st_o["newton_iterations"]
+        assert rel(uN, uN_o) <= 1e-11
+        assert np.allclose(err, orc.error_l2(uN_o, st_o["final_time"]), rtol=1e-9, atol=0)
+        assert st["conservation_drift"] <= 1e-13
+        finals[method] = uN
+    assert np.abs(finals["dg"] - finals["hdg"]).max() <= 1e-9
