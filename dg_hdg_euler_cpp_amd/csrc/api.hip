@@ -9,6 +9,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -67,6 +68,35 @@ struct Timer {
     double ms = 0;
     long long launches = 0;
 };
+
+// ------------------------------------------------ fixed scratch (no allocation)
+// Device memory of dgh_hdg_apply_noalloc: the skeleton-diagonal product D x
+// (2 cells 4n doubles, padded to 256 bytes) followed by the per-element Schur
+// products S_K gather(x) (cells 16n doubles). Every GPU the library runs on
+// holds its own copy of this array until the process ends. In host code the
+// name `scratch` is not a device address: its per-device address comes from
+// hipGetSymbolAddress (scratch_address) and only that address is passed on.
+constexpr size_t SCRATCH_DOUBLES = size_t(8) << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// Address of `scratch` on the current device, looked up once per device.
+double *scratch_address() {
+    int device = 0;
+    HIPCHECK(hipGetDevice(&device));
+    require(device >= 0 && device < SCRATCH_MAX_DEVICES, "Device ordinal beyond the fixed scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        require(p != nullptr, "hipGetSymbolAddress returned no address for the fixed scratch");
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    return g_scratch_ptr[device];
+}
 
 }  // namespace
 
@@ -755,6 +785,20 @@ struct dgh_ctx {
             HIPCHECK(launch_hdg_schur(n, hS.p, x, hloc.p, geo, stream));
             HIPCHECK(launch_hdg_face_sum(n, nullptr, 0.0, hdiag.p, hloc.p, y, geo, stream));
         });
+    }
+    // hdg_apply with its two intermediate vectors in caller-given device
+    // memory of `capacity` doubles (the fixed scratch): no allocation.
+    void hdg_apply_fixed(const double *x, double *y, double *work, size_t capacity) {
+        const size_t diag = (static_cast<size_t>(T) + 31) / 32 * 32;
+        const size_t need = diag + static_cast<size_t>(cells) * 16 * n;
+        if (need > capacity)
+            throw std::runtime_error("dgh_hdg_apply_noalloc: this mesh needs " + std::to_string(need) +
+                                     " doubles of scratch, beyond the fixed maximum of " +
+                                     std::to_string(capacity) + " (use dgh_hdg_apply)");
+        require(hdg_ready, "Refresh the HDG condensation before applying it");
+        HIPCHECK(launch_hdg_diag_x(n, fcache.p, x, work, basis.p, geo, stream));
+        HIPCHECK(launch_hdg_schur(n, hS.p, x, work + diag, geo, stream));
+        HIPCHECK(launch_hdg_face_sum(n, nullptr, 0.0, work, work + diag, y, geo, stream));
     }
     void hdg_bj_refresh() {
         require(hdg_ready, "Refresh the HDG condensation before assembling a block");
@@ -1665,6 +1709,13 @@ int dgh_hdg_apply(dgh_ctx *ctx, const double *x, double *y) {
     return guarded(ctx, [&] {
         check_ctx(ctx);
         ctx->hdg_apply(x, y);
+        ctx->sync();
+    });
+}
+int dgh_hdg_apply_noalloc(dgh_ctx *ctx, const double *x, double *y) {
+    return guarded(ctx, [&] {
+        check_ctx(ctx);
+        ctx->hdg_apply_fixed(x, y, scratch_address(), SCRATCH_DOUBLES);
         ctx->sync();
     });
 }

@@ -210,6 +210,15 @@ int dgh_hdg_refresh_projected(dgh_ctx *ctx, const double *u, const double *r);
 int dgh_hdg_rhs(dgh_ctx *ctx, double *out);
 /* HybridOperator::apply (hybrid.cpp:73-82): y = D x + sum_K scatter(S_K gather(x)). */
 int dgh_hdg_apply(dgh_ctx *ctx, const double *x, double *y);
+/* dgh_hdg_apply without any device allocation (no reference counterpart; for
+ * callers that cannot allocate, e.g. inside a serving loop or a region that
+ * forbids hipMalloc). Its two intermediate vectors live in a fixed 64 MiB
+ * array per GPU, so the mesh must satisfy 2 cells 4n (rounded up to 32) +
+ * cells 16n <= 8388608 doubles (e.g. 128 x 128 at k = 8, 256 x 256 at k = 4);
+ * larger meshes get DGH_ERROR with the sizes in dgh_last_error. Results equal
+ * dgh_hdg_apply bit for bit. Calls on one GPU share that array: calls that
+ * overlap in time on one GPU (several host threads, several contexts) race. */
+int dgh_hdg_apply_noalloc(dgh_ctx *ctx, const double *x, double *y);
 /* HybridOperator::block(face, face) (hybrid.cpp:83-96), column-major 4n x 4n, HOST buffer. */
 int dgh_hdg_block(dgh_ctx *ctx, int face, double *out_host);
 /* BlockJacobi over the face blocks: refresh (LU of every face block) and apply. */
