@@ -4,6 +4,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <limits>
@@ -35,6 +36,11 @@ thread_local std::string g_create_error;
 void require(bool condition, const char *message) {
     if (!condition) throw std::runtime_error(message);
 }
+// An environment switch that selects an alternative kernel for measurement.
+bool env_flag(const char *name) {
+    const char *e = std::getenv(name);
+    return e && e[0] == '1';
+}
 void hip_check(hipError_t e, const char *what) {
     if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
 }
@@ -44,6 +50,12 @@ using Clock = std::chrono::steady_clock;
 double elapsed(Clock::time_point start) { return std::chrono::duration<double>(Clock::now() - start).count(); }
 
 // ------------------------------------------------------------- device memory
+// Bumped whenever a device buffer is (re)allocated or freed: captured graphs
+// record buffer addresses and stay valid while it is unchanged.
+uint64_t &alloc_epoch() {
+    static uint64_t epoch = 0;
+    return epoch;
+}
 template <class T> struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
@@ -53,16 +65,43 @@ template <class T> struct DevBuf {
         HIPCHECK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
         n = count;
         counter += count * sizeof(T);
+        ++alloc_epoch();
     }
     void release(size_t &counter) {
         if (p) {
             (void)hipFree(p);
             counter -= n * sizeof(T);
+            ++alloc_epoch();
         }
         p = nullptr;
         n = 0;
     }
 };
+
+// One executable graph per Arnoldi step index of a Krylov solver, kept while
+// the key (solver, operators, allocation epoch, step size, restart) is
+// unchanged, i.e. across the solves of one stage, and captured again after.
+struct StepGraphs {
+    std::vector<hipGraphExec_t> execs;
+    std::vector<uint64_t> key;
+    void drop() {
+        for (auto &e : execs)
+            if (e) (void)hipGraphExecDestroy(e);
+        execs.clear();
+        key.clear();
+    }
+    void prepare(const std::vector<uint64_t> &k, int count) {
+        if (k == key && static_cast<int>(execs.size()) == count) return;
+        drop();
+        key = k;
+        execs.assign(count, nullptr);
+    }
+};
+uint64_t bits_of(double v) {
+    uint64_t b;
+    std::memcpy(&b, &v, sizeof b);
+    return b;
+}
 
 struct Timer {
     double ms = 0;
@@ -135,7 +174,7 @@ struct dgh_ctx {
     bool bj32_ready = false;
     // HDG: X [cells][m x t], y [cells][m], S_K [cells][t x t], per-element face
     // results, trace work vectors, face block-Jacobi factors
-    DevBuf<double> hX, hY, hS, hfres, hloc, hdiag, hrhs, itm, face_factors, trace_dir, hlambda, indep;
+    DevBuf<double> hX, hY, hS, hfres, hloc, hdiag, hrhs, hdinv, itm, face_factors, trace_dir, hlambda, indep;
     DevBuf<int> face_rowsrc;
     bool hdg_ready = false, face_bj_ready = false;
     // Newton
@@ -151,7 +190,7 @@ struct dgh_ctx {
     // index, captured within a linear solve on first use; the Hessenberg
     // matrix, rotations and rotated right-hand side live on the device and
     // three scalars per step come back through pinned, mapped host memory.
-    std::vector<hipGraphExec_t> step_graphs;
+    StepGraphs step_graphs, inner_graphs;
     // Block ILU(0) in red-black order (cfg.precond == 1): compact red factors
     // in factors[0, cells/2), black ones in factors[cells/2, cells); the red
     // couplings S [red cell][side][4n x 4n]; apply work vectors
@@ -162,8 +201,8 @@ struct dgh_ctx {
     // mixed precision: FP32 factors in factors_f32 (same split), FP32 Y and work
     DevBuf<float> ilu_Y32, ilu_z32, ilu_w32;
     bool ilu32_ready = false;
-    DevBuf<double> gH, gcs, gsn, gg, gnext;
-    double *gout_host = nullptr, *gout_dev = nullptr;
+    DevBuf<double> gH, gcs, gsn, gg, gnext, igH, igcs, igsn, igg, ignext;
+    double *gout_host = nullptr, *gout_dev = nullptr;  // [0, 4): FP64 solver, [4, 8): inner FP32 solver
     int ck_stages = 0;
     bool adj_bj_ready = false;
 
@@ -218,7 +257,7 @@ struct dgh_ctx {
         DevBuf<double> *bufs[] = {&old, &speeds, &tr_u, &tr_x, &fq, &gtr, &vcache, &fcache, &partials, &scalars,
                                   &factors, &kbasis, &kw, &kz, &kwork, &kres, &ky, &state, &trial, &residual,
                                   &trial_residual, &direction, &rhs, &equation_error, &step_state, &stage_base,
-                                  &slopes, &check_dir, &hX, &hY, &hS, &hfres, &hloc, &hdiag, &hrhs, &itm,
+                                  &slopes, &check_dir, &hX, &hY, &hS, &hfres, &hloc, &hdiag, &hrhs, &hdinv, &itm,
                                   &face_factors, &trace_dir, &hlambda, &indep, &ck_states, &ck_speeds,
                                   &adj_kbar, &adj_lambda, &adj_rhs, &adj_base};
         for (auto *b : bufs) b->release(bytes);
@@ -236,7 +275,8 @@ struct dgh_ctx {
         basis.release(bytes);
         drop_step_graphs();
         if (gout_host) (void)hipHostFree(gout_host);
-        DevBuf<double> *gbufs[] = {&gH, &gcs, &gsn, &gg, &gnext, &ilu_Y, &ilu_S, &ilu_z, &ilu_w, &pack_scratch};
+        DevBuf<double> *gbufs[] = {&gH,  &gcs,  &gsn,   &gg,    &gnext, &igH,  &igcs,        &igsn,
+                                   &igg, &ignext, &ilu_Y, &ilu_S, &ilu_z, &ilu_w, &pack_scratch};
         for (auto *b : gbufs) b->release(bytes);
         DevBuf<float> *ibufs[] = {&ilu_Y32, &ilu_z32, &ilu_w32};
         for (auto *b : ibufs) b->release(bytes);
@@ -246,9 +286,30 @@ struct dgh_ctx {
     }
 
     void drop_step_graphs() {
-        for (auto &e : step_graphs)
-            if (e) (void)hipGraphExecDestroy(e);
-        step_graphs.clear();
+        step_graphs.drop();
+        inner_graphs.drop();
+    }
+    // Pinned, mapped host memory the captured Givens steps report to.
+    void graph_out() {
+        if (gout_host) return;
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&gout_host), 8 * sizeof(double), hipHostMallocMapped));
+        HIPCHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&gout_dev), gout_host, 0));
+    }
+    // Capture fn() on the context stream as one executable graph in *exec.
+    template <class F> void capture(hipGraphExec_t *exec, F &&fn) {
+        hipGraph_t graph = nullptr;
+        HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            fn();
+        } catch (...) {
+            // leave the stream out of capture mode before reporting
+            if (hipStreamEndCapture(stream, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        HIPCHECK(hipStreamEndCapture(stream, &graph));
+        const hipError_t e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHECK(e);
     }
 
     // ---- timing of kernel groups on the context stream
@@ -586,22 +647,24 @@ struct dgh_ctx {
     struct Linear { int iterations; double relative; };
     using Op = std::function<void(const double *, double *)>;
     Linear gmres(const double *rhs, double *x, int limit, double tolerance) {
-        return gmres_generic(D, nullptr, [&](const double *a, double *b) { dg_apply(a, b); },
+        return gmres_generic(1, D, nullptr, [&](const double *a, double *b) { dg_apply(a, b); },
                              [&](const double *a, double *b) { precond_apply(a, b); }, rhs, x, limit, tolerance);
     }
     // GMRES on J^T lambda = rhs, right-preconditioned by the transposed block
     // Jacobi; the residual of J^T lambda is a weak-form vector like the
     // primal one, so the same M^-1 metric applies
     Linear gmres_transposed(const double *rhs, double *x, int limit, double tolerance) {
-        return gmres_generic(D, nullptr, [&](const double *a, double *b) { dg_apply_transpose(a, b); },
+        return gmres_generic(2, D, nullptr, [&](const double *a, double *b) { dg_apply_transpose(a, b); },
                              [&](const double *a, double *b) { bj_apply_transposed(a, b); }, rhs, x, limit,
                              tolerance);
     }
     double vec_norm(const double *x, long long N, const double *weights) {
         return std::sqrt(std::max(0.0, weights ? reduce_sq(x, N, 3, weights) : reduce_sq(x, N, 0)));
     }
-    Linear gmres_generic(long long D, const double *weights, const Op &A, const Op &M, const double *rhs, double *x,
-                         int limit, double tolerance) {
+    // solver: 1 DG, 2 transposed DG, 3 HDG Schur system (the operator pair,
+    // part of the key of the captured step graphs)
+    Linear gmres_generic(int solver, long long D, const double *weights, const Op &A, const Op &M, const double *rhs,
+                         double *x, int limit, double tolerance) {
         const int restart = static_cast<int>(std::min<long long>(D, cfg.restart));
         require(restart + 1 <= 32, "GMRES restart above 31 is not supported on the device");
         kbasis.alloc(static_cast<size_t>(restart + 1) * D, bytes);
@@ -621,25 +684,28 @@ struct dgh_ctx {
         auto Hm = [&](int i, int j) -> double & { return H[i + (restart + 1) * j]; };
         int iterations = 0;
         double host_scalars[65];
-        // every captured argument (operators, caches, step size, buffers) is
-        // fixed for the duration of this solve, so graphs live for one solve
+        // A captured step records the operators' kernel arguments (buffers,
+        // caches, step size): its graph is replayed while none of them moved.
         const bool use_graph = cfg.graph != 0 && !timing;
         if (use_graph) {
-            drop_step_graphs();
-            step_graphs.assign(restart, nullptr);
             gH.alloc(static_cast<size_t>(restart + 1) * restart, bytes);
             gcs.alloc(restart, bytes);
             gsn.alloc(restart, bytes);
             gg.alloc(restart + 1, bytes);
             gnext.alloc(1, bytes);
-            if (!gout_host) {
-                HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&gout_host), 4 * sizeof(double), hipHostMallocMapped));
-                HIPCHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&gout_dev), gout_host, 0));
+            graph_out();
+            auto key = [&] {
+                return std::vector<uint64_t>{static_cast<uint64_t>(solver), alloc_epoch(), bits_of(inv_step),
+                                             static_cast<uint64_t>(restart), static_cast<uint64_t>(D),
+                                             reinterpret_cast<uintptr_t>(weights)};
+            };
+            // before new captures, one eager application of both operators, so
+            // that any buffer they allocate on first use exists
+            if (key() != step_graphs.key) {
+                M(rhs, kz.p);
+                A(kz.p, kw.p);
             }
-            // one eager application of both operators, so that any buffer they
-            // allocate on first use exists before a capture begins
-            M(rhs, kz.p);
-            A(kz.p, kw.p);
+            step_graphs.prepare(key(), restart);
         }
         auto arnoldi_ortho = [&](int j) {
             HIPCHECK(launch_multi_dot(n, V.data(), j + 1, kw.p, weights, D, partials.p, basis.p, stream));
@@ -662,27 +728,17 @@ struct dgh_ctx {
             if (use_graph) {
                 HIPCHECK(hipMemcpyAsync(gg.p, g.data(), (restart + 1) * sizeof(double), hipMemcpyHostToDevice, stream));
                 for (int j = 0; j < restart && iterations < limit; ++j) {
-                    if (!step_graphs[j]) {
-                        hipGraph_t graph = nullptr;
-                        HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-                        try {
+                    if (!step_graphs.execs[j]) {
+                        capture(&step_graphs.execs[j], [&] {
                             M(Vk(j), kz.p);
                             A(kz.p, kw.p);
                             arnoldi_ortho(j);
                             HIPCHECK(launch_arnoldi_givens(j, restart, scalars.p, gH.p, gcs.p, gsn.p, gg.p, gnext.p,
-                                                           gout_dev, stream));
+                                                           gout_dev, 2, stream));
                             HIPCHECK(launch_scale_div_dev(kw.p, gnext.p, Vk(j + 1), D, stream));
-                        } catch (...) {
-                            // leave the stream out of capture mode before reporting
-                            if (hipStreamEndCapture(stream, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
-                            throw;
-                        }
-                        HIPCHECK(hipStreamEndCapture(stream, &graph));
-                        const hipError_t e = hipGraphInstantiate(&step_graphs[j], graph, nullptr, nullptr, 0);
-                        (void)hipGraphDestroy(graph);
-                        HIPCHECK(e);
+                        });
                     }
-                    HIPCHECK(hipGraphLaunch(step_graphs[j], stream));
+                    HIPCHECK(hipGraphLaunch(step_graphs.execs[j], stream));
                     sync();
                     const double gabs = gout_host[0], next = gout_host[1], d = gout_host[2];
                     require(std::isfinite(d) && d > 0, "GMRES Arnoldi breakdown");
@@ -757,6 +813,8 @@ struct dgh_ctx {
         const int batch = static_cast<int>(std::max<long long>(1, std::min<long long>(cells, 40000000000LL / per)));
         factors.alloc(static_cast<size_t>(batch) * m * m, bytes);
         rowsrc.alloc(static_cast<size_t>(batch) * m, bytes);
+        const bool mfma_solve = !env_flag("DGH_HDG_SOLVE_VECTOR");
+        if (mfma_solve) hdinv.alloc(hdg_solve_workspace(n, batch), bytes);
         bj_ready = false;
         set_flag(2, 0);
         for (int first = 0; first < cells; first += batch) {
@@ -766,7 +824,7 @@ struct dgh_ctx {
                                              stream));
                 HIPCHECK(launch_lu(m, factors.p, rowsrc.p, count, flags.p + 2, stream));
                 HIPCHECK(launch_hdg_solve(n, factors.p, rowsrc.p, fcache.p, r, hX.p, hY.p, first, count, basis.p,
-                                          geo, stream));
+                                          geo, stream, mfma_solve ? hdinv.p : nullptr));
             });
         }
         require(read_flag(2) == 0, "Singular or nonfinite LU factors");
@@ -833,7 +891,7 @@ struct dgh_ctx {
         return itm.p;
     }
     Linear hdg_gmres(const double *rhs, double *x, int limit, double tolerance) {
-        return gmres_generic(T, inverse_trace_mass(), [&](const double *a, double *b) { hdg_apply(a, b); },
+        return gmres_generic(3, T, inverse_trace_mass(), [&](const double *a, double *b) { hdg_apply(a, b); },
                              [&](const double *a, double *b) { hdg_bj_apply(a, b); }, rhs, x, limit, tolerance);
     }
     void scale_div(const double *src, double d, double *dst) { HIPCHECK(launch_scale_div(src, d, dst, D, stream)); }
@@ -933,6 +991,38 @@ struct dgh_ctx {
         auto Hm = [&](int i, int j) -> double & { return H[i + (restart + 1) * j]; };
         int iterations = 0;
         double host_scalars[65];
+        auto ortho = [&](int j) {
+            HIPCHECK(launch_multi_dot(n, V.data(), j + 1, iw.p, nullptr, D, partials.p, basis.p, stream));
+            HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p, stream));
+            if (DGH_INNER_GS_PASSES == 2) {
+                HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p, iw.p, nullptr, D, partials.p, 0, basis.p,
+                                           stream));
+                HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p + 32, stream));
+            }
+            HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p + (DGH_INNER_GS_PASSES == 2 ? 32 : 0), iw.p,
+                                       nullptr, D, partials.p, 1, basis.p, stream));
+            HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 1, scalars.p + 64, stream));
+        };
+        // Captured Arnoldi steps as in gmres_generic (the Givens update on the
+        // device); the graphs are kept while the key is unchanged.
+        const bool use_graph = cfg.graph != 0 && !timing;
+        if (use_graph) {
+            igH.alloc(static_cast<size_t>(restart + 1) * restart, bytes);
+            igcs.alloc(restart, bytes);
+            igsn.alloc(restart, bytes);
+            igg.alloc(restart + 1, bytes);
+            ignext.alloc(1, bytes);
+            graph_out();
+            auto key = [&] {
+                return std::vector<uint64_t>{10, alloc_epoch(), bits_of(inv_step), static_cast<uint64_t>(restart),
+                                             static_cast<uint64_t>(D)};
+            };
+            if (key() != inner_graphs.key) {
+                precond_apply32(b, it.p);
+                dg_apply32(it.p, iw.p);
+            }
+            inner_graphs.prepare(key(), restart);
+        }
         while (iterations < max_its) {
             const double beta = dot32(ires.p);
             if (beta <= tol * rhs_norm) break;
@@ -942,20 +1032,39 @@ struct dgh_ctx {
             g[0] = beta;
             int count = 0;
             bool done = false;
-            for (int j = 0; j < restart && iterations < max_its; ++j) {
+            if (use_graph) {
+                HIPCHECK(hipMemcpyAsync(igg.p, g.data(), (restart + 1) * sizeof(double), hipMemcpyHostToDevice, stream));
+                for (int j = 0; j < restart && iterations < max_its; ++j) {
+                    if (!inner_graphs.execs[j]) {
+                        capture(&inner_graphs.execs[j], [&] {
+                            precond_apply32(Vk(j), it.p);
+                            dg_apply32(it.p, iw.p);
+                            ortho(j);
+                            HIPCHECK(launch_arnoldi_givens(j, restart, scalars.p, igH.p, igcs.p, igsn.p, igg.p,
+                                                           ignext.p, gout_dev + 4, DGH_INNER_GS_PASSES, stream));
+                            HIPCHECK(launch_scale_div_dev(iw.p, ignext.p, Vk(j + 1), D, stream));
+                        });
+                    }
+                    HIPCHECK(hipGraphLaunch(inner_graphs.execs[j], stream));
+                    sync();
+                    const double gabs = gout_host[4], next = gout_host[5], d = gout_host[6];
+                    require(std::isfinite(d) && d > 0, "GMRES Arnoldi breakdown");
+                    ++iterations;
+                    count = j + 1;
+                    if (gabs <= tol * rhs_norm || next <= std::numeric_limits<double>::min()) {
+                        done = true;
+                        break;
+                    }
+                }
+                HIPCHECK(hipMemcpyAsync(H.data(), igH.p, H.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+                HIPCHECK(hipMemcpyAsync(g.data(), igg.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+                sync();
+            }
+            for (int j = 0; !use_graph && j < restart && iterations < max_its; ++j) {
                 precond_apply32(Vk(j), it.p);
                 dg_apply32(it.p, iw.p);
                 timed("gmres_ortho", [&] {
-                    HIPCHECK(launch_multi_dot(n, V.data(), j + 1, iw.p, nullptr, D, partials.p, basis.p, stream));
-                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p, stream));
-                    if (DGH_INNER_GS_PASSES == 2) {
-                        HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p, iw.p, nullptr, D, partials.p, 0,
-                                                   basis.p, stream));
-                        HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, j + 1, scalars.p + 32, stream));
-                    }
-                    HIPCHECK(launch_update_dot(n, V.data(), j + 1, scalars.p + (DGH_INNER_GS_PASSES == 2 ? 32 : 0),
-                                               iw.p, nullptr, D, partials.p, 1, basis.p, stream));
-                    HIPCHECK(launch_finish_sum(partials.p, REDUCE_BLOCKS, 1, scalars.p + 64, stream));
+                    ortho(j);
                     HIPCHECK(hipMemcpyAsync(host_scalars, scalars.p, 65 * sizeof(double), hipMemcpyDeviceToHost,
                                             stream));
                     sync();

@@ -42,6 +42,7 @@ dgh_config default_config() {
     c.inner_max = 300;
     c.inner_tolerance = 1e-3;
     c.seed = 2025;
+    c.graph = 1;
     std::strncpy(c.prefix, "output/run_2d", sizeof(c.prefix) - 1);
     return c;
 }
@@ -169,6 +170,7 @@ dgh_config parse_config(int argc, const char *const *argv) {
         else if (option == "--inner-max") c.inner_max = integer();
         else if (option == "--inner-tol") c.inner_tolerance = real();
         else if (option == "--graph") c.graph = 1;
+        else if (option == "--no-graph") c.graph = 0;
         else if (option == "--seed") {
             auto text = value();
             size_t end;
@@ -184,7 +186,7 @@ dgh_config parse_config(int argc, const char *const *argv) {
                          "--integrator ssp-dirk3|be --newton-steps N --warmup-updates N\n"
                          "--timing-only --check-update --no-save-solution\n"
                          "--device gpu --precision fp64|mixed|mixed-bf16|mixed-fp16 --inner-restart N --inner-max N\n"
-                         "--inner-tol R --seed S --graph\n";
+                         "--inner-tol R --seed S --graph | --no-graph\n";
             throw std::runtime_error("help");
         } else
             throw std::runtime_error("Unknown option: " + option);

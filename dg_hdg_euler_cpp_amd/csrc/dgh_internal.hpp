@@ -190,10 +190,13 @@ hipError_t launch_sub_div(const double *x, const double *y, double d, double *ou
 hipError_t launch_assemble_blocks_transposed(int n, const double *vcache, const double *fcache, double *blocks,
                                              double inv_step, const Basis *b, const Geo &g, hipStream_t s);
 // Graph-captured GMRES: one Arnoldi step's Hessenberg / Givens update on the
-// device, and out = x / *beta with beta in device memory.
+// device (H(k, j) from scalars[k] (+ scalars[32 + k] for two Gram-Schmidt
+// passes), the squared norm of the next vector in scalars[64]), and
+// out = x / *beta with beta in device memory.
 hipError_t launch_arnoldi_givens(int j, int restart, const double *scalars, double *H, double *cs, double *sn,
-                                 double *g, double *next, double *out, hipStream_t s);
+                                 double *g, double *next, double *out, int passes, hipStream_t s);
 hipError_t launch_scale_div_dev(const double *x, const double *beta, double *out, long long count, hipStream_t s);
+hipError_t launch_scale_div_dev(const float *x, const double *beta, float *out, long long count, hipStream_t s);
 // gradient of the density integral over the quadrant x > 0, y > 0
 hipError_t launch_quadrant_weights(int n, double *g, const Basis *b, const Geo &geo, hipStream_t s);
 // partials[REDUCE_BLOCKS] of sum a_i b_i
@@ -231,9 +234,13 @@ hipError_t launch_ilu_offdiag(int n, int colour, const double *traces, const dou
 // Statically condensed HDG (hybrid.cpp:31-121).
 hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache, double *blocks, double inv_step,
                                const Basis *b, const Geo &g, int first, int count, hipStream_t s);
+// With a workspace of hdg_solve_workspace(n, count) doubles the solve runs as
+// MFMA tile products on precomputed diagonal-block inverses; with nullptr,
+// as the column-streamed substitution.
+size_t hdg_solve_workspace(int n, int count);
 hipError_t launch_hdg_solve(int n, const double *factors, const int *rowsrc, const double *fcache,
                             const double *resid, double *X, double *Y, int first, int count, const Basis *b,
-                            const Geo &g, hipStream_t s);
+                            const Geo &g, hipStream_t s, double *workspace);
 hipError_t launch_hdg_cx(int n, const double *X, const double *Y, const double *fcache, double *S, double *fres,
                          const Basis *b, const Geo &g, hipStream_t s);
 hipError_t launch_hdg_face_sum(int n, const double *base, double base_scale, const double *diag_x, const double *loc,

@@ -122,6 +122,222 @@ __global__ __launch_bounds__(SOLVE_WAVES * 64) void k_hdg_solve(const double *__
     }
 }
 
+// ------------------------------------------- condensation solve on MFMA
+// The same [X | y] = A^-1 [B | r_K] as k_hdg_solve, as a blocked triangular
+// solve whose every product is a v_mfma_f64_16x16x4 tile product, over
+// 16-row blocks (the last one padded with identity rows) with the inverses of
+// the 16 x 16 diagonal blocks of L and U precomputed per element
+// (k_diag_inverses). One wavefront owns 16 right-hand-side columns of one
+// element and keeps its whole M x 16 tile in registers as MFMA accumulators,
+// block row I in acc[I]. With the diagonal inverses stored negated every
+// product accumulates:
+//   forward   Y_J = (-Linv_JJ) t_J (= -X_J),  t_I += L_IJ Y_J   (I > J)
+//   backward  X_J = (-Uinv_JJ) t'_J,          t'_I += U_IJ X_J  (I < J)
+// where t' = -t starts as the forward's Y.
+//
+// Register layout. The f64 16x16x4 MFMA leaves D[l/16 + 4r][l%16] in lane l,
+// register r. The accumulator rows are relabelled so that hardware row i holds
+// block row tile_row(i) = 4 (i % 4) + i / 4: lane l then holds block rows
+// 4 (l/16) .. 4 (l/16) + 3 of column l % 16, 32 contiguous bytes of an output
+// column. Used as the B operand of k-slice r, register r of lane l is
+// B[k = l/16][j = l%16], the value of block row 4 (l/16) + r, so the A operand
+// of slice r is A[i = l%16][k = l/16] = L_IJ[tile_row(l%16)][4 (l/16) + r].
+__device__ __forceinline__ int tile_row(int i) { return 4 * (i % 4) + i / 4; }
+
+typedef double hdg_d4 __attribute__((ext_vector_type(4)));
+
+// Negated inverses of the unit-lower and upper 16 x 16 diagonal blocks of the
+// factors, padded with identity rows past row M, stored per (element, block,
+// L|U) as 256 doubles in MFMA A-operand order (slice r, lane l). Lanes 0-15 of
+// a 32-lane half invert L (column j = lane), lanes 16-31 invert U, by
+// substitution against e_j.
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_diag_inverses(const double *__restrict__ factors,
+                                                         double *__restrict__ dinv, int count) {
+    constexpr int M = 4 * N * N, NT = (M + 15) / 16;
+    const long long tid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long task = tid / 32;
+    if (task >= static_cast<long long>(count) * NT) return;
+    const int j = static_cast<int>(tid % 16), upper = static_cast<int>((tid / 16) % 2);
+    const int local = static_cast<int>(task / NT), J = static_cast<int>(task % NT);
+    const double *F = factors + static_cast<long long>(local) * M * M;
+    const int base = 16 * J, valid = min(16, M - base);
+    auto at = [&](int i, int k) {  // block entry (i, k), identity past the valid rows
+        return (i < valid && k < valid) ? F[static_cast<long long>(base + k) * M + base + i] : (i == k ? 1.0 : 0.0);
+    };
+    double x[16];
+    if (!upper) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            double s = i == j ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < i; ++k) s -= at(i, k) * x[k];
+            x[i] = s;
+        }
+    } else {
+#pragma unroll
+        for (int i = 15; i >= 0; --i) {
+            double s = i == j ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = i + 1; k < 16; ++k) s -= at(i, k) * x[k];
+            x[i] = s / at(i, i);
+        }
+    }
+    double *D = dinv + (task * 2 + upper) * 256;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) D[(j % 4) * 64 + (j / 4) * 16 + tile_row(p)] = -x[p];
+}
+
+// Logical workgroup index that keeps consecutive groups on one XCD: the
+// hardware hands workgroup b to XCD b % 8, so logical groups are dealt out
+// XCD by XCD (a bijection on 0..nblocks-1).
+__device__ __forceinline__ int xcd_grouped(int b, int nblocks) {
+    const int q = nblocks / 8, rem = nblocks % 8, xcd = b % 8, idx = b / 8;
+    return (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + idx;
+}
+
+#ifndef DGH_SOLVE_MFMA_WAVES
+#define DGH_SOLVE_MFMA_WAVES 8
+#endif
+#ifndef DGH_SOLVE_AHEAD
+#define DGH_SOLVE_AHEAD 2  // L / U tiles loaded ahead of their MFMAs
+#endif
+constexpr int MFMA_SOLVE_WAVES = DGH_SOLVE_MFMA_WAVES;
+
+template <int N>
+__global__ __launch_bounds__(MFMA_SOLVE_WAVES * 64) void k_hdg_solve_mfma(
+    const double *__restrict__ factors, const int *__restrict__ rowsrc, const double *__restrict__ dinv,
+    const double *__restrict__ fc, const double *__restrict__ resid, double *__restrict__ X, double *__restrict__ Y,
+    int first, int count, const Basis *__restrict__ B, Geo g) {
+    constexpr int P = N * N, M = 4 * P, T = 16 * N, NT = (M + 15) / 16, CT = (T + 1 + 15) / 16;
+    constexpr int AHEAD = DGH_SOLVE_AHEAD, RING = AHEAD + 1;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const long long tile = static_cast<long long>(xcd_grouped(blockIdx.x, gridDim.x)) * MFMA_SOLVE_WAVES + wave;
+    if (tile >= static_cast<long long>(count) * CT) return;  // whole wavefront
+    const int local = static_cast<int>(tile / CT), ct = static_cast<int>(tile % CT), cell = first + local;
+    const double *F = factors + static_cast<long long>(local) * M * M;
+    const int *perm = rowsrc + static_cast<long long>(local) * M;
+    const double *Dv = dinv + static_cast<long long>(local) * NT * 512;
+    const long long nodes = 2LL * g.cells * N;
+    const int g4 = lane / 16, arow = tile_row(lane % 16);
+
+    // this lane's right-hand-side column c (B columns, hybrid.cpp:41-42; column T is r_K)
+    const int c = ct * 16 + lane % 16;
+    int s = 0, v = 0, q = 0;
+    double stab = 0;
+    if (c < T) {
+        s = c / (4 * N);
+        v = (c / N) % 4;
+        q = c % N;
+        const int face = cell_face(cell, s, g);
+        const long long fn = static_cast<long long>(face) * N + q;
+        stab = face_weight(face, q, B, g) * fc[(s % 2 ? F_TAUL : F_TAUR) * nodes + fn];
+    }
+    const double *shape = s % 2 ? B->right : B->left;
+    hdg_d4 acc[NT];
+#pragma unroll
+    for (int I = 0; I < NT; ++I) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * I + 4 * g4 + r;
+            double val = 0;
+            if (row < M && c <= T) {
+                const int src = perm[row];
+                if (c == T) {
+                    val = resid[static_cast<long long>(cell) * M + src];
+                } else if (src / P == v) {
+                    const int node = src % P, in = node % N, jn = node / N;
+                    if (s < 2 && jn == q) val = -stab * shape[in];
+                    if (s >= 2 && in == q) val = -stab * shape[jn];
+                }
+            }
+            acc[I][r] = val;
+        }
+    }
+    double ring[RING][4];
+    hdg_d4 cur = acc[0];  // pending block row J, the B operand of the diagonal product
+    // forward substitution over block columns J
+#pragma clang loop unroll(disable)
+    for (int J = 0; J < NT; ++J) {
+        // Tile L(I, J) for I > J; the loads of tiles I <= J (not needed) are
+        // clamped to tile J + 1, so that every position issues the same loads
+        // and the waits before the MFMAs count exactly AHEAD tiles in flight.
+        // Rows past M read row M - 1: they only reach the padding rows. For
+        // J = NT - 1 no tile is needed and the columns are clamped to M - 1.
+        const double *cb = F + static_cast<long long>(16 * J + 4 * g4) * M;
+        const int cols_left = M - 1 - 16 * J - 4 * g4;  // last in-range k-slice of this lane
+        auto load_l = [&](int I, double(&op)[4]) {
+            const int row = min(16 * min(max(I, J + 1), NT - 1) + arow, M - 1);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) op[r] = cb[min(r, cols_left) * M + row];
+        };
+#pragma unroll
+        for (int I = 0; I < AHEAD && I < NT; ++I) load_l(I, ring[I % RING]);
+        const double *Li = Dv + (2 * J) * 256;
+        hdg_d4 y = {0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) y = __builtin_amdgcn_mfma_f64_16x16x4f64(Li[r * 64 + lane], cur[r], y, 0, 0, 0);
+#pragma unroll
+        for (int I = 0; I < NT; ++I) {
+            if (I + AHEAD < NT) load_l(I + AHEAD, ring[(I + AHEAD) % RING]);
+            if (I == J) acc[I] = y;
+            if (I > J) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    acc[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(ring[I % RING][r], y[r], acc[I], 0, 0, 0);
+                if (I == J + 1) cur = acc[I];
+            }
+        }
+    }
+    // backward substitution over block columns J, from the last. The padding
+    // rows are cleared, so the U entries read past column M meet zeros.
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        if (16 * (NT - 1) + 4 * g4 + r >= M) acc[NT - 1][r] = 0.0;
+    cur = acc[NT - 1];
+#pragma clang loop unroll(disable)
+    for (int J = NT - 1; J >= 0; --J) {
+        // Tile U(I, J) for I < J, loads of tiles I >= J clamped to tile J - 1
+        // (to tile 0 for J = 0) as in the forward sweep. Columns past M read
+        // column M - 1; they meet the zero padding rows of X_J.
+        const double *cb = F + static_cast<long long>(16 * J + 4 * g4) * M;
+        const int cols_left = M - 1 - 16 * J - 4 * g4;  // last in-range k-slice of this lane
+        auto load_u = [&](int I, double(&op)[4]) {
+            const int row = 16 * max(min(I, J - 1), 0) + arow;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) op[r] = cb[min(r, cols_left) * M + row];
+        };
+#pragma unroll
+        for (int t = 0; t < AHEAD && t < NT; ++t) load_u(NT - 1 - t, ring[t % RING]);
+        const double *Ui = Dv + (2 * J + 1) * 256;
+        hdg_d4 x = {0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x = __builtin_amdgcn_mfma_f64_16x16x4f64(Ui[r * 64 + lane], cur[r], x, 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int I = NT - 1 - t;  // block rows from the last one up
+            if (t + AHEAD < NT) load_u(I - AHEAD, ring[(t + AHEAD) % RING]);
+            if (I == J) acc[I] = x;
+            if (I < J) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    acc[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(ring[t % RING][r], x[r], acc[I], 0, 0, 0);
+                if (I == J - 1) cur = acc[I];
+            }
+        }
+    }
+    if (c > T) return;
+    double *out = c < T ? X + (static_cast<long long>(cell) * T + c) * M : Y + static_cast<long long>(cell) * M;
+#pragma unroll
+    for (int I = 0; I < NT; ++I) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * I + 4 * g4 + r;
+            if (row < M) out[row] = acc[I][r];
+        }
+    }
+}
+
 // face_jacobian.cpp:88-110 + hybrid.cpp:65-70: C [X | y] per side (restriction
 // of every column to the side, then the 4x4 HDG derivative per face node);
 // S_K = -C X (column-major t x t), face_result = C y.
@@ -355,9 +571,29 @@ unsigned grid_for(long long threads, int block) { return static_cast<unsigned>((
     }                                                   \
     return hipGetLastError();
 
+namespace {
+hipError_t launch_hdg_solve_mfma(int n, const double *factors, const int *rowsrc, double *dinv,
+                                 const double *fcache, const double *resid, double *X, double *Y, int first,
+                                 int count, const Basis *b, const Geo &g, hipStream_t s) {
+    DGH_DISPATCH_N(n, {
+        constexpr int M = 4 * N * N, NT = (M + 15) / 16, CT = (16 * N + 1 + 15) / 16;
+        k_diag_inverses<N><<<grid_for(32LL * count * NT, BLOCK), BLOCK, 0, s>>>(factors, dinv, count);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        k_hdg_solve_mfma<N><<<grid_for(static_cast<long long>(count) * CT, MFMA_SOLVE_WAVES), MFMA_SOLVE_WAVES * 64,
+                              0, s>>>(factors, rowsrc, dinv, fcache, resid, X, Y, first, count, b, g);
+    })
+}
+}  // namespace
+size_t hdg_solve_workspace(int n, int count) {
+    const int M = 4 * n * n, NT = (M + 15) / 16;
+    return static_cast<size_t>(count) * NT * 512;
+}
 hipError_t launch_hdg_solve(int n, const double *factors, const int *rowsrc, const double *fcache,
                             const double *resid, double *X, double *Y, int first, int count, const Basis *b,
-                            const Geo &g, hipStream_t s) {
+                            const Geo &g, hipStream_t s, double *workspace) {
+    if (workspace)
+        return launch_hdg_solve_mfma(n, factors, rowsrc, workspace, fcache, resid, X, Y, first, count, b, g, s);
     DGH_DISPATCH_N(n, {
         const int cols = 16 * N + 1;
         for (int col0 = 0; col0 < cols; col0 += SOLVE_WAVES * SOLVE_COLS) {

@@ -1442,10 +1442,10 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate(PtrsT<TV> V, const double *__
 // [0..j] / [32..32+j] and the squared norm at [64]. out (host-visible)
 // receives |g_{j+1}|, the new norm and the rotation's d.
 __global__ void k_arnoldi_givens(int j, int R, const double *__restrict__ s, double *H, double *cs, double *sn,
-                                 double *g, double *next_out, double *out) {
+                                 double *g, double *next_out, double *out, int passes) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     auto Hm = [&](int i, int k) -> double & { return H[i + (R + 1) * k]; };
-    for (int k = 0; k <= j; ++k) Hm(k, j) = (0.0 + s[k]) + s[32 + k];
+    for (int k = 0; k <= j; ++k) Hm(k, j) = (0.0 + s[k]) + (passes == 2 ? s[32 + k] : 0.0);
     const double next = sqrt(fmax(0.0, s[64]));
     Hm(j + 1, j) = next;
     *next_out = next;
@@ -1469,13 +1469,14 @@ __global__ void k_arnoldi_givens(int j, int R, const double *__restrict__ s, dou
 }
 
 // out = x / *beta when *beta > 0 (basis_.col(j + 1) = column_ / next)
-__global__ __launch_bounds__(BLOCK) void k_scale_div_dev(const double *__restrict__ x, const double *__restrict__ beta,
-                                                         double *__restrict__ out, long long count) {
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_scale_div_dev(const T *__restrict__ x, const double *__restrict__ beta,
+                                                         T *__restrict__ out, long long count) {
     const double b = *beta;
     if (!(b > 0)) return;
     for (long long i = static_cast<long long>(blockIdx.x) * BLOCK + threadIdx.x; i < count;
          i += static_cast<long long>(gridDim.x) * BLOCK)
-        out[i] = x[i] / b;
+        out[i] = static_cast<T>(static_cast<double>(x[i]) / b);
 }
 
 // out = sum_k V_k y_k (gmres.cpp:75)
@@ -1916,12 +1917,16 @@ hipError_t launch_quadrant_weights(int n, double *g, const Basis *b, const Geo &
     DGH_DISPATCH_N(n, { k_quadrant_weights<N><<<4096, BLOCK, 0, s>>>(g, b, geo); })
 }
 hipError_t launch_arnoldi_givens(int j, int restart, const double *scalars, double *H, double *cs, double *sn,
-                                 double *g, double *next, double *out, hipStream_t s) {
-    k_arnoldi_givens<<<1, 64, 0, s>>>(j, restart, scalars, H, cs, sn, g, next, out);
+                                 double *g, double *next, double *out, int passes, hipStream_t s) {
+    k_arnoldi_givens<<<1, 64, 0, s>>>(j, restart, scalars, H, cs, sn, g, next, out, passes);
     return hipGetLastError();
 }
 hipError_t launch_scale_div_dev(const double *x, const double *beta, double *out, long long count, hipStream_t s) {
-    k_scale_div_dev<<<4096, BLOCK, 0, s>>>(x, beta, out, count);
+    k_scale_div_dev<double><<<4096, BLOCK, 0, s>>>(x, beta, out, count);
+    return hipGetLastError();
+}
+hipError_t launch_scale_div_dev(const float *x, const double *beta, float *out, long long count, hipStream_t s) {
+    k_scale_div_dev<float><<<4096, BLOCK, 0, s>>>(x, beta, out, count);
     return hipGetLastError();
 }
 hipError_t launch_dot(const double *a, const double *b, long long count, double *partials, hipStream_t s) {

@@ -68,7 +68,8 @@ typedef struct dgh_config {
     int inner_max;           /* mixed: iteration cap of one inner solve */
     double inner_tolerance;  /* mixed: relative tolerance of one inner solve */
     unsigned long long seed;
-    int graph;               /* 1: each GMRES Arnoldi step replays a captured hipGraph */
+    int graph;               /* 1 (default, --graph): each GMRES Arnoldi step, FP64 or the inner FP32
+                                solve, replays a captured hipGraph; 0 (--no-graph): host-driven loop */
     int precond;             /* 0 block-jacobi, 1 block-ilu0 (block_ilu.cpp; DG, FP64, red-black order) */
     int ordering;            /* 0 natural, 1 colamd, 2 red-black (block-ilu0 on the device) */
     char prefix[512];

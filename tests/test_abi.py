@@ -91,7 +91,8 @@ def test_config_struct_layout_and_defaults():
     c = Cfg.from_buffer_copy(parse(["--nx", "256", "--k", "8", "--integrator", "be", "--steps", "1",
                                     "--precision", "mixed", "--device", "gpu", "--restart", "20"]))
     assert (c.nx, c.degree, c.integrator, c.steps, c.precision, c.restart) == (256, 8, 0, 1, 1, 20)
-    assert c.graph == 0 and Cfg.from_buffer_copy(parse(["--graph"])).graph == 1
+    assert c.graph == 1 and Cfg.from_buffer_copy(parse(["--no-graph"])).graph == 0
+    assert Cfg.from_buffer_copy(parse(["--no-graph", "--graph"])).graph == 1
     # 2-byte factor storage: BF16, or FP16 with a per-block scale of U
     assert Cfg.from_buffer_copy(parse(["--precision", "mixed-bf16"])).precision == 2
     assert Cfg.from_buffer_copy(parse(["--precision", "mixed-fp16"])).precision == 3

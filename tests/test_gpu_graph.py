@@ -17,7 +17,7 @@ def _solve(args, graph, torch):
     from dg_hdg_euler_cpp_amd import DeviceContext
 
     print(f"\n[case] {args} graph={graph}", flush=True)
-    with DeviceContext(args + (["--graph"] if graph else [])) as dev:
+    with DeviceContext(args + (["--graph"] if graph else ["--no-graph"])) as dev:
         u = dev.initial()
         rate = dev.freeze(u)
         dev.set_step(5.0 / rate, u)
@@ -56,7 +56,7 @@ def test_graph_newton_updates_match(torch_cuda):
             "--steps", "1", "--newton-steps", "3", "--restart", "20"]
     runs = []
     for graph in (False, True):
-        with DeviceContext(args + (["--graph"] if graph else [])) as dev:
+        with DeviceContext(args + (["--graph"] if graph else ["--no-graph"])) as dev:
             u = dev.initial()
             runs.append(dev.advance(u))
     a = [x["residual_physical_rms"] for x in runs[0]["updates"]]
@@ -78,7 +78,7 @@ def test_graph_newton_updates_with_ilu_match(torch_cuda):
     runs = []
     for graph in (False, True):
         print(f"\n[case] ilu graph={graph}", flush=True)
-        with DeviceContext(args + (["--graph"] if graph else [])) as dev:
+        with DeviceContext(args + (["--graph"] if graph else ["--no-graph"])) as dev:
             u = dev.initial()
             runs.append(dev.advance(u))
     a = [x["residual_physical_rms"] for x in runs[0]["updates"]]
@@ -89,3 +89,53 @@ def test_graph_newton_updates_with_ilu_match(torch_cuda):
     for k, (x, y) in enumerate(zip(a, b)):
         assert abs(x - y) <= [1e-13, 1e-10, 1e-6][k] * x + 1e-12 * a[0]
     assert all(abs(x - y) <= 1 for x, y in zip(ka, kb))
+
+
+@pytest.mark.parametrize("precision", ["mixed", "mixed-fp16"])
+def test_graph_mixed_newton_updates_match(precision, torch_cuda):
+    """The inner FP32 GMRES of the mixed forms with captured Arnoldi steps
+    (block-Jacobi apply in FP32 or from FP16 factors, FP32 Jv, one Gram-Schmidt
+    pass, the Givens update on the device) against its host-driven loop: the
+    same Newton residual sequence to rounding and the same outer and inner
+    Krylov counts to one iteration, over updates that replay the graphs."""
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    args = ["--nx", "32", "--ny", "32", "--k", "8", "--method", "dg", "--integrator", "be", "--cfl", "5",
+            "--steps", "1", "--newton-steps", "3", "--restart", "20", "--precision", precision]
+    runs = []
+    for graph in (False, True):
+        print(f"\n[case] {precision} graph={graph}", flush=True)
+        with DeviceContext(args + (["--graph"] if graph else ["--no-graph"])) as dev:
+            u = dev.initial()
+            runs.append(dev.advance(u))
+    a = [x["residual_physical_rms"] for x in runs[0]["updates"]]
+    b = [x["residual_physical_rms"] for x in runs[1]["updates"]]
+    ka = [(x["krylov_iterations"], x["outer_iterations"]) for x in runs[0]["updates"]]
+    kb = [(x["krylov_iterations"], x["outer_iterations"]) for x in runs[1]["updates"]]
+    print("rms", a, b, "krylov (inner, outer)", ka, kb)
+    for k, (x, y) in enumerate(zip(a, b)):
+        assert abs(x - y) <= [1e-13, 1e-8, 1e-4][k] * x + 1e-12 * a[0]
+    assert all(abs(x[0] - y[0]) <= 1 and x[1] == y[1] for x, y in zip(ka, kb))
+
+
+def test_graphs_follow_the_step_size(torch_cuda):
+    """Captured graphs are kept across the linear solves of a stage and
+    captured again when the step size changes: two SSP-DIRK3 steps (each with
+    its own dt, four stages) of the mixed and FP64 forms agree with the
+    host-driven loop to the solver tolerance."""
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    for precision in ("fp64", "mixed"):
+        args = ["--nx", "12", "--ny", "10", "--k", "4", "--testcase", "wave", "--integrator", "ssp-dirk3",
+                "--steps", "2", "--cfl", "3", "--restart", "20", "--precision", precision]
+        out = []
+        for graph in (False, True):
+            print(f"\n[case] dirk3 {precision} graph={graph}", flush=True)
+            with DeviceContext(args + (["--graph"] if graph else ["--no-graph"])) as dev:
+                u = dev.initial()
+                st = dev.advance(u)
+                out.append((u.cpu().numpy(), st))
+        (ua, sa), (ub, sb) = out
+        assert sa["steps"] == sb["steps"] == 2
+        assert abs(sa["newton_iterations"] - sb["newton_iterations"]) <= 1
+        assert rel(ub, ua) <= 1e-10, precision
