@@ -169,6 +169,7 @@ struct dgh_ctx {
     DevBuf<unsigned short> factors_bf16;  // 2-byte factors (BF16 or FP16) in the pair layout
     DevBuf<float> factor_scale;           // FP16: 2^-e per block (U' = sigma U)
     DevBuf<float> factor_rdiag;           // 2-byte factors: FP32 1 / (sigma U_ii) per row
+    DevBuf<float> factor_umax;            // FP16: max |U| per block, recorded by the LU
     DevBuf<int> rowsrc32;
     DevBuf<double> check_dir;
     bool bj32_ready = false;
@@ -266,6 +267,7 @@ struct dgh_ctx {
         factors_bf16.release(bytes);
         factor_scale.release(bytes);
         factor_rdiag.release(bytes);
+        factor_umax.release(bytes);
         rowsrc32.release(bytes);
         face_rowsrc.release(bytes);
         factors32.release(bytes);
@@ -910,7 +912,11 @@ struct dgh_ctx {
             HIPCHECK(launch_assemble_blocks(n, vcache.p, fcache.p, factors_f32.p, inv_step, basis.p, geo, 0, cells,
                                             stream));
         });
-        timed("lu", [&] { HIPCHECK(launch_lu(m, factors_f32.p, rowsrc32.p, cells, flags.p + 2, stream)); });
+        const bool fp16 = cfg.precision == DGH_PRECISION_MIXED_FP16;
+        if (fp16) factor_umax.alloc(cells, bytes);
+        timed("lu", [&] {
+            HIPCHECK(launch_lu(m, factors_f32.p, rowsrc32.p, cells, flags.p + 2, stream, fp16 ? factor_umax.p : nullptr));
+        });
         require(read_flag(2) == 0, "Singular block Jacobi factor");
         ilu32_ready = false;
         // the 2-byte forms convert the square LU factors directly; only the
@@ -921,8 +927,8 @@ struct dgh_ctx {
             factor_rdiag.alloc(static_cast<size_t>(cells) * m, bytes);
             set_flag(2, 0);
             timed("pack", [&] {
-                HIPCHECK(launch_to_pairs(m, factors_f32.p, factors_bf16.p, factor_scale.p, factor_rdiag.p, cells,
-                                         cfg.precision == DGH_PRECISION_MIXED_FP16, flags.p + 2, stream));
+                HIPCHECK(launch_to_pairs(m, factors_f32.p, factors_bf16.p, factor_scale.p, factor_rdiag.p,
+                                         factor_umax.p, cells, fp16, flags.p + 2, stream));
             });
             require(read_flag(2) == 0, "Singular block Jacobi factor");
         } else {

@@ -133,7 +133,9 @@ hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag,
 // time through `scratch` (chunk * m*m entries).
 hipError_t launch_pack(int m, double *blocks, int nblocks, double *scratch, int chunk, hipStream_t s);
 hipError_t launch_pack(int m, float *blocks, int nblocks, float *scratch, int chunk, hipStream_t s);
-hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s);
+// umax (optional): max |U| of every block, for the FP16 factor scale
+hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s,
+                     float *umax = nullptr);
 hipError_t launch_bj_apply(int m, const double *factors, const int *rowsrc, const double *r, double *z,
                            int nblocks, hipStream_t s);
 hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const float *r, float *z, int nblocks,
@@ -150,8 +152,9 @@ hipError_t launch_convert(const float *src, double *dst, long long count, hipStr
 // 2-byte factors in the pair layout (BF16, or FP16 with a per-block scale of U),
 // converted from the square column-major FP32 LU factors
 long long pair_factor_stride(int m);
-hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, float *rdiag, int nblocks, int fp16,
-                           int *flag, hipStream_t s);
+// (FP16 takes max |U| per block from umax, as launch_lu records it)
+hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, float *rdiag, const float *umax,
+                           int nblocks, int fp16, int *flag, hipStream_t s);
 hipError_t launch_bj_apply_pairs(int m, const void *factors, const float *sigma, const float *rdiag,
                                  const int *rowsrc, const float *r, float *z, int nblocks, int fp16, hipStream_t s);
 

@@ -361,17 +361,24 @@ template <int M, int NB, typename T, int LT> struct LuShared {
     double red_val[LT / 64];
     int red_idx[LT / 64];
     int bad;
+    unsigned umax;  // bits of max |U| over the finished U entries (non-negative floats order as integers)
 };
 
 template <int M, int NB, typename T, int LT> __device__ void lu_begin(LuShared<M, NB, T, LT> &S) {
     for (int r = threadIdx.x; r < M; r += LT) S.perm[r] = r;
-    if (threadIdx.x == 0) S.bad = 0;
+    if (threadIdx.x == 0) {
+        S.bad = 0;
+        S.umax = 0;
+    }
     __syncthreads();
 }
 
 // Panel columns k0..k0+nb-1 (rows k0..M-1, already in the current row order):
 // load into LDS, unblocked factorisation with partial pivoting, write back,
 // and the gather map sigma of the panel's interchanges. Ends synchronised.
+// Two barriers per column: the pivot search of column c + 1 runs inside the
+// scale-and-update pass of column c (each thread owns its rows), and every
+// thread combines the per-wave maxima itself.
 template <int M, int NB, typename T, int LT>
 __device__ void lu_panel(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int k0, int nb) {
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64, rows = M - k0;
@@ -382,14 +389,18 @@ __device__ void lu_panel(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int k0, i
         panel[c * M + r] = A[static_cast<long long>(k0 + c) * M + r];
     }
     __syncthreads();
+    // max |a| of the next column over this thread's rows, first row on ties
+    double best = -1.0;
+    int bidx = M;
+    auto consider = [&](double a, int r) {
+        if (a > best || (a == best && r < bidx)) {
+            best = a;
+            bidx = r;
+        }
+    };
+    for (int r = k0 + threadIdx.x; r < M; r += LT) consider(fabs(static_cast<double>(panel[r])), r);
     for (int c = 0; c < nb; ++c) {
         const int col = k0 + c;
-        double best = -1.0;
-        int bidx = M;
-        for (int r = col + threadIdx.x; r < M; r += LT) {
-            const double a = fabs(static_cast<double>(panel[c * M + r]));
-            if (a > best || (a == best && r < bidx)) { best = a; bidx = r; }
-        }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             const double ob = __shfl_xor(best, off, 64);
@@ -398,17 +409,15 @@ __device__ void lu_panel(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int k0, i
         }
         if (lane == 0) { S.red_val[wave] = best; S.red_idx[wave] = bidx; }
         __syncthreads();
+        double b = S.red_val[0];
+        int p = S.red_idx[0];
+        for (int w = 1; w < LT / 64; ++w)
+            if (S.red_val[w] > b || (S.red_val[w] == b && S.red_idx[w] < p)) { b = S.red_val[w]; p = S.red_idx[w]; }
+        if (p >= M) p = col;  // all-NaN column: keep the diagonal, flagged below
         if (threadIdx.x == 0) {
-            double b = S.red_val[0];
-            int bi = S.red_idx[0];
-            for (int w = 1; w < LT / 64; ++w)
-                if (S.red_val[w] > b || (S.red_val[w] == b && S.red_idx[w] < bi)) { b = S.red_val[w]; bi = S.red_idx[w]; }
-            if (bi >= M) bi = col;  // all-NaN column: keep the diagonal, flagged below
-            S.piv[col] = bi;
-            if (bi != col) { const int t = S.perm[col]; S.perm[col] = S.perm[bi]; S.perm[bi] = t; }
+            S.piv[col] = p;
+            if (p != col) { const int t = S.perm[col]; S.perm[col] = S.perm[p]; S.perm[p] = t; }
         }
-        __syncthreads();
-        const int p = S.piv[col];
         if (p != col)
             for (int cc = threadIdx.x; cc < nb; cc += LT) {
                 const T t = panel[cc * M + col];
@@ -416,23 +425,31 @@ __device__ void lu_panel(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int k0, i
                 panel[cc * M + p] = t;
             }
         __syncthreads();
+        // scale column c, update columns c+1.., search column c+1 (rows > col)
         const T pivot = panel[c * M + col];
-        if (pivot != T(0))
-            for (int r = col + 1 + threadIdx.x; r < M; r += LT) panel[c * M + r] /= pivot;
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < (nb - c - 1) * (M - col - 1); idx += LT) {
-            const int cc = c + 1 + idx / (M - col - 1), r = col + 1 + idx % (M - col - 1);
-            panel[cc * M + r] -= panel[c * M + r] * panel[cc * M + col];
+        best = -1.0;
+        bidx = M;
+        for (int r = col + 1 + threadIdx.x; r < M; r += LT) {
+            T l = panel[c * M + r];
+            if (pivot != T(0)) {
+                l /= pivot;
+                panel[c * M + r] = l;
+            }
+            for (int cc = c + 1; cc < nb; ++cc) panel[cc * M + r] -= l * panel[cc * M + col];
+            if (c + 1 < nb) consider(fabs(static_cast<double>(panel[(c + 1) * M + r])), r);
         }
-        __syncthreads();
     }
+    __syncthreads();
+    float um = 0.f;  // max |U11| of this thread's entries
     for (int idx = threadIdx.x; idx < nb * rows; idx += LT) {
         const int c = idx / rows, r = k0 + idx % rows;
         const T val = panel[c * M + r];
         if (!isfinite(static_cast<double>(val)) || (r == k0 + c && !(fabs(static_cast<double>(val)) > tiny)))
             S.bad = 1;
+        if (r <= k0 + c) um = fmaxf(um, static_cast<float>(fabs(static_cast<double>(val))));
         A[static_cast<long long>(k0 + c) * M + r] = val;
     }
+    if (um > 0.f) atomicMax(&S.umax, __float_as_uint(um));
     if (threadIdx.x == 0) {
         for (int r = k0; r < M; ++r) S.sigma[r] = r;
         for (int c = 0; c < nb; ++c) {
@@ -446,8 +463,11 @@ __device__ void lu_panel(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int k0, i
 // Interchanges of later panels applied to the finished L columns (tau = gather
 // map of all interchanges after the panel, built backwards), then rowsrc and
 // the singularity flag.
+// umax (optional): max |U| per block, complete only when every finished U
+// entry went through lu_panel (U11) or the caller's trailing pass (U12).
 template <int M, int NB, typename T, int LT>
-__device__ void lu_finish(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int *__restrict__ rowsrc, int *flag) {
+__device__ void lu_finish(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int *__restrict__ rowsrc, int *flag,
+                          float *umax = nullptr) {
     constexpr int NA = (M + 63) / 64;
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     int *sigma = S.sigma, *tmp = S.tmp;
@@ -488,17 +508,19 @@ __device__ void lu_finish(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int *__r
     }
     for (int r = threadIdx.x; r < M; r += LT) rowsrc[static_cast<long long>(blockIdx.x) * M + r] = S.perm[r];
     if (threadIdx.x == 0 && S.bad) atomicOr(flag, 8);
+    if (threadIdx.x == 0 && umax) umax[blockIdx.x] = __uint_as_float(S.umax);
 }
 
 template <int M, int NB, typename T, int LT>
 __global__ __launch_bounds__(LT) void k_lu(T *__restrict__ blocks, int *__restrict__ rowsrc, int nblocks,
-                                              int *flag) {
+                                              int *flag, float *umax) {
     constexpr int NA = (M + 63) / 64;  // row chunks per lane
     __shared__ LuShared<M, NB, T, LT> S;
     T *A = blocks + static_cast<long long>(blockIdx.x) * M * M;
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const T *panel = S.panel;
     const int *sigma = S.sigma;
+    float um = 0.f;  // max |U12| of this thread's entries
     lu_begin(S);
     for (int k0 = 0; k0 < M; k0 += NB) {
         const int nb = min(NB, M - k0);
@@ -534,6 +556,7 @@ __global__ __launch_bounds__(LT) void k_lu(T *__restrict__ blocks, int *__restri
                 }
             }
             if (lane < nb && !isfinite(static_cast<double>(x[0]))) S.bad = 1;
+            if (umax && lane < nb) um = fmaxf(um, static_cast<float>(fabs(static_cast<double>(x[0]))));  // U12
 #pragma unroll
             for (int a = 0; a < NA; ++a) {
                 const int row = k0 + a * 64 + lane;
@@ -542,7 +565,9 @@ __global__ __launch_bounds__(LT) void k_lu(T *__restrict__ blocks, int *__restri
         }
         __syncthreads();
     }
-    lu_finish(A, S, rowsrc, flag);
+    if (um > 0.f) atomicMax(&S.umax, __float_as_uint(um));
+    __syncthreads();
+    lu_finish(A, S, rowsrc, flag, umax);
 }
 
 // MFMA 16x16x4 tiles (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32): lane l
@@ -1009,6 +1034,24 @@ __device__ __forceinline__ float half_value(unsigned short u, bool fp16) {
     __builtin_memcpy(&b, &u, 2);
     return __bfloat162float(b);
 }
+// max |U| of each square FP32 LU block (U on and above the diagonal), for the
+// factorisations that do not record it themselves.
+__global__ __launch_bounds__(BLOCK) void k_umax(const float *__restrict__ square, float *__restrict__ umax, int M) {
+    __shared__ float red[BLOCK / 64];
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const float *src = square + blockIdx.x * static_cast<long long>(M) * M;
+    float mx = 0;
+    for (int c = wave; c < M; c += BLOCK / 64)
+        for (int r = lane; r <= c; r += 64) mx = fmaxf(mx, fabsf(src[static_cast<long long>(c) * M + r]));
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    if (lane == 0) red[wave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0;
+        for (int w = 0; w < BLOCK / 64; ++w) m = fmaxf(m, red[w]);
+        umax[blockIdx.x] = m;
+    }
+}
 // One block per factor: the square column-major FP32 LU (k_lu's output:
 // unit L strictly below the diagonal, U on and above it) into the pair layout
 // in 2 bytes. FP16 scales U by sigma = 2^-e, e = floor(log2 max|U|), so that
@@ -1017,34 +1060,20 @@ __device__ __forceinline__ float half_value(unsigned short u, bool fp16) {
 // copy: rdiag[j] = 1 / (sigma U_jj) in FP32, so that a small pivot keeps its
 // FP32 value instead of becoming a subnormal or zero 2-byte number; a
 // reciprocal that is not finite and nonzero raises bit 16 of *flag.
-// Each wave converts whole columns: every pair (4 bytes) of the layout is
-// written once, pads included, and every source entry is read once (the scale
-// pass reads the upper triangles first).
+// max|U| comes from the factorisation (umax[block], recorded by k_lu or
+// k_umax), so every source entry is read once. Each wave converts whole
+// columns: every pair (4 bytes) of the layout is written once, pads included.
 __global__ __launch_bounds__(BLOCK) void k_to_pairs(const float *__restrict__ square, unsigned short *__restrict__ out,
-                                                    float *__restrict__ sigma, float *__restrict__ rdiag, int M,
-                                                    int fp16, int *flag) {
-    __shared__ float red[BLOCK / 64];
-    __shared__ float scale;
+                                                    float *__restrict__ sigma, float *__restrict__ rdiag,
+                                                    const float *__restrict__ umax, int M, int fp16, int *flag) {
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const long long stride = pair_stride(M);
     const float *src = square + blockIdx.x * static_cast<long long>(M) * M;
     unsigned short *dst = out + blockIdx.x * stride;
     unsigned *dst32 = reinterpret_cast<unsigned *>(dst);
-    float mx = 0;
-    if (fp16)
-        for (int c = wave; c < M; c += BLOCK / 64)
-            for (int r = lane; r <= c; r += 64) mx = fmaxf(mx, fabsf(src[static_cast<long long>(c) * M + r]));
-    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-    if (lane == 0) red[wave] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float m = 0;
-        for (int w = 0; w < BLOCK / 64; ++w) m = fmaxf(m, red[w]);
-        scale = (fp16 && m > 0) ? ldexpf(1.0f, -ilogbf(m)) : 1.0f;
-        sigma[blockIdx.x] = scale;
-    }
-    __syncthreads();
-    const float sc = scale;
+    const float m = fp16 ? umax[blockIdx.x] : 0.f;
+    const float sc = (fp16 && m > 0) ? ldexpf(1.0f, -ilogbf(m)) : 1.0f;
+    if (threadIdx.x == 0) sigma[blockIdx.x] = sc;
     bool bad = false;
     for (int j = threadIdx.x; j < M; j += BLOCK) {
         const float rd = 1.0f / (sc * src[static_cast<long long>(j) * M + j]);
@@ -1709,17 +1738,22 @@ hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag,
         return hipGetLastError();
     }
     DGH_DISPATCH_M(m, {
-        k_lu<M, DGH_LU_NB64, double, DGH_LU_THREADS64><<<nblocks, DGH_LU_THREADS64, 0, s>>>(blocks, rowsrc, nblocks, flag);
+        k_lu<M, DGH_LU_NB64, double, DGH_LU_THREADS64>
+            <<<nblocks, DGH_LU_THREADS64, 0, s>>>(blocks, rowsrc, nblocks, flag, nullptr);
     })
 }
-hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s) {
+hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s, float *umax) {
     if (m == 324 && lu_use_mfma()) {
         k_lu_mfma<324, DGH_LU_MFMA_NB32, float, DGH_LU_MFMA_THREADS32, true>
             <<<nblocks, DGH_LU_MFMA_THREADS32, 0, s>>>(blocks, rowsrc, nblocks, flag);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess || !umax) return e;
+        k_umax<<<nblocks, BLOCK, 0, s>>>(blocks, umax, m);
         return hipGetLastError();
     }
     DGH_DISPATCH_M(m, {
-        k_lu<M, DGH_LU_NB32, float, DGH_LU_THREADS32><<<nblocks, DGH_LU_THREADS32, 0, s>>>(blocks, rowsrc, nblocks, flag);
+        k_lu<M, DGH_LU_NB32, float, DGH_LU_THREADS32>
+            <<<nblocks, DGH_LU_THREADS32, 0, s>>>(blocks, rowsrc, nblocks, flag, umax);
     })
 }
 hipError_t launch_bj_apply(int m, const double *factors, const int *rowsrc, const double *r, double *z,
@@ -1747,10 +1781,11 @@ hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const
     })
 }
 long long pair_factor_stride(int m) { return pair_stride(m); }
-hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, float *rdiag, int nblocks, int fp16,
-                           int *flag, hipStream_t s) {
-    if (m % 2 != 0) return hipErrorInvalidValue;
-    k_to_pairs<<<nblocks, BLOCK, 0, s>>>(square, static_cast<unsigned short *>(out), sigma, rdiag, m, fp16, flag);
+hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, float *rdiag, const float *umax,
+                           int nblocks, int fp16, int *flag, hipStream_t s) {
+    if (m % 2 != 0 || (fp16 && !umax)) return hipErrorInvalidValue;
+    k_to_pairs<<<nblocks, BLOCK, 0, s>>>(square, static_cast<unsigned short *>(out), sigma, rdiag, umax, m, fp16,
+                                         flag);
     return hipGetLastError();
 }
 // columns in flight per wave (measured at 256^2, k=8: FP16 2.67 / 2.77 / 3.18 ms
