@@ -82,117 +82,168 @@ void validate_config(const dgh_config &c) {
     }
 }
 
+namespace {
+// The flag table: each row names a flag, how its argument is read and what it
+// sets. The messages are the reference's (config.cpp:26-45, 100-122), since
+// its harness and ours match on them.
+enum class Arg { None, Int, Real, Word, Count };
+struct Value {
+    int i = 0;
+    double d = 0;
+    unsigned long long u = 0;
+    std::string word;
+};
+struct Flag {
+    const char *name;
+    Arg arg;
+    void (*set)(dgh_config &, const Value &);
+};
+struct Name {
+    const char *word;
+    int code;
+};
+template <size_t K>
+int named(const std::string &word, const Name (&names)[K], const char *message) {
+    for (const Name &n : names)
+        if (word == n.word) return n.code;
+    throw std::runtime_error(message);
+}
+const Name kTestcases[] = {{"diagonal", DGH_DIAGONAL}, {"wave", DGH_WAVE}, {"axis", DGH_AXIS}, {"seeded", DGH_SEEDED}};
+const Name kIntegrators[] = {{"be", 0}, {"ssp-dirk3", 1}};
+const Name kMethods[] = {{"dg", DGH_METHOD_DG}, {"hdg", DGH_METHOD_HDG}, {"both", DGH_METHOD_BOTH}};
+const Name kLinear[] = {{"gmres", 0}, {"direct", 1}};
+const Name kOrderings[] = {{"natural", 0}, {"colamd", 1}, {"redblack", 2}};
+const Name kPreconds[] = {{"block-jacobi", 0}, {"block-ilu0", 1}};
+const Name kPrecisions[] = {{"fp64", DGH_PRECISION_FP64}, {"mixed", DGH_PRECISION_MIXED},
+                            {"mixed-bf16", DGH_PRECISION_MIXED_BF16}, {"mixed-fp16", DGH_PRECISION_MIXED_FP16}};
+
+const char kUsage[] =
+    "2D periodic tensor-product Euler DG / condensed HDG (MI355X device path)\n"
+    "--nx NX --ny NY --testcase diagonal|wave|axis|seeded --k K --quad Q --cfl C (--T T | --steps N)\n"
+    "--method dg|hdg|both --linear direct|gmres --precond block-jacobi|block-ilu0\n"
+    "--ordering natural|colamd|redblack --restart N --max-linear N --max-newton N\n"
+    "--ntol R --ltol R --safety S --prefix PATH\n"
+    "--integrator ssp-dirk3|be --newton-steps N --warmup-updates N\n"
+    "--timing-only --check-update --no-save-solution\n"
+    "--device gpu --precision fp64|mixed|mixed-bf16|mixed-fp16 --inner-restart N --inner-max N\n"
+    "--inner-tol R --seed S --graph | --no-graph\n";
+
+// --T and --steps are exclusive; the parser records which one it saw here.
+constexpr int kSawTime = 1, kSawSteps = 2;
+thread_local int g_seen = 0;
+
+const Flag kFlags[] = {
+    {"--nx", Arg::Int, [](dgh_config &c, const Value &v) { c.nx = v.i; }},
+    {"--ny", Arg::Int, [](dgh_config &c, const Value &v) { c.ny = v.i; }},
+    {"--k", Arg::Int, [](dgh_config &c, const Value &v) { c.degree = v.i; }},
+    {"--quad", Arg::Int, [](dgh_config &c, const Value &v) { c.quadrature = v.i; }},
+    {"--testcase", Arg::Word,
+     [](dgh_config &c, const Value &v) { c.testcase = named(v.word, kTestcases, "Invalid testcase"); }},
+    {"--integrator", Arg::Word,
+     [](dgh_config &c, const Value &v) { c.integrator = named(v.word, kIntegrators, "Invalid integrator"); }},
+    {"--method", Arg::Word,
+     [](dgh_config &c, const Value &v) { c.method = named(v.word, kMethods, "Method must be dg, hdg, or both"); }},
+    {"--linear", Arg::Word,
+     [](dgh_config &c, const Value &v) { c.direct = named(v.word, kLinear, "Invalid linear solver"); }},
+    {"--ordering", Arg::Word,
+     [](dgh_config &c, const Value &v) {
+         c.ordering = named(v.word, kOrderings, "Invalid ordering");
+         c.colamd = c.ordering == 1;
+     }},
+    {"--precond", Arg::Word,
+     [](dgh_config &c, const Value &v) { c.precond = named(v.word, kPreconds, "Invalid block preconditioner"); }},
+    {"--precision", Arg::Word,
+     [](dgh_config &c, const Value &v) { c.precision = named(v.word, kPrecisions, "Invalid precision"); }},
+    {"--device", Arg::Word,
+     [](dgh_config &, const Value &v) {
+         require(v.word == "gpu", "Only --device gpu is built; the CPU path is the reference executable");
+     }},
+    {"--prefix", Arg::Word,
+     [](dgh_config &c, const Value &v) {
+         require(v.word.size() < sizeof(c.prefix), "Prefix too long");
+         std::memset(c.prefix, 0, sizeof(c.prefix));
+         std::strncpy(c.prefix, v.word.c_str(), sizeof(c.prefix) - 1);
+     }},
+    {"--cfl", Arg::Real, [](dgh_config &c, const Value &v) { c.cfl = v.d; }},
+    {"--T", Arg::Real,
+     [](dgh_config &c, const Value &v) {
+         c.final_time = v.d;
+         g_seen |= kSawTime;
+     }},
+    {"--steps", Arg::Int,
+     [](dgh_config &c, const Value &v) {
+         c.steps = v.i;
+         g_seen |= kSawSteps;
+     }},
+    {"--safety", Arg::Real, [](dgh_config &c, const Value &v) { c.safety = v.d; }},
+    {"--ntol", Arg::Real, [](dgh_config &c, const Value &v) { c.nonlinear_tolerance = v.d; }},
+    {"--ltol", Arg::Real, [](dgh_config &c, const Value &v) { c.linear_tolerance = v.d; }},
+    {"--restart", Arg::Int, [](dgh_config &c, const Value &v) { c.restart = v.i; }},
+    {"--max-linear", Arg::Int, [](dgh_config &c, const Value &v) { c.max_linear = v.i; }},
+    {"--max-newton", Arg::Int, [](dgh_config &c, const Value &v) { c.max_newton = v.i; }},
+    {"--newton-steps", Arg::Int, [](dgh_config &c, const Value &v) { c.newton_steps = v.i; }},
+    {"--warmup-updates", Arg::Int, [](dgh_config &c, const Value &v) { c.warmup_updates = v.i; }},
+    {"--inner-restart", Arg::Int, [](dgh_config &c, const Value &v) { c.inner_restart = v.i; }},
+    {"--inner-max", Arg::Int, [](dgh_config &c, const Value &v) { c.inner_max = v.i; }},
+    {"--inner-tol", Arg::Real, [](dgh_config &c, const Value &v) { c.inner_tolerance = v.d; }},
+    {"--seed", Arg::Count, [](dgh_config &c, const Value &v) { c.seed = v.u; }},
+    {"--timing-only", Arg::None, [](dgh_config &c, const Value &) { c.timing_only = 1; }},
+    {"--check-update", Arg::None, [](dgh_config &c, const Value &) { c.check_update = 1; }},
+    {"--no-save-solution", Arg::None, [](dgh_config &c, const Value &) { c.save_solution = 0; }},
+    {"--graph", Arg::None, [](dgh_config &c, const Value &) { c.graph = 1; }},
+    {"--no-graph", Arg::None, [](dgh_config &c, const Value &) { c.graph = 0; }},
+    {"--help", Arg::None,
+     [](dgh_config &, const Value &) {
+         std::cout << kUsage;
+         throw std::runtime_error("help");
+     }},
+};
+
+// The argument text of a flag, converted whole or rejected.
+Value read_argument(Arg arg, const char *text) {
+    Value v;
+    size_t end = 0;
+    const std::string word = text;
+    switch (arg) {
+    case Arg::Int:
+        v.i = std::stoi(word, &end);
+        require(end == word.size(), "Invalid integer");
+        break;
+    case Arg::Count:
+        v.u = std::stoull(word, &end);
+        require(end == word.size(), "Invalid integer");
+        break;
+    case Arg::Real:
+        v.d = std::stod(word, &end);
+        require(end == word.size(), "Invalid number");
+        break;
+    default:
+        v.word = word;
+        break;
+    }
+    return v;
+}
+}  // namespace
+
 dgh_config parse_config(int argc, const char *const *argv) {
     dgh_config c = default_config();
-    bool time_seen = false, steps_seen = false;
-    for (int i = 1; i < argc; ++i) {
-        std::string option = argv[i];
-        auto value = [&]() {
-            require(i + 1 < argc, "Missing option value");
-            return std::string(argv[++i]);
-        };
-        auto integer = [&]() {
-            auto text = value();
-            size_t end;
-            int n = std::stoi(text, &end);
-            require(end == text.size(), "Invalid integer");
-            return n;
-        };
-        auto real = [&]() {
-            auto text = value();
-            size_t end;
-            double n = std::stod(text, &end);
-            require(end == text.size(), "Invalid number");
-            return n;
-        };
-        if (option == "--nx") c.nx = integer();
-        else if (option == "--ny") c.ny = integer();
-        else if (option == "--testcase") {
-            auto name = value();
-            if (name == "diagonal") c.testcase = DGH_DIAGONAL;
-            else if (name == "wave") c.testcase = DGH_WAVE;
-            else if (name == "axis") c.testcase = DGH_AXIS;
-            else if (name == "seeded") c.testcase = DGH_SEEDED;
-            else throw std::runtime_error("Invalid testcase");
-        } else if (option == "--no-save-solution") c.save_solution = 0;
-        else if (option == "--newton-steps") c.newton_steps = integer();
-        else if (option == "--warmup-updates") c.warmup_updates = integer();
-        else if (option == "--integrator") {
-            auto name = value();
-            require(name == "be" || name == "ssp-dirk3", "Invalid integrator");
-            c.integrator = name == "be" ? 0 : 1;
-        } else if (option == "--timing-only") c.timing_only = 1;
-        else if (option == "--check-update") c.check_update = 1;
-        else if (option == "--k") c.degree = integer();
-        else if (option == "--quad") c.quadrature = integer();
-        else if (option == "--cfl") c.cfl = real();
-        else if (option == "--T") { c.final_time = real(); time_seen = true; }
-        else if (option == "--steps") { c.steps = integer(); steps_seen = true; }
-        else if (option == "--safety") c.safety = real();
-        else if (option == "--method") {
-            auto name = value();
-            require(name == "dg" || name == "hdg" || name == "both", "Method must be dg, hdg, or both");
-            c.method = name == "dg" ? DGH_METHOD_DG : name == "hdg" ? DGH_METHOD_HDG : DGH_METHOD_BOTH;
-        } else if (option == "--prefix") {
-            auto p = value();
-            require(p.size() < sizeof(c.prefix), "Prefix too long");
-            std::memset(c.prefix, 0, sizeof(c.prefix));
-            std::strncpy(c.prefix, p.c_str(), sizeof(c.prefix) - 1);
-        } else if (option == "--ntol") c.nonlinear_tolerance = real();
-        else if (option == "--ltol") c.linear_tolerance = real();
-        else if (option == "--restart") c.restart = integer();
-        else if (option == "--max-linear") c.max_linear = integer();
-        else if (option == "--max-newton") c.max_newton = integer();
-        else if (option == "--linear") {
-            auto name = value();
-            require(name == "direct" || name == "gmres", "Invalid linear solver");
-            c.direct = name == "direct";
-        } else if (option == "--ordering") {
-            auto order = value();
-            require(order == "natural" || order == "colamd" || order == "redblack", "Invalid ordering");
-            c.colamd = order == "colamd";
-            c.ordering = order == "natural" ? 0 : order == "colamd" ? 1 : 2;
-        } else if (option == "--precond") {
-            auto name = value();
-            require(name == "block-jacobi" || name == "block-ilu0", "Invalid block preconditioner");
-            c.precond = name == "block-ilu0";
-        } else if (option == "--device") {
-            auto name = value();
-            require(name == "gpu", "Only --device gpu is built; the CPU path is the reference executable");
-        } else if (option == "--precision") {
-            auto name = value();
-            if (name == "fp64") c.precision = DGH_PRECISION_FP64;
-            else if (name == "mixed") c.precision = DGH_PRECISION_MIXED;
-            else if (name == "mixed-bf16") c.precision = DGH_PRECISION_MIXED_BF16;
-            else if (name == "mixed-fp16") c.precision = DGH_PRECISION_MIXED_FP16;
-            else throw std::runtime_error("Invalid precision");
-        } else if (option == "--inner-restart") c.inner_restart = integer();
-        else if (option == "--inner-max") c.inner_max = integer();
-        else if (option == "--inner-tol") c.inner_tolerance = real();
-        else if (option == "--graph") c.graph = 1;
-        else if (option == "--no-graph") c.graph = 0;
-        else if (option == "--seed") {
-            auto text = value();
-            size_t end;
-            unsigned long long s = std::stoull(text, &end);
-            require(end == text.size(), "Invalid integer");
-            c.seed = s;
-        } else if (option == "--help") {
-            std::cout << "2D periodic tensor-product Euler DG / condensed HDG (MI355X device path)\n"
-                         "--nx NX --ny NY --testcase diagonal|wave|axis|seeded --k K --quad Q --cfl C (--T T | --steps N)\n"
-                         "--method dg|hdg|both --linear direct|gmres --precond block-jacobi|block-ilu0\n"
-                         "--ordering natural|colamd|redblack --restart N --max-linear N --max-newton N\n"
-                         "--ntol R --ltol R --safety S --prefix PATH\n"
-                         "--integrator ssp-dirk3|be --newton-steps N --warmup-updates N\n"
-                         "--timing-only --check-update --no-save-solution\n"
-                         "--device gpu --precision fp64|mixed|mixed-bf16|mixed-fp16 --inner-restart N --inner-max N\n"
-                         "--inner-tol R --seed S --graph | --no-graph\n";
-            throw std::runtime_error("help");
-        } else
-            throw std::runtime_error("Unknown option: " + option);
+    g_seen = 0;
+    int i = 1;
+    while (i < argc) {
+        const std::string option = argv[i++];
+        const Flag *flag = nullptr;
+        for (const Flag &f : kFlags)
+            if (option == f.name) flag = &f;
+        if (!flag) throw std::runtime_error("Unknown option: " + option);
+        Value v;
+        if (flag->arg != Arg::None) {
+            require(i < argc, "Missing option value");
+            v = read_argument(flag->arg, argv[i++]);
+        }
+        flag->set(c, v);
     }
-    require(!(time_seen && steps_seen), "Choose either --T or --steps");
-    require(!steps_seen || c.steps > 0, "steps must be positive");
+    require(g_seen != (kSawTime | kSawSteps), "Choose either --T or --steps");
+    require(!(g_seen & kSawSteps) || c.steps > 0, "steps must be positive");
     validate_config(c);
     return c;
 }

@@ -6,6 +6,7 @@
 #include <hip/hip_fp16.h>
 
 #include <cfloat>
+#include <climits>
 #include <cstdlib>
 #include <cstdint>
 #include <type_traits>
@@ -335,6 +336,42 @@ __device__ __forceinline__ float readlane_d(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// Wavefront max / min through DPP row shifts and row broadcasts instead of
+// LDS permutes: after row_shr 1, 2, 4, 8 lane 15 of each row holds its row's
+// result; row_bcast 15 and 31 carry it into rows 1 and 3, so lane 63 ends with
+// the result over all 64 lanes. Lanes without a source keep the identity.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_dmax(double v, double identity) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(identity), __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(identity), __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return fmax(v, __hiloint2double(hi, lo));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_imin(int v) {
+    return min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, CTRL, ROWS, 0xf, false));
+}
+// Pivot rule of one wavefront: the largest `best` (>= -1, never NaN), the
+// smallest `idx` among the lanes holding it; the result in every lane.
+__device__ __forceinline__ void wave_argmax(double &best, int &idx) {
+    double m = best;
+    m = dpp_dmax<0x111, 0xf>(m, -1.0);
+    m = dpp_dmax<0x112, 0xf>(m, -1.0);
+    m = dpp_dmax<0x114, 0xf>(m, -1.0);
+    m = dpp_dmax<0x118, 0xf>(m, -1.0);
+    m = dpp_dmax<0x142, 0xa>(m, -1.0);
+    m = dpp_dmax<0x143, 0xc>(m, -1.0);
+    const double mx = readlane_d(m, 63);
+    int k = best == mx ? idx : INT_MAX;
+    k = dpp_imin<0x111, 0xf>(k);
+    k = dpp_imin<0x112, 0xf>(k);
+    k = dpp_imin<0x114, 0xf>(k);
+    k = dpp_imin<0x118, 0xf>(k);
+    k = dpp_imin<0x142, 0xa>(k);
+    k = dpp_imin<0x143, 0xc>(k);
+    best = mx;
+    idx = __builtin_amdgcn_readlane(k, 63);
+}
+
 // Partial-pivot LU of one column-major M x M block per workgroup with Eigen
 // PartialPivLU / LAPACK semantics (block_jacobi.hpp:12-23): pivot = max |a| in
 // the current column (first row in the current order on ties), rows
@@ -401,12 +438,7 @@ __device__ void lu_panel(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int k0, i
     for (int r = k0 + threadIdx.x; r < M; r += LT) consider(fabs(static_cast<double>(panel[r])), r);
     for (int c = 0; c < nb; ++c) {
         const int col = k0 + c;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ob = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bidx, off, 64);
-            if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
-        }
+        wave_argmax(best, bidx);
         if (lane == 0) { S.red_val[wave] = best; S.red_idx[wave] = bidx; }
         __syncthreads();
         double b = S.red_val[0];
