@@ -345,6 +345,17 @@ def probe(args):
             dev.bj_apply_f32(x32, y32)
         torch.cuda.synchronize()
         log("[probe] bj_apply fp16 factors x3")
+    with DeviceContext(solver_args(args.nx, args.k, args.init, "fp64", method="hdg")) as dev:
+        u = dev.initial()
+        rate = dev.freeze(u)
+        dev.set_step(5.0 / rate, u)
+        r = dev.dg_residual(u)
+        dev.hdg_refresh_projected(u, r)
+        t = dev.hdg_rhs()
+        for _ in range(3):
+            dev.hdg_apply(t)
+        torch.cuda.synchronize()
+        log("[probe] hdg condensation x1, schur apply x3")
     print(json.dumps({"probe": "done", "nx": args.nx, "k": args.k}), flush=True)
 
 
@@ -407,6 +418,66 @@ def summarize_counters(fetch_csv, write_csv, out_json):
         print(f"  {(key + f' [{grid}]')[:72]:72s} {f / 1e9:10.3f} {w / 1e9:10.3f} {(f + w) / 1e9:10.3f} {extra}")
     with open(out_json, "w") as f:
         json.dump(out, f, indent=1)
+
+
+COMPUTE_COUNTERS = ["SQ_INSTS_VALU_FMA_F32", "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_MFMA_MOPS_F32",
+                    "SQ_INSTS_VALU_MFMA_MOPS_F64", "SQ_ACTIVE_INST_VALU", "SQ_VALU_MFMA_BUSY_CYCLES", "SQ_INSTS_LDS",
+                    "SQ_INSTS_VALU", "GRBM_GUI_ACTIVE"]
+
+
+def summarize_compute(counter_csv, trace_csv, kernels=("k_lu", "k_hdg_solve", "k_diag_inverses", "k_ilu_coupling",
+                                                      "k_bj_apply", "k_assemble", "k_element_jv", "k_hdg_schur")):
+    """Arithmetic of the dense kernels from one rocprofv3 --pmc run (the
+    COMPUTE_COUNTERS above) with --kernel-trace durations of the same
+    dispatches: vector FMA flops (FMA instructions x 64 lanes x 2; lanes masked
+    off still count, so this is an upper bound), MFMA flops (MOPS x 512), the
+    rate against the FP64 / FP32 peaks, VALU busy (rocprofv3's VALUBusy:
+    SQ_ACTIVE_INST_VALU / CUs / GRBM_GUI_ACTIVE) and MFMA busy (MfmaUtil:
+    SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x SIMDs)). The counter file
+    sums GRBM_GUI_ACTIVE over the 8 XCDs, so it is divided by 8 (the clock it
+    then implies is printed as a check against the duration)."""
+    import collections
+    import csv
+    import re
+
+    cus, simds, xcds = 256, 1024, 8
+    dur = {}
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            dur[int(r["Dispatch_Id"])] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
+    per = collections.defaultdict(dict)
+    names = {}
+    with open(counter_csv) as f:
+        for r in csv.DictReader(f):
+            d = int(r["Dispatch_Id"])
+            per[d][r["Counter_Name"]] = per[d].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+            names[d] = r["Kernel_Name"]
+    agg = collections.defaultdict(lambda: collections.defaultdict(float))
+    for d, c in per.items():
+        short = re.sub(r"^(void )?dgh::\(anonymous namespace\)::", "", names[d])
+        short = short[:short.index("(")] if "(" in short else short
+        if not short.startswith(kernels) or d not in dur:
+            continue
+        a = agg[short]
+        a["launches"] += 1
+        a["seconds"] += dur[d]
+        for k, v in c.items():
+            a[k] += v
+    print(f"# {'kernel':44s} {'calls':>5s} {'avg ms':>9s} {'FMA TFLOP/s':>11s} {'MFMA TFLOP/s':>12s} "
+          f"{'of peak':>7s} {'VALU busy':>9s} {'MFMA busy':>9s} {'LDS/VALU':>8s} {'clock GHz':>9s}")
+    for k, a in sorted(agg.items(), key=lambda kv: -kv[1]["seconds"]):
+        t = a["seconds"]
+        f64 = a["SQ_INSTS_VALU_FMA_F64"] * 128 + a["SQ_INSTS_VALU_MFMA_MOPS_F64"] * 512
+        f32 = a["SQ_INSTS_VALU_FMA_F32"] * 128 + a["SQ_INSTS_VALU_MFMA_MOPS_F32"] * 512
+        fma = (a["SQ_INSTS_VALU_FMA_F64"] + a["SQ_INSTS_VALU_FMA_F32"]) * 128 / t / 1e12
+        mfma = (a["SQ_INSTS_VALU_MFMA_MOPS_F64"] + a["SQ_INSTS_VALU_MFMA_MOPS_F32"]) * 512 / t / 1e12
+        peak = f64 / t / 1e12 / FP64_PEAK_TFLOPS if f64 >= f32 else f32 / t / 1e12 / FP32_PEAK_TFLOPS
+        gui = a["GRBM_GUI_ACTIVE"] / xcds
+        valu = 100 * a["SQ_ACTIVE_INST_VALU"] / cus / gui if gui else 0.0
+        mf = 100 * a["SQ_VALU_MFMA_BUSY_CYCLES"] / (gui * simds) if gui else 0.0
+        lds = a["SQ_INSTS_LDS"] / a["SQ_INSTS_VALU"] if a["SQ_INSTS_VALU"] else 0.0
+        print(f"  {k[:44]:44s} {int(a['launches']):5d} {1e3 * t / a['launches']:9.3f} {fma:11.2f} {mfma:12.2f} "
+              f"{peak:7.3f} {valu:8.1f}% {mf:8.1f}% {lds:8.3f} {gui / t / 1e9:9.2f}")
 
 
 def summarize_kernel_stats(stats_csv, top=14):
@@ -478,6 +549,8 @@ def main():
     ap.add_argument("--counters", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
                     help="summarise two rocprofv3 counter runs of --probe into HBM bytes per launch")
     ap.add_argument("--kernel-stats", metavar="STATS_CSV", help="summarise a rocprofv3 kernel_stats.csv")
+    ap.add_argument("--compute", nargs=2, metavar=("COUNTER_CSV", "TRACE_CSV"),
+                    help="summarise a rocprofv3 --pmc run of the compute counters (with --kernel-trace)")
     ap.add_argument("--cpu-leg", nargs=4, metavar=("LIB", "NX", "K", "INIT"), help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_leg:
@@ -489,6 +562,9 @@ def main():
         return
     if args.kernel_stats:
         summarize_kernel_stats(args.kernel_stats)
+        return
+    if args.compute:
+        summarize_compute(*args.compute)
         return
     global GRAPH
     GRAPH = not args.no_graph

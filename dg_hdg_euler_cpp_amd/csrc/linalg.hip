@@ -1756,12 +1756,10 @@ hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache
 #ifndef DGH_LU_MFMA_THREADS32
 #define DGH_LU_MFMA_THREADS32 256
 #endif
+// read at each launch, so that a caller can compare the kernels on the same blocks
 static bool lu_use_mfma() {
-    static const bool v = [] {
-        const char *e = std::getenv("DGH_LU_MFMA");
-        return e && e[0] == '1';
-    }();
-    return v;
+    const char *e = std::getenv("DGH_LU_MFMA");
+    return e && e[0] == '1';
 }
 hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag, hipStream_t s) {
     if (m == 324 && lu_use_mfma()) {
