@@ -1728,6 +1728,13 @@ hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache
 #ifndef DGH_LU_NB32
 #define DGH_LU_NB32 12
 #endif
+// The k = 8 element blocks (m = 324 = 18 x 18) take 18-column panels since the
+// DPP pivot search: FP64 / FP32 ms per refresh of 65,536 blocks, NB = 8: 299 /
+// 207; 12: 216 / 141; 16: 227 / 176; 18: 176 / 136; 20 / 22: - / 202, 249 / -;
+// 27: 216 / 144; 36: 503 / 183 (factors bit-identical at every width).
+#ifndef DGH_LU_NB_324
+#define DGH_LU_NB_324 18
+#endif
 // FP64 blocks factor faster with 8 waves per workgroup, FP32 blocks with 4
 // (measured, 256^2 k=8, NB = 12: 224 vs 248 ms (256 threads) and 149 vs
 // 171 ms (512 threads))
@@ -1767,6 +1774,11 @@ hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag,
             <<<nblocks, DGH_LU_MFMA_THREADS64, 0, s>>>(blocks, rowsrc, nblocks, flag);
         return hipGetLastError();
     }
+    if (m == 324) {
+        k_lu<324, DGH_LU_NB_324, double, DGH_LU_THREADS64>
+            <<<nblocks, DGH_LU_THREADS64, 0, s>>>(blocks, rowsrc, nblocks, flag, nullptr);
+        return hipGetLastError();
+    }
     DGH_DISPATCH_M(m, {
         k_lu<M, DGH_LU_NB64, double, DGH_LU_THREADS64>
             <<<nblocks, DGH_LU_THREADS64, 0, s>>>(blocks, rowsrc, nblocks, flag, nullptr);
@@ -1779,6 +1791,11 @@ hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, 
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess || !umax) return e;
         k_umax<<<nblocks, BLOCK, 0, s>>>(blocks, umax, m);
+        return hipGetLastError();
+    }
+    if (m == 324) {
+        k_lu<324, DGH_LU_NB_324, float, DGH_LU_THREADS32>
+            <<<nblocks, DGH_LU_THREADS32, 0, s>>>(blocks, rowsrc, nblocks, flag, umax);
         return hipGetLastError();
     }
     DGH_DISPATCH_M(m, {
