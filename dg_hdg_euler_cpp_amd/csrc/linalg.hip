@@ -543,7 +543,9 @@ __device__ void lu_finish(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int *__r
     if (threadIdx.x == 0 && umax) umax[blockIdx.x] = __uint_as_float(S.umax);
 }
 
-template <int M, int NB, typename T, int LT>
+// CB trailing columns per wavefront pass: every LDS read of L21 (and of
+// L11 in the U12 solve) serves CB columns' FMAs.
+template <int M, int NB, typename T, int LT, int CB = 1>
 __global__ __launch_bounds__(LT) void k_lu(T *__restrict__ blocks, int *__restrict__ rowsrc, int nblocks,
                                               int *flag, float *umax) {
     constexpr int NA = (M + 63) / 64;  // row chunks per lane
@@ -559,40 +561,63 @@ __global__ __launch_bounds__(LT) void k_lu(T *__restrict__ blocks, int *__restri
         lu_panel(A, S, k0, nb);
         // per trailing column j: gather rows through sigma, U12 segment by
         // forward substitution with the unit L11, A22 -= L21 U12, write back.
-        for (int j = k0 + nb + wave; j < M; j += LT / 64) {
-            T *colp = A + static_cast<long long>(j) * M;
-            T x[NA];
+        for (int j0 = k0 + nb + CB * wave; j0 < M; j0 += CB * (LT / 64)) {
+            T *colp[CB];
+            bool live[CB];
+            T x[CB][NA];
 #pragma unroll
-            for (int a = 0; a < NA; ++a) {
-                const int row = k0 + a * 64 + lane;
-                x[a] = row < M ? colp[sigma[row]] : T(0);
+            for (int b = 0; b < CB; ++b) {
+                live[b] = j0 + b < M;
+                colp[b] = A + static_cast<long long>(live[b] ? j0 + b : j0) * M;
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                    const int row = k0 + a * 64 + lane;
+                    x[b][a] = row < M && live[b] ? colp[b][sigma[row]] : T(0);
+                }
             }
-            T u[NB];
+            T u[CB][NB];
 #pragma unroll
             for (int c = 0; c < NB; ++c) {
-                u[c] = T(0);
+#pragma unroll
+                for (int b = 0; b < CB; ++b) u[b][c] = T(0);
                 if (c < nb) {
-                    u[c] = readlane_d(x[0], c);
-                    if (lane > c && lane < nb) x[0] -= panel[c * M + k0 + lane] * u[c];
+#pragma unroll
+                    for (int b = 0; b < CB; ++b) u[b][c] = readlane_d(x[b][0], c);
+                    if (lane > c && lane < nb) {
+                        const T l = panel[c * M + k0 + lane];
+#pragma unroll
+                        for (int b = 0; b < CB; ++b) x[b][0] -= l * u[b][c];
+                    }
                 }
             }
 #pragma unroll
             for (int a = 0; a < NA; ++a) {
                 const int row = k0 + a * 64 + lane;
                 if (row >= k0 + nb && row < M) {
-                    T acc = x[a];
+                    T acc[CB];
+#pragma unroll
+                    for (int b = 0; b < CB; ++b) acc[b] = x[b][a];
 #pragma unroll
                     for (int c = 0; c < NB; ++c)
-                        if (c < nb) acc -= panel[c * M + row] * u[c];
-                    x[a] = acc;
+                        if (c < nb) {
+                            const T l = panel[c * M + row];
+#pragma unroll
+                            for (int b = 0; b < CB; ++b) acc[b] -= l * u[b][c];
+                        }
+#pragma unroll
+                    for (int b = 0; b < CB; ++b) x[b][a] = acc[b];
                 }
             }
-            if (lane < nb && !isfinite(static_cast<double>(x[0]))) S.bad = 1;
-            if (umax && lane < nb) um = fmaxf(um, static_cast<float>(fabs(static_cast<double>(x[0]))));  // U12
 #pragma unroll
-            for (int a = 0; a < NA; ++a) {
-                const int row = k0 + a * 64 + lane;
-                if (row < M) colp[row] = x[a];
+            for (int b = 0; b < CB; ++b) {
+                if (!live[b]) continue;
+                if (lane < nb && !isfinite(static_cast<double>(x[b][0]))) S.bad = 1;
+                if (umax && lane < nb) um = fmaxf(um, static_cast<float>(fabs(static_cast<double>(x[b][0]))));  // U12
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                    const int row = k0 + a * 64 + lane;
+                    if (row < M) colp[b][row] = x[b][a];
+                }
             }
         }
         __syncthreads();
@@ -1732,8 +1757,21 @@ hipError_t launch_assemble_hdg(int n, const double *vcache, const double *fcache
 // DPP pivot search: FP64 / FP32 ms per refresh of 65,536 blocks, NB = 8: 299 /
 // 207; 12: 216 / 141; 16: 227 / 176; 18: 176 / 136; 20 / 22: - / 202, 249 / -;
 // 27: 216 / 144; 36: 503 / 183 (factors bit-identical at every width).
-#ifndef DGH_LU_NB_324
-#define DGH_LU_NB_324 18
+#ifndef DGH_LU_NB64_324
+#define DGH_LU_NB64_324 18
+#endif
+#ifndef DGH_LU_NB32_324
+#define DGH_LU_NB32_324 18
+#endif
+// Trailing columns per wavefront pass (each LDS read of L21 feeds CB
+// columns), m = 324, NB = 18, ms per refresh FP64 / FP32: CB = 1: 176 / 136;
+// 2: 161 / 111; 3: 165 / 104; 4: 210 / 103; 6: - / 105 (NB = 12: CB = 2 / 4:
+// 204 / 115; 3 / 6: 204 / 114); factors bit-identical throughout.
+#ifndef DGH_LU_CB64
+#define DGH_LU_CB64 2
+#endif
+#ifndef DGH_LU_CB32
+#define DGH_LU_CB32 4
 #endif
 // FP64 blocks factor faster with 8 waves per workgroup, FP32 blocks with 4
 // (measured, 256^2 k=8, NB = 12: 224 vs 248 ms (256 threads) and 149 vs
@@ -1775,7 +1813,7 @@ hipError_t launch_lu(int m, double *blocks, int *rowsrc, int nblocks, int *flag,
         return hipGetLastError();
     }
     if (m == 324) {
-        k_lu<324, DGH_LU_NB_324, double, DGH_LU_THREADS64>
+        k_lu<324, DGH_LU_NB64_324, double, DGH_LU_THREADS64, DGH_LU_CB64>
             <<<nblocks, DGH_LU_THREADS64, 0, s>>>(blocks, rowsrc, nblocks, flag, nullptr);
         return hipGetLastError();
     }
@@ -1794,7 +1832,7 @@ hipError_t launch_lu(int m, float *blocks, int *rowsrc, int nblocks, int *flag, 
         return hipGetLastError();
     }
     if (m == 324) {
-        k_lu<324, DGH_LU_NB_324, float, DGH_LU_THREADS32>
+        k_lu<324, DGH_LU_NB32_324, float, DGH_LU_THREADS32, DGH_LU_CB32>
             <<<nblocks, DGH_LU_THREADS32, 0, s>>>(blocks, rowsrc, nblocks, flag, umax);
         return hipGetLastError();
     }
