@@ -411,7 +411,7 @@ def summarize_counters(fetch_csv, write_csv, out_json):
         short = re.sub(r"^(void )?dgh::\(anonymous namespace\)::", "", name)
         short = short[:short.index("(")] if "(" in short else short
         a = next((v for k, v in alg.items() if short.startswith(k)), None) if grid == full[name] else None
-        key = short if grid == full[name] else f"{short} [one colour]"
+        key = short if grid == full[name] else f"{short} [{'one colour' if 2 * grid == full[name] else 'batch'}]"
         out["kernels"][key] = {"grid_threads": grid, "fetch_bytes": f, "write_bytes": w,
                                "hbm_bytes_per_launch": f + w, "algorithmic_bytes": a}
         extra = "" if a is None else f"{a / 1e9:15.3f} {(f + w) / a:6.3f}"
