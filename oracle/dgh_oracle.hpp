@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY. This is a from-scratch restatement of the reference
 // algorithm (YueWu2002/DG_HDG_Euler_Cpp, dg_hdg_2d/ and common/) in plain C++
-// without Eigen. It is the checker for the HIP path: only tests/, the smoke
+// without Eigen. It is the test oracle of the HIP path: only tests/, the smoke
 // entry point and the CPU-baseline leg of bench.py may load it. The product
 // library (dg_hdg_euler_cpp_amd/) never links or calls it.
 //
@@ -28,7 +28,7 @@ inline void require(bool condition, const char *message) {
 }
 
 // ---------------------------------------------------------------- threading
-// The CPU-baseline build of the oracle is compiled with -fopenmp; the checker
+// The CPU-baseline build of the oracle is compiled with -fopenmp; the tests'
 // build is not, and then these loops run in index order. Both builds give
 // bit-identical results: threaded loops write disjoint entries (cells, faces,
 // blocks, vector entries); reductions keep fixed 256-term blocks, each summed

@@ -7,7 +7,7 @@ step combination :197-201). The schedule (step sizes and the states each stage
 froze the HLL speeds at) is recorded on the first run and reused by replays,
 so the map u_0 -> u_N is smooth and its gradient is what the adjoint returns.
 Stage equations are solved by Newton with dense LU (small meshes only).
-Test infrastructure: a checker for the device adjoint, not a product path.
+Test infrastructure: a reference for the device adjoint, not a product path.
 """
 import numpy as np
 

@@ -1,6 +1,6 @@
 """ctypes binding of the CPU oracle (oracle/libdgh_oracle.so).
 
-Test infrastructure: the checker for the HIP path. Imported only by tests/,
+Test infrastructure: the oracle of the HIP path. Imported only by tests/,
 ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py``.
 """
 from __future__ import annotations

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle before it is trusted as the checker of the HIP path.
+"""Pins the CPU oracle before the tests compare the HIP path with it.
 
 Each check is one the reference itself runs (check_2d.cpp / check_1d.cpp) or a
 closed form / archived golden vector of the reference (tests/golden/

@@ -1,5 +1,5 @@
 """The threaded oracle build (oracle/libdgh_oracle_omp.so, the CPU baseline on
-all host cores) gives bit-identical results to the serial checker build: same
+all host cores) gives bit-identical results to the serial build the tests use: same
 residual, Jacobian action, block-Jacobi apply, GMRES solution and Newton run.
 Each build runs in its own process (the binding loads one library per process).
 """
