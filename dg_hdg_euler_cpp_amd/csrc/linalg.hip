@@ -410,6 +410,17 @@ template <int M, int NB, typename T, int LT> __device__ void lu_begin(LuShared<M
     __syncthreads();
 }
 
+// Gather map of one panel's interchanges (row k0 + c with row piv[k0 + c],
+// c = 0..nb-1 in order): the place whose row ends at place r, found by undoing
+// the interchanges from the last one back.
+__device__ __forceinline__ int lu_undo_interchanges(const int *piv, int k0, int nb, int r) {
+    for (int c = nb - 1; c >= 0; --c) {
+        const int a = k0 + c, b = piv[a];
+        r = r == a ? b : (r == b ? a : r);
+    }
+    return r;
+}
+
 // Panel columns k0..k0+nb-1 (rows k0..M-1, already in the current row order):
 // load into LDS, unblocked factorisation with partial pivoting, write back,
 // and the gather map sigma of the panel's interchanges. Ends synchronised.
@@ -418,12 +429,21 @@ template <int M, int NB, typename T, int LT> __device__ void lu_begin(LuShared<M
 // thread combines the per-wave maxima itself.
 template <int M, int NB, typename T, int LT>
 __device__ void lu_panel(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int k0, int nb) {
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64, rows = M - k0;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const double tiny = sizeof(T) == 8 ? DBL_MIN : FLT_MIN;
     T *panel = S.panel;
-    for (int idx = threadIdx.x; idx < nb * rows; idx += LT) {
-        const int c = idx / rows, r = k0 + idx % rows;
-        panel[c * M + r] = A[static_cast<long long>(k0 + c) * M + r];
+    // each thread's rows, all columns' loads in flight together
+#pragma unroll
+    for (int j = 0; j < (M + LT - 1) / LT; ++j) {
+        const int r = k0 + threadIdx.x + j * LT;
+        if (r < M) {
+            T v[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) v[c] = c < nb ? A[static_cast<long long>(k0 + c) * M + r] : T(0);
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c < nb) panel[c * M + r] = v[c];
+        }
     }
     __syncthreads();
     // max |a| of the next column over this thread's rows, first row on ties
@@ -473,22 +493,22 @@ __device__ void lu_panel(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int k0, i
     }
     __syncthreads();
     float um = 0.f;  // max |U11| of this thread's entries
-    for (int idx = threadIdx.x; idx < nb * rows; idx += LT) {
-        const int c = idx / rows, r = k0 + idx % rows;
-        const T val = panel[c * M + r];
-        if (!isfinite(static_cast<double>(val)) || (r == k0 + c && !(fabs(static_cast<double>(val)) > tiny)))
-            S.bad = 1;
-        if (r <= k0 + c) um = fmaxf(um, static_cast<float>(fabs(static_cast<double>(val))));
-        A[static_cast<long long>(k0 + c) * M + r] = val;
+#pragma unroll
+    for (int j = 0; j < (M + LT - 1) / LT; ++j) {
+        const int r = k0 + threadIdx.x + j * LT;
+        if (r >= M) continue;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            if (c >= nb) continue;
+            const T val = panel[c * M + r];
+            if (!isfinite(static_cast<double>(val)) || (r == k0 + c && !(fabs(static_cast<double>(val)) > tiny)))
+                S.bad = 1;
+            if (r <= k0 + c) um = fmaxf(um, static_cast<float>(fabs(static_cast<double>(val))));
+            A[static_cast<long long>(k0 + c) * M + r] = val;
+        }
+        S.sigma[r] = lu_undo_interchanges(S.piv, k0, nb, r);
     }
     if (um > 0.f) atomicMax(&S.umax, __float_as_uint(um));
-    if (threadIdx.x == 0) {
-        for (int r = k0; r < M; ++r) S.sigma[r] = r;
-        for (int c = 0; c < nb; ++c) {
-            const int a = k0 + c, b = S.piv[a];
-            const int t = S.sigma[a]; S.sigma[a] = S.sigma[b]; S.sigma[b] = t;
-        }
-    }
     __syncthreads();
 }
 
@@ -501,41 +521,42 @@ template <int M, int NB, typename T, int LT>
 __device__ void lu_finish(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int *__restrict__ rowsrc, int *flag,
                           float *umax = nullptr) {
     constexpr int NA = (M + 63) / 64;
+    constexpr int G = (NB + LT / 64 - 1) / (LT / 64);  // panel columns per wavefront
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    int *sigma = S.sigma, *tmp = S.tmp;
+    int *sigma = S.sigma;
     for (int r = threadIdx.x; r < M; r += LT) sigma[r] = r;
     __syncthreads();
     const int npanels = (M + NB - 1) / NB;
     for (int q = npanels - 1; q >= 0; --q) {
         const int k0 = q * NB, nb = min(NB, M - k0), lo = k0 + nb;
-        if (q < npanels - 1)
-            for (int c = k0 + wave; c < k0 + nb; c += LT / 64) {
+        if (q < npanels - 1) {
+            // all of this wavefront's columns gathered before any is written
+            T x[G][NA];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int c = k0 + wave + g * (LT / 64);
+                const T *colp = A + static_cast<long long>(c < lo ? c : k0) * M;
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                    const int row = lo + a * 64 + lane;
+                    x[g][a] = c < lo && row < M ? colp[sigma[row]] : T(0);
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int c = k0 + wave + g * (LT / 64);
+                if (c >= lo) continue;
                 T *colp = A + static_cast<long long>(c) * M;
-                T x[NA];
 #pragma unroll
                 for (int a = 0; a < NA; ++a) {
                     const int row = lo + a * 64 + lane;
-                    x[a] = row < M ? colp[sigma[row]] : T(0);
+                    if (row < M) colp[row] = x[g][a];
                 }
-#pragma unroll
-                for (int a = 0; a < NA; ++a) {
-                    const int row = lo + a * 64 + lane;
-                    if (row < M) colp[row] = x[a];
-                }
-            }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int r = k0; r < M; ++r) tmp[r] = r;
-            for (int c = 0; c < nb; ++c) {
-                const int a = k0 + c, b = S.piv[a];
-                const int t = tmp[a]; tmp[a] = tmp[b]; tmp[b] = t;
             }
         }
         __syncthreads();
-        for (int r = k0 + threadIdx.x; r < M; r += LT) {
-            const int src = sigma[r];
-            sigma[r] = tmp[src];
-        }
+        // compose with this panel's interchanges (every sigma[r] lies in k0..M-1)
+        for (int r = k0 + threadIdx.x; r < M; r += LT) sigma[r] = lu_undo_interchanges(S.piv, k0, nb, sigma[r]);
         __syncthreads();
     }
     for (int r = threadIdx.x; r < M; r += LT) rowsrc[static_cast<long long>(blockIdx.x) * M + r] = S.perm[r];
