@@ -389,12 +389,15 @@ __device__ __forceinline__ void wave_argmax(double &best, int &idx) {
 //    does it for 16-column tiles with the A22 -= L21 U12 product on MFMA;
 //  * interchanges of later panels are applied to the finished L columns in one
 //    final gather pass (lu_later_interchanges).
+// Row indices in LDS are 16-bit: with the panel they keep an FP32 block at
+// about 25 KB, six workgroups per CU.
+using lu_row = short;
 template <int M, int NB, typename T, int LT> struct LuShared {
-    T panel[NB * M];  // [c][r], rows k0..M-1 of the current panel
-    int perm[M];      // original row held by each logical row
-    int piv[M];       // row interchanged with row k at step k
-    int sigma[M];     // gather map of the current panel's interchanges
-    int tmp[M];
+    static_assert(M <= 32767, "row indices are 16-bit");
+    T panel[NB * M];    // [c][r], rows k0..M-1 of the current panel
+    lu_row perm[M];     // original row held by each logical row
+    lu_row piv[M];      // row interchanged with row k at step k
+    lu_row sigma[M];    // gather map of the current panel's interchanges
     double red_val[LT / 64];
     int red_idx[LT / 64];
     int bad;
@@ -413,7 +416,7 @@ template <int M, int NB, typename T, int LT> __device__ void lu_begin(LuShared<M
 // Gather map of one panel's interchanges (row k0 + c with row piv[k0 + c],
 // c = 0..nb-1 in order): the place whose row ends at place r, found by undoing
 // the interchanges from the last one back.
-__device__ __forceinline__ int lu_undo_interchanges(const int *piv, int k0, int nb, int r) {
+__device__ __forceinline__ int lu_undo_interchanges(const lu_row *piv, int k0, int nb, int r) {
     for (int c = nb - 1; c >= 0; --c) {
         const int a = k0 + c, b = piv[a];
         r = r == a ? b : (r == b ? a : r);
@@ -468,7 +471,7 @@ __device__ void lu_panel(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int k0, i
         if (p >= M) p = col;  // all-NaN column: keep the diagonal, flagged below
         if (threadIdx.x == 0) {
             S.piv[col] = p;
-            if (p != col) { const int t = S.perm[col]; S.perm[col] = S.perm[p]; S.perm[p] = t; }
+            if (p != col) { const lu_row t = S.perm[col]; S.perm[col] = S.perm[p]; S.perm[p] = t; }
         }
         if (p != col)
             for (int cc = threadIdx.x; cc < nb; cc += LT) {
@@ -523,7 +526,7 @@ __device__ void lu_finish(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int *__r
     constexpr int NA = (M + 63) / 64;
     constexpr int G = (NB + LT / 64 - 1) / (LT / 64);  // panel columns per wavefront
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    int *sigma = S.sigma;
+    lu_row *sigma = S.sigma;
     for (int r = threadIdx.x; r < M; r += LT) sigma[r] = r;
     __syncthreads();
     const int npanels = (M + NB - 1) / NB;
@@ -574,7 +577,7 @@ __global__ __launch_bounds__(LT) void k_lu(T *__restrict__ blocks, int *__restri
     T *A = blocks + static_cast<long long>(blockIdx.x) * M * M;
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const T *panel = S.panel;
-    const int *sigma = S.sigma;
+    const lu_row *sigma = S.sigma;
     float um = 0.f;  // max |U12| of this thread's entries
     lu_begin(S);
     for (int k0 = 0; k0 < M; k0 += NB) {
@@ -696,13 +699,13 @@ __global__ __launch_bounds__(LT) void k_lu_mfma(T *__restrict__ blocks, int *__r
     __shared__ LuShared<M, NB, T, LT> S;
     __shared__ T saved[WAVES][RESIDENT ? 1 : SLOTS * 16];  // [wave][slot][column in tile]
     __shared__ int srcrow[SLOTS];
+    __shared__ int slot[M];  // rows >= k0: slot of a moved row, -1 otherwise
     __shared__ int nslots;
     T *A = blocks + static_cast<long long>(blockIdx.x) * M * M;
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const int cl = lane % 16, kl = lane / 16;  // column in the tile; k (B) / row quarter (C)
     const T *panel = S.panel;
-    const int *sigma = S.sigma;
-    int *slot = S.tmp;  // rows >= k0: slot of a moved row, -1 otherwise
+    const lu_row *sigma = S.sigma;
     lu_begin(S);
     for (int k0 = 0; k0 < M; k0 += NB) {
         const int nb = min(NB, M - k0);
