@@ -335,6 +335,11 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
 __device__ __forceinline__ float readlane_d(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
+// The wavefront's index in its workgroup, marked uniform so that what is
+// derived from it (columns, column addresses, loop bounds) stays scalar.
+__device__ __forceinline__ int wave_index() {
+    return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64));
+}
 
 // Wavefront max / min through DPP row shifts and row broadcasts instead of
 // LDS permutes: after row_shr 1, 2, 4, 8 lane 15 of each row holds its row's
@@ -432,7 +437,7 @@ __device__ __forceinline__ int lu_undo_interchanges(const lu_row *piv, int k0, i
 // thread combines the per-wave maxima itself.
 template <int M, int NB, typename T, int LT>
 __device__ void lu_panel(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int k0, int nb) {
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64, wave = wave_index();
     const double tiny = sizeof(T) == 8 ? DBL_MIN : FLT_MIN;
     T *panel = S.panel;
     // each thread's rows, all columns' loads in flight together
@@ -525,7 +530,7 @@ __device__ void lu_finish(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int *__r
                           float *umax = nullptr) {
     constexpr int NA = (M + 63) / 64;
     constexpr int G = (NB + LT / 64 - 1) / (LT / 64);  // panel columns per wavefront
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64, wave = wave_index();
     lu_row *sigma = S.sigma;
     for (int r = threadIdx.x; r < M; r += LT) sigma[r] = r;
     __syncthreads();
@@ -575,7 +580,7 @@ __global__ __launch_bounds__(LT) void k_lu(T *__restrict__ blocks, int *__restri
     constexpr int NA = (M + 63) / 64;  // row chunks per lane
     __shared__ LuShared<M, NB, T, LT> S;
     T *A = blocks + static_cast<long long>(blockIdx.x) * M * M;
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64, wave = wave_index();
     const T *panel = S.panel;
     const lu_row *sigma = S.sigma;
     float um = 0.f;  // max |U12| of this thread's entries
@@ -702,7 +707,7 @@ __global__ __launch_bounds__(LT) void k_lu_mfma(T *__restrict__ blocks, int *__r
     __shared__ int slot[M];  // rows >= k0: slot of a moved row, -1 otherwise
     __shared__ int nslots;
     T *A = blocks + static_cast<long long>(blockIdx.x) * M * M;
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64, wave = wave_index();
     const int cl = lane % 16, kl = lane / 16;  // column in the tile; k (B) / row quarter (C)
     const T *panel = S.panel;
     const lu_row *sigma = S.sigma;
