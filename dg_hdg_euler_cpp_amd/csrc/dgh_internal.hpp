@@ -50,6 +50,12 @@ __host__ __device__ inline int colour_cell(int k, int colour, int nx) {
 }
 __host__ __device__ inline int colour_index(int cell, int nx) { return (cell / nx) * (nx / 2) + (cell % nx) / 2; }
 
+// The wavefront's index in its workgroup, marked uniform so that what is
+// derived from it (columns, cells, their addresses, loop bounds) stays scalar.
+__device__ __forceinline__ int wave_index() {
+    return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64));
+}
+
 // ------------------------------------------------------------ launchers
 // All launchers enqueue on `stream` and return hipError_t of the launch.
 // Vectors use the reference layout cell*4n^2 + v*n^2 + (i + n*j).

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(SOLVE_WAVES * 64) void k_hdg_solve(const double *__
                                                               int first, int col0, const Basis *__restrict__ B,
                                                               Geo g) {
     constexpr int P = N * N, M = 4 * P, T = 16 * N, NA = (M + 63) / 64;
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64, wave = wave_index();
     const int local = blockIdx.x, cell = first + local;
     const double *F = factors + static_cast<long long>(local) * M * M;
     const int *perm = rowsrc + static_cast<long long>(local) * M;
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(MFMA_SOLVE_WAVES * 64) void k_hdg_solve_mfma(
     int first, int count, const Basis *__restrict__ B, Geo g) {
     constexpr int P = N * N, M = 4 * P, T = 16 * N, NT = (M + 15) / 16, CT = (T + 1 + 15) / 16;
     constexpr int AHEAD = DGH_SOLVE_AHEAD, RING = AHEAD + 1;
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64, wave = wave_index();
     const long long tile = static_cast<long long>(xcd_grouped(blockIdx.x, gridDim.x)) * MFMA_SOLVE_WAVES + wave;
     if (tile >= static_cast<long long>(count) * CT) return;  // whole wavefront
     const int local = static_cast<int>(tile / CT), ct = static_cast<int>(tile % CT), cell = first + local;
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(BLOCK) void k_hdg_schur(const double *__restrict__ 
                                                      double *__restrict__ loc, Geo g) {
     constexpr int T = 16 * N, NA = (T + 63) / 64, FW = 4 * N;
     const int lane = threadIdx.x % 64;
-    const long long cell = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + threadIdx.x / 64;
+    const long long cell = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + wave_index();
     if (cell >= g.cells) return;
     double xg[NA], acc[NA];
 #pragma unroll
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(BLOCK) void k_hdg_recover(const double *__restrict_
                                                        Geo g) {
     constexpr int P = N * N, M = 4 * P, T = 16 * N, FW = 4 * N, NA = (M + 63) / 64, NT = (T + 63) / 64;
     const int lane = threadIdx.x % 64;
-    const long long cell = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + threadIdx.x / 64;
+    const long long cell = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + wave_index();
     if (cell >= g.cells) return;
     double xg[NT], acc[NA];
 #pragma unroll

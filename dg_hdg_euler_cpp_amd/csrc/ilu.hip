@@ -92,7 +92,7 @@ __global__ __launch_bounds__(ILU_WAVES * 64) void k_ilu_coupling(const TF *__res
                                                                  int col0, const Basis *__restrict__ B, Geo g) {
     constexpr int P = N * N, M = 4 * P, TS = 4 * N, T = 4 * TS, NA = (M + 63) / 64;
     __shared__ TF colbuf[ILU_WAVES][M];
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64, wave = wave_index();
     const long long k = blockIdx.x;
     const TF *F = factors + k * M * M;
     const int *perm = rowsrc + k * M;
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_red_back(const TY *__restrict__ Y
                                                         TV *__restrict__ z, Geo g) {
     constexpr int P = N * N, M = 4 * P, T = 16 * N, NA = (M + 63) / 64, NG = (T + 63) / 64;
     const int lane = threadIdx.x % 64;
-    const long long k = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + threadIdx.x / 64;
+    const long long k = static_cast<long long>(blockIdx.x) * (BLOCK / 64) + wave_index();
     if (k >= g.cells / 2) return;
     const int cell = colour_cell(static_cast<int>(k), 0, g.nx);
     const long long nodes = 2LL * g.cells * N;

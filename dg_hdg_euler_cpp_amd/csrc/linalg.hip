@@ -335,11 +335,6 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
 __device__ __forceinline__ float readlane_d(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
-// The wavefront's index in its workgroup, marked uniform so that what is
-// derived from it (columns, column addresses, loop bounds) stays scalar.
-__device__ __forceinline__ int wave_index() {
-    return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64));
-}
 
 // Wavefront max / min through DPP row shifts and row broadcasts instead of
 // LDS permutes: after row_shr 1, 2, 4, 8 lane 15 of each row holds its row's
@@ -1124,7 +1119,7 @@ __device__ __forceinline__ float half_value(unsigned short u, bool fp16) {
 // factorisations that do not record it themselves.
 __global__ __launch_bounds__(BLOCK) void k_umax(const float *__restrict__ square, float *__restrict__ umax, int M) {
     __shared__ float red[BLOCK / 64];
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64, wave = wave_index();
     const float *src = square + blockIdx.x * static_cast<long long>(M) * M;
     float mx = 0;
     for (int c = wave; c < M; c += BLOCK / 64)
@@ -1152,7 +1147,7 @@ __global__ __launch_bounds__(BLOCK) void k_umax(const float *__restrict__ square
 __global__ __launch_bounds__(BLOCK) void k_to_pairs(const float *__restrict__ square, unsigned short *__restrict__ out,
                                                     float *__restrict__ sigma, float *__restrict__ rdiag,
                                                     const float *__restrict__ umax, int M, int fp16, int *flag) {
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64, wave = wave_index();
     const long long stride = pair_stride(M);
     const float *src = square + blockIdx.x * static_cast<long long>(M) * M;
     unsigned short *dst = out + blockIdx.x * stride;
