@@ -384,11 +384,12 @@ __device__ __forceinline__ void wave_argmax(double &best, int &idx) {
 //  * the panel's interchanges, the U12 triangular solve and the trailing update
 //    are one pass over each trailing column: the column's rows are gathered
 //    through the panel's row permutation, the NB-row U12 segment is solved in
-//    registers and the updated column is written back. k_lu does this one
-//    column per wavefront with vector FMAs and L21 read from LDS; k_lu_mfma
-//    does it for 16-column tiles with the A22 -= L21 U12 product on MFMA;
+//    registers and the updated column is written back. k_lu does this for CB
+//    columns per wavefront pass with vector FMAs and L21 read from LDS (each
+//    LDS read feeds CB FMAs); k_lu_mfma does it for 16-column tiles with the
+//    A22 -= L21 U12 product on MFMA;
 //  * interchanges of later panels are applied to the finished L columns in one
-//    final gather pass (lu_later_interchanges).
+//    final gather pass (lu_finish).
 // Row indices in LDS are 16-bit: with the panel they keep an FP32 block at
 // about 25 KB, six workgroups per CU.
 using lu_row = short;
@@ -515,9 +516,9 @@ __device__ void lu_panel(T *__restrict__ A, LuShared<M, NB, T, LT> &S, int k0, i
     __syncthreads();
 }
 
-// Interchanges of later panels applied to the finished L columns (tau = gather
-// map of all interchanges after the panel, built backwards), then rowsrc and
-// the singularity flag.
+// Interchanges of later panels applied to the finished L columns (sigma here:
+// the gather map of all interchanges after the panel, composed from the last
+// panel back), then rowsrc and the singularity flag.
 // umax (optional): max |U| per block, complete only when every finished U
 // entry went through lu_panel (U11) or the caller's trailing pass (U12).
 template <int M, int NB, typename T, int LT>
