@@ -201,10 +201,12 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ v
     __syncthreads();
     T *out = blocks + static_cast<long long>(blockIdx.x) * M * M;
     const double sx = .5 * g.hy, sy = -.5 * g.hx;
-    for (int col = 0; col < M; ++col) {
-        const int w = col / P, cnode = col % P, jx = cnode % N, jy = cnode / N;
-        for (int row = threadIdx.x; row < M; row += BLOCK) {
-            const int v = row / P, rnode = row % P, i = rnode % N, j = rnode / N;
+    // a thread keeps its row's indices across the columns; the column's are
+    // uniform (one store per column stays coalesced over the rows)
+    for (int row = threadIdx.x; row < M; row += BLOCK) {
+        const int v = row / P, rnode = row % P, i = rnode % N, j = rnode / N;
+        for (int col = 0; col < M; ++col) {
+            const int w = col / P, cnode = col % P, jx = cnode % N, jy = cnode / N;
             const int vw = v * 4 + w;
             double val = 0;
             if (N > 1) {
