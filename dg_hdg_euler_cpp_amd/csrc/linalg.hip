@@ -201,29 +201,52 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(const double *__restrict__ v
     __syncthreads();
     T *out = blocks + static_cast<long long>(blockIdx.x) * M * M;
     const double sx = .5 * g.hy, sy = -.5 * g.hx;
-    // a thread keeps its row's indices across the columns; the column's are
-    // uniform (one store per column stays coalesced over the rows)
-    for (int row = threadIdx.x; row < M; row += BLOCK) {
-        const int v = row / P, rnode = row % P, i = rnode % N, j = rnode / N;
-        for (int col = 0; col < M; ++col) {
-            const int w = col / P, cnode = col % P, jx = cnode % N, jy = cnode / N;
-            const int vw = v * 4 + w;
-            double val = 0;
-            if (N > 1) {
-                if (j == jy) val -= (sx * tw[jy] * ax[cnode * 16 + vw]) * twd[jx + N * i];
-                if (i == jx) val += (sy * tw[jx] * ay[cnode * 16 + vw]) * twd[jy + N * j];
+    // entry (row, col); a thread keeps its rows' indices across the columns,
+    // the column's are uniform
+    auto entry = [&](int row, int v, int rnode, int i, int j, int col, int w, int cnode, int jx, int jy) {
+        const int vw = v * 4 + w;
+        double val = 0;
+        if (N > 1) {
+            if (j == jy) val -= (sx * tw[jy] * ax[cnode * 16 + vw]) * twd[jx + N * i];
+            if (i == jx) val += (sy * tw[jx] * ay[cnode * 16 + vw]) * twd[jy + N * j];
+        }
+        if (row == col) val += tmass[rnode] * inv_step;
+        if (j == jy) {
+            val += (tl[jx] * hs[(0 * N + j) * 16 + vw]) * tl[i];
+            val += (tr[jx] * hs[(1 * N + j) * 16 + vw]) * tr[i];
+        }
+        if (i == jx) {
+            val += (tl[jy] * hs[(2 * N + i) * 16 + vw]) * tl[j];
+            val += (tr[jy] * hs[(3 * N + i) * 16 + vw]) * tr[j];
+        }
+        return static_cast<T>(val);
+    };
+    if constexpr (!TRANSPOSE && M % 2 == 0 && sizeof(T) == 8) {
+        // FP64 column-major: a thread writes two consecutive rows of a column
+        // with one 16-byte store (FP64 blocks: 17.8 -> 15.1 ms at 256^2, k = 8;
+        // FP32 blocks are faster with a row per thread, 12.9 against 13.9 ms)
+        using T2 = double2;
+        for (int pr = threadIdx.x; pr < M / 2; pr += BLOCK) {
+            const int r0 = 2 * pr, r1 = r0 + 1;
+            const int v0 = r0 / P, n0 = r0 % P, i0 = n0 % N, j0 = n0 / N;
+            const int v1 = r1 / P, n1 = r1 % P, i1 = n1 % N, j1 = n1 / N;
+            for (int col = 0; col < M; ++col) {
+                const int w = col / P, cnode = col % P, jx = cnode % N, jy = cnode / N;
+                T2 pair;
+                pair.x = entry(r0, v0, n0, i0, j0, col, w, cnode, jx, jy);
+                pair.y = entry(r1, v1, n1, i1, j1, col, w, cnode, jx, jy);
+                *reinterpret_cast<T2 *>(out + static_cast<long long>(col) * M + r0) = pair;
             }
-            if (row == col) val += tmass[rnode] * inv_step;
-            if (j == jy) {
-                val += (tl[jx] * hs[(0 * N + j) * 16 + vw]) * tl[i];
-                val += (tr[jx] * hs[(1 * N + j) * 16 + vw]) * tr[i];
+        }
+    } else {
+        for (int row = threadIdx.x; row < M; row += BLOCK) {
+            const int v = row / P, rnode = row % P, i = rnode % N, j = rnode / N;
+            for (int col = 0; col < M; ++col) {
+                const int w = col / P, cnode = col % P, jx = cnode % N, jy = cnode / N;
+                const T val = entry(row, v, rnode, i, j, col, w, cnode, jx, jy);
+                if (TRANSPOSE) out[static_cast<long long>(row) * M + col] = val;
+                else out[static_cast<long long>(col) * M + row] = val;
             }
-            if (i == jx) {
-                val += (tl[jy] * hs[(2 * N + i) * 16 + vw]) * tl[j];
-                val += (tr[jy] * hs[(3 * N + i) * 16 + vw]) * tr[j];
-            }
-            if (TRANSPOSE) out[static_cast<long long>(row) * M + col] = static_cast<T>(val);
-            else out[static_cast<long long>(col) * M + row] = static_cast<T>(val);
         }
     }
 }
