@@ -885,8 +885,10 @@ constexpr int BJ_RING64 = DGH_BJ_RING64, BJ_RING32 = DGH_BJ_RING32;
 // so they keep predicated loads and the tested refill.
 // Block-Jacobi applies: factor loads as raw buffer loads with scalar column
 // offsets, bit-identical to the predicated loads. Measured at 256^2, k=8: the
-// FP16 apply 2.68-2.71 -> 2.60 ms (kept); the FP64 and FP32 applies unchanged
-// within run-to-run noise (8.8-9.0 and 4.5-4.6 ms either way; off).
+// FP16 apply 2.68-2.71 -> 2.60 ms, then 2.72 -> 2.48 ms without the
+// non-temporal bit (kept); FP64 factors 9.27-9.35 -> 8.94 ms with it (kept for
+// the large FP64 blocks); FP32 unchanged either way (DGH_BJ_BUFFER=1 forces
+// the buffer loads for every factor type).
 #ifndef DGH_BJ_PAIRS_BUFFER
 #define DGH_BJ_PAIRS_BUFFER 1
 #endif
@@ -972,11 +974,15 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
     // that no load sits under a branch. Otherwise the loads are predicated and
     // the lanes outside the segment hold zeros.
     constexpr bool BF = NA <= DGH_BJ_BRANCH_FREE_CHUNKS;
-#if DGH_BJ_BUFFER
     // Raw buffer loads (as in k_bj_apply_pairs): scalar column offsets, fixed
     // lane offsets, and an offset past the end for lanes outside the column's
     // segment, which load 0. The descriptor starts M entries before the block
-    // so that the lower part's column offset stays non-negative.
+    // so that the lower part's column offset stays non-negative. Used for FP64
+    // factors of the large blocks (measured at 256^2, k = 8: 8.94 ms per apply
+    // against 9.27-9.35 ms with non-temporal global loads, same box); FP32
+    // factors ran the same either way and keep the predicated global loads, as
+    // do the small blocks. DGH_BJ_BUFFER=1 selects the buffer loads for all.
+    constexpr bool BUF = DGH_BJ_BUFFER || (sizeof(T) == 8 && !BF);
     constexpr unsigned OOB = 0x80000000u;
     constexpr int S = sizeof(T);
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
@@ -992,49 +998,47 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(const TF *__restrict__ facto
             return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, vo, soff, 2));
         }
     };
-    // strictly lower part of column c (rows c+1..M-1), chunks a >= a0; the
-    // segment starts in chunk a0 or a0 + 1 (c < 64 (a0 + 1) + U)
+    // strictly lower part of column c (rows c+1..M-1), chunks a >= a0; buffer
+    // loads: the segment starts in chunk a0 or a0 + 1 (c < 64 (a0 + 1) + U)
     auto load_lower = [&](T(&b)[NA], int c, int a0) {
-        const int soff = static_cast<int>(packed_lower(M, c) - c - 1 + M) * S;
+        if constexpr (BUF) {
+            const int soff = static_cast<int>(packed_lower(M, c) - c - 1 + M) * S;
 #pragma unroll
-        for (int a = 0; a < NA; ++a)
-            if (a >= a0) b[a] = raw_load((a <= a0 + 1 && 64 * a + lane <= c) ? OOB : voff[a], soff);
+            for (int a = 0; a < NA; ++a)
+                if (a >= a0) b[a] = raw_load((a <= a0 + 1 && 64 * a + lane <= c) ? OOB : voff[a], soff);
+        } else {
+            const T *colp = F + packed_lower(M, c);
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const int row = a * 64 + lane;
+                const bool in = row > c && row < M;
+                if (a >= a0) {
+                    if constexpr (BF) b[a] = factor_load(colp + (in ? row - c - 1 : 0));
+                    else b[a] = in ? factor_load(colp + (row - c - 1)) : T(0);
+                }
+            }
+        }
     };
     // upper part (rows 0..c, diagonal included) of column c, chunks a <= a1;
-    // c lies in chunk a1 or a1 - 1
+    // buffer loads: c lies in chunk a1 or a1 - 1
     auto load_upper = [&](T(&b)[NA], int c, int a1) {
-        const int soff = static_cast<int>(packed_upper(M, c) + M) * S;
+        if constexpr (BUF) {
+            const int soff = static_cast<int>(packed_upper(M, c) + M) * S;
 #pragma unroll
-        for (int a = 0; a < NA; ++a)
-            if (a <= a1) b[a] = raw_load((a >= a1 - 1 && 64 * a + lane > c) ? OOB : voff[a], soff);
-    };
-#else
-    // strictly lower part of column c (rows c+1..M-1), chunks a >= a0
-    auto load_lower = [&](T(&b)[NA], int c, int a0) {
-        const T *colp = F + packed_lower(M, c);
+            for (int a = 0; a < NA; ++a)
+                if (a <= a1) b[a] = raw_load((a >= a1 - 1 && 64 * a + lane > c) ? OOB : voff[a], soff);
+        } else {
+            const T *colp = F + packed_upper(M, c);
 #pragma unroll
-        for (int a = 0; a < NA; ++a) {
-            const int row = a * 64 + lane;
-            const bool in = row > c && row < M;
-            if (a >= a0) {
-                if constexpr (BF) b[a] = factor_load(colp + (in ? row - c - 1 : 0));
-                else b[a] = in ? factor_load(colp + (row - c - 1)) : T(0);
+            for (int a = 0; a < NA; ++a) {
+                const int row = a * 64 + lane;
+                if (a <= a1) {
+                    if constexpr (BF) b[a] = factor_load(colp + (row <= c ? row : 0));
+                    else b[a] = row <= c ? factor_load(colp + row) : T(0);
+                }
             }
         }
     };
-    // upper part (rows 0..c, diagonal included) of column c, chunks a <= a1
-    auto load_upper = [&](T(&b)[NA], int c, int a1) {
-        const T *colp = F + packed_upper(M, c);
-#pragma unroll
-        for (int a = 0; a < NA; ++a) {
-            const int row = a * 64 + lane;
-            if (a <= a1) {
-                if constexpr (BF) b[a] = factor_load(colp + (row <= c ? row : 0));
-                else b[a] = row <= c ? factor_load(colp + row) : T(0);
-            }
-        }
-    };
-#endif
     // forward substitution with the unit lower L: x_i -= L_ij x_j for i > j;
     // column j sits in ring slot j % U
 #pragma unroll
@@ -1251,7 +1255,10 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
     // Raw buffer loads: the column's start is a scalar offset and each lane's
     // offset within the column is fixed per chunk, so no per-load address
     // arithmetic runs on the vector ALUs. Lanes outside the column's segment
-    // get an offset past the block's end, for which the load returns 0.
+    // get an offset past the block's end, for which the load returns 0. The
+    // loads use the default cache policy: with the non-temporal bit the apply
+    // took 2.717 ms against 2.481 ms (same box, 256^2, k = 8).
+    constexpr int PAIR_LOAD_POLICY = 0;
     constexpr unsigned OOB = 0x80000000u;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<unsigned *>(F), 0, static_cast<int>(pair_stride(M) * 2), 0x00020000);
@@ -1267,7 +1274,7 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
         for (int a = 0; a < NA; ++a) {
             if (a >= a0) {
                 const unsigned vo = (a <= a0 + 1 && 128 * a + 2 * lane < s0) ? OOB : voff[a];
-                b[a] = __builtin_amdgcn_raw_buffer_load_b32(rs, vo, soff, 2);
+                b[a] = __builtin_amdgcn_raw_buffer_load_b32(rs, vo, soff, PAIR_LOAD_POLICY);
             }
         }
     };
@@ -1279,7 +1286,7 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
         for (int a = 0; a < NA; ++a) {
             if (a <= a1) {
                 const unsigned vo = (a >= a1 - 1 && 128 * a + 2 * lane > c) ? OOB : voff[a];
-                b[a] = __builtin_amdgcn_raw_buffer_load_b32(rs, vo, soff, 2);
+                b[a] = __builtin_amdgcn_raw_buffer_load_b32(rs, vo, soff, PAIR_LOAD_POLICY);
             }
         }
     };
