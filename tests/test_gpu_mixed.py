@@ -109,10 +109,10 @@ def test_mixed_newton_runs_match_reference(precision, torch_cuda):
         assert abs(err[v] - float(summ[k])) <= 1e-9 * float(summ[k])
 
 
-@pytest.mark.parametrize("precision", ["mixed", "mixed-fp16"])
+@pytest.mark.parametrize("precision", ["mixed", "mixed-fp16", "mixed-bf16"])
 def test_mixed_update_error_vs_fp64_256(precision, torch_cuda):
-    """256^2, k=8: two BE Newton updates with FP32 factors (stored in FP32, or
-    in FP16 with a per-block scale of U) + FP32 inner GMRES; each update's
+    """256^2, k=8: two BE Newton updates with FP32 factors (stored in FP32, in
+    FP16 with a per-block scale of U, or in BF16) + FP32 inner GMRES; each update's
     direction is compared with the FP64 path's solution of the same system
     (precision_error) and the RMS sequence with the reference's."""
     from dg_hdg_euler_cpp_amd import DeviceContext
