@@ -392,7 +392,8 @@ def summarize_counters(fetch_csv, write_csv, out_json):
     d = cells * m
     alg = {"k_bj_apply<324, double": cells * m * m * 8 + cells * m * 4 + 2 * d * 8,
            "k_bj_apply<324, float": cells * m * m * 4 + cells * m * 4 + 2 * d * 4,
-           "k_bj_apply_pairs<324": cells * m * m * 2 + cells * m * 4 + 2 * d * 4}
+           "k_bj_apply_pairs<324": cells * m * m * 2 + cells * m * 4 + 2 * d * 4,
+           "k_bj_apply_colpairs<324": cells * m * m * 2 + cells * m * 4 + 2 * d * 4}
     out = {"fetch_scale": sf, "write_scale": sw, "kernels": {}}
     print(f"# copy kernel (1 GiB read + 1 GiB write): FETCH_SIZE reads {1 / sf:.4f} GiB, WRITE_SIZE {1 / sw:.4f} GiB")
     print(f"# scale factors: fetch x{sf:.4f}, write x{sw:.4f}")
@@ -644,7 +645,7 @@ def main():
     if os.path.exists(tpath) and args.nx == 256 and args.k == 8:
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-    prof = profile_figure("k_bj_apply_pairs" if head in ("mixed-fp16", "mixed-bf16") else "k_bj_apply",
+    prof = profile_figure("k_bj_apply_colpairs" if head in ("mixed-fp16", "mixed-bf16") else "k_bj_apply",
                           f"newton_update_{args.nx}_k{args.k}_{head.replace('-', '_')}")
     lu = dev["kernel_times"].get("lu")
     lu_rate = None

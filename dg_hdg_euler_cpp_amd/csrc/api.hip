@@ -900,6 +900,7 @@ struct dgh_ctx {
 
     // ---- mixed precision: FP32 operator, FP32/BF16 block Jacobi, FP32 inner
     //      GMRES inside FP64 flexible GMRES (no reference counterpart)
+    bool pair_rows = false;  // layout of factors_bf16: row pairs (comparison path) or column pairs
     bool two_byte_factors() const {
         return cfg.precision == DGH_PRECISION_MIXED_BF16 || cfg.precision == DGH_PRECISION_MIXED_FP16;
     }
@@ -922,13 +923,14 @@ struct dgh_ctx {
         // the 2-byte forms convert the square LU factors directly; only the
         // FP32 apply reads the packed layout
         if (two_byte_factors()) {
-            factors_bf16.alloc(static_cast<size_t>(cells) * pair_factor_stride(m), bytes);
+            pair_rows = two_byte_rowpairs_requested();  // fixed until the next refresh
+            factors_bf16.alloc(static_cast<size_t>(cells) * pair_factor_stride(m, pair_rows), bytes);
             factor_scale.alloc(cells, bytes);
             factor_rdiag.alloc(static_cast<size_t>(cells) * m, bytes);
             set_flag(2, 0);
             timed("pack", [&] {
                 HIPCHECK(launch_to_pairs(m, factors_f32.p, factors_bf16.p, factor_scale.p, factor_rdiag.p,
-                                         factor_umax.p, cells, fp16, flags.p + 2, stream));
+                                         factor_umax.p, cells, fp16, pair_rows, flags.p + 2, stream));
             });
             require(read_flag(2) == 0, "Singular block Jacobi factor");
         } else {
@@ -941,7 +943,7 @@ struct dgh_ctx {
         timed("bj_apply", [&] {
             if (two_byte_factors())
                 HIPCHECK(launch_bj_apply_pairs(m, factors_bf16.p, factor_scale.p, factor_rdiag.p, rowsrc32.p, r, z,
-                                               cells, cfg.precision == DGH_PRECISION_MIXED_FP16, stream));
+                                               cells, cfg.precision == DGH_PRECISION_MIXED_FP16, pair_rows, stream));
             else
                 HIPCHECK(launch_bj_apply(m, factors_f32.p, rowsrc32.p, r, z, cells, stream));
         });

@@ -155,14 +155,18 @@ hipError_t launch_bj_apply_colour(int m, const float *factors, const int *rowsrc
 // FP64 <-> FP32 vector conversion
 hipError_t launch_convert(const double *src, float *dst, long long count, hipStream_t s);
 hipError_t launch_convert(const float *src, double *dst, long long count, hipStream_t s);
-// 2-byte factors in the pair layout (BF16, or FP16 with a per-block scale of U),
-// converted from the square column-major FP32 LU factors
-long long pair_factor_stride(int m);
+// 2-byte factors (BF16, or FP16 with a per-block scale of U), converted from
+// the square column-major FP32 LU factors into the column-pair layout, or
+// into the row-pair layout when rowpairs is set (a comparison path, chosen
+// with DGH_BJ_ROWPAIRS=1); conversion, storage and apply must use one layout
+bool two_byte_rowpairs_requested();
+long long pair_factor_stride(int m, bool rowpairs);
 // (FP16 takes max |U| per block from umax, as launch_lu records it)
 hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, float *rdiag, const float *umax,
-                           int nblocks, int fp16, int *flag, hipStream_t s);
+                           int nblocks, int fp16, bool rowpairs, int *flag, hipStream_t s);
 hipError_t launch_bj_apply_pairs(int m, const void *factors, const float *sigma, const float *rdiag,
-                                 const int *rowsrc, const float *r, float *z, int nblocks, int fp16, hipStream_t s);
+                                 const int *rowsrc, const float *r, float *z, int nblocks, int fp16, bool rowpairs,
+                                 hipStream_t s);
 
 // GMRES building blocks over vectors of length `count` with the DG metric
 // inv_mass(node) (weights == nullptr) or a per-entry weight vector; dot

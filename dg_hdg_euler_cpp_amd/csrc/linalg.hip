@@ -1398,6 +1398,205 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply_pairs(const unsigned short *
         }
 }
 
+// ---------------------------------------------- 2-byte factors, column pairs
+// The same 2-byte factors with the entries of two columns side by side: the
+// lower parts of columns (c, c+1), c even, as rows c+1..M-1 of 4-byte words
+// (L_rc | L_r,c+1 << 16; row c+1 holds 0 for the unit diagonal), pair g = c/2
+// at word g (M - g); the upper parts of columns (c, c-1), c = M-1, M-3, ..., as
+// rows 0..c-1 of words (sigma U_rc | sigma U_r,c-1 << 16; row c-1 holds 0 for
+// the diagonal, whose FP32 reciprocal is in rdiag), pair q at word
+// M^2/4 + q (M - q). One column-pair step of the apply reads up to 4 M bytes
+// in 64-row chunks, as the 4-byte kernels' column steps do, instead of the
+// 2 M bytes of the row-pair layout (a dependency-free stream of the two
+// layouts at 256^2, k = 8: 5.95 against 5.68 TB/s).
+__host__ __device__ __forceinline__ long long colpair_stride(int M) {  // entries (2 bytes) per block
+    return 2 * ((static_cast<long long>(M) * M / 2 + 63) / 64 * 64);
+}
+// One block per factor, as k_to_pairs (scale, reciprocal pivots, flag), into
+// the column-pair layout; every word of the block is written once.
+__global__ __launch_bounds__(BLOCK) void k_to_colpairs(const float *__restrict__ square,
+                                                       unsigned short *__restrict__ out, float *__restrict__ sigma,
+                                                       float *__restrict__ rdiag, const float *__restrict__ umax,
+                                                       int M, int fp16, int *flag) {
+    const int lane = threadIdx.x % 64, wave = wave_index();
+    const float *src = square + blockIdx.x * static_cast<long long>(M) * M;
+    unsigned *dst = reinterpret_cast<unsigned *>(out + blockIdx.x * colpair_stride(M));
+    const float m = fp16 ? umax[blockIdx.x] : 0.f;
+    const float sc = (fp16 && m > 0) ? ldexpf(1.0f, -ilogbf(m)) : 1.0f;
+    if (threadIdx.x == 0) sigma[blockIdx.x] = sc;
+    bool bad = false;
+    for (int j = threadIdx.x; j < M; j += BLOCK) {
+        const float rd = 1.0f / (sc * src[static_cast<long long>(j) * M + j]);
+        rdiag[blockIdx.x * static_cast<long long>(M) + j] = rd;
+        bad = bad || !(isfinite(rd) && rd != 0.f);
+    }
+    if (bad) atomicOr(flag, 16);
+    const int half = M / 2;
+    for (int g = wave; g < half; g += BLOCK / 64) {
+        const int c = 2 * g, s0 = c + 1;
+        const float *c0 = src + static_cast<long long>(c) * M, *c1 = c0 + M;
+        unsigned *w = dst + static_cast<long long>(g) * (M - g);
+        for (int r = s0 + lane; r < M; r += 64) {
+            const unsigned a = half_bits(c0[r], fp16);
+            const unsigned b = r > c + 1 ? half_bits(c1[r], fp16) : 0u;
+            w[r - s0] = a | (b << 16);
+        }
+    }
+    unsigned *up = dst + static_cast<long long>(M) * M / 4;
+    for (int q = wave; q < half; q += BLOCK / 64) {
+        const int c = M - 1 - 2 * q;
+        const float *c0 = src + static_cast<long long>(c) * M, *c1 = c0 - M;
+        unsigned *w = up + static_cast<long long>(q) * (M - q);
+        for (int r = lane; r < c; r += 64) {
+            const unsigned a = half_bits(c0[r] * sc, fp16);
+            const unsigned b = r < c - 1 ? half_bits(c1[r] * sc, fp16) : 0u;
+            w[r] = a | (b << 16);
+        }
+    }
+    for (long long t = static_cast<long long>(M) * M / 2 + threadIdx.x; t < colpair_stride(M) / 2; t += BLOCK)
+        dst[t] = 0;
+}
+
+// Block Jacobi apply on the column-pair layout: lane l owns rows 64 a + l; per
+// column pair one 4-byte load per chunk brings both columns' entries of a row.
+// The forward sweep takes x_c (readlane), applies column c to every row below,
+// then takes the updated x_{c+1} and applies column c+1; the backward sweep
+// likewise with the reciprocal pivots. Every row sees the same operations in
+// the same order as in k_bj_apply_pairs. A ring of U column pairs is in flight
+// per wave; a pair's ring slot follows its index, which is a compile-time
+// constant because every 64-row chunk but the last holds 32 pairs (32 % U == 0).
+template <int M, bool FP16, int U>
+__global__ __launch_bounds__(BLOCK) void k_bj_apply_colpairs(const unsigned short *__restrict__ factors,
+                                                             const float *__restrict__ sigma,
+                                                             const float *__restrict__ rdiag,
+                                                             const int *__restrict__ rowsrc,
+                                                             const float *__restrict__ r, float *__restrict__ z,
+                                                             int nblocks) {
+    static_assert(M % 2 == 0 && 32 % U == 0, "column-pair layout");
+    constexpr int NA = (M + 63) / 64, HALF = M / 2;
+    constexpr int LAST = (M - 64 * (NA - 1)) / 2;  // pairs in the last chunk
+    constexpr unsigned OOB = 0x80000000u;
+    const int lane = threadIdx.x % 64;
+    const long long blk = static_cast<long long>(blockIdx.x) * (BLOCK / 64) +
+                          __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64));
+    if (blk >= nblocks) return;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<unsigned short *>(factors + blk * colpair_stride(M)), 0, static_cast<int>(colpair_stride(M) * 2),
+        0x00020000);
+    float x[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        const int row = 64 * a + lane;
+        x[a] = row < M ? r[blk * M + rowsrc[blk * M + row]] : 0.f;
+    }
+    auto lo = [](unsigned w) { return half_value(static_cast<unsigned short>(w & 0xffffu), FP16); };
+    auto hi = [](unsigned w) { return half_value(static_cast<unsigned short>(w >> 16), FP16); };
+    unsigned buf[U][NA];
+    // lower part of pair g (rows 2g+1..M-1), chunks a >= a0
+    auto load_lower = [&](unsigned(&b)[NA], int g, int a0) {
+        const int s0 = 2 * g + 1, soff = 4 * g * (M - g);
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            if (a >= a0) {
+                const int row = 64 * a + lane;
+                const unsigned vo = row >= s0 && row < M ? 4u * static_cast<unsigned>(row - s0) : OOB;
+                b[a] = __builtin_amdgcn_raw_buffer_load_b32(rs, vo, soff, 0);
+            }
+        }
+    };
+    // upper part of pair q (columns c = M-1-2q and c-1, rows 0..c-1), chunks a <= a1
+    auto load_upper = [&](unsigned(&b)[NA], int q, int a1) {
+        const int c = M - 1 - 2 * q, soff = 4 * (M * M / 4 + q * (M - q));
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            if (a <= a1) {
+                const int row = 64 * a + lane;
+                const unsigned vo = row < c ? 4u * static_cast<unsigned>(row) : OOB;
+                b[a] = __builtin_amdgcn_raw_buffer_load_b32(rs, vo, soff, 0);
+            }
+        }
+    };
+    // forward sweep with the unit lower factor
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (u < HALF) load_lower(buf[u], u, 0);
+#pragma unroll
+    for (int ja = 0; ja < NA; ++ja) {
+        const int npairs = ja < NA - 1 ? 32 : LAST;
+#pragma clang loop unroll(disable)
+        for (int p0 = 0; p0 < npairs; p0 += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int p = p0 + u;
+                if (npairs % U == 0 || p < npairs) {
+                    const int g = 32 * ja + p;
+                    unsigned(&cur)[NA] = buf[u];
+                    const float xc = readlane_d(x[ja], 2 * p);
+#pragma unroll
+                    for (int a = ja; a < NA; ++a)
+                        if (a > ja || lane > 2 * p) x[a] -= lo(cur[a]) * xc;
+                    const float xc1 = readlane_d(x[ja], 2 * p + 1);
+#pragma unroll
+                    for (int a = ja; a < NA; ++a)
+                        if (a > ja || lane > 2 * p + 1) x[a] -= hi(cur[a]) * xc1;
+                    if (g + U < HALF) load_lower(cur, g + U, ja);
+                }
+            }
+        }
+    }
+    if (FP16) {
+        const float sc = sigma[blk];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) x[a] *= sc;
+    }
+    float rd[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        const int row = 64 * a + lane;
+        rd[a] = row < M ? rdiag[blk * M + row] : 0.f;
+    }
+    // backward sweep: pair q's slot is (q + PAD) % U, so that the pairs of
+    // every full chunk start at slot 0
+    constexpr int PAD = (U - LAST % U) % U;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (u < HALF) load_upper(buf[(u + PAD) % U], u, NA - 1);
+#pragma unroll
+    for (int ja = NA - 1; ja >= 0; --ja) {
+        const int npairs = ja == NA - 1 ? LAST : 32;
+        const int qbase = ja == NA - 1 ? 0 : LAST + 32 * (NA - 2 - ja);
+        const int pad = ja == NA - 1 ? PAD : 0;
+#pragma clang loop unroll(disable)
+        for (int p0 = 0; p0 < npairs; p0 += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int p = p0 + u;
+                if (npairs % U == 0 || p < npairs) {
+                    const int q = qbase + p;
+                    unsigned(&cur)[NA] = buf[(u + pad) % U];
+                    const int lc = 2 * (npairs - p) - 1;  // lane of row c in chunk ja
+                    const float xj = readlane_d(x[ja] * rd[ja], lc);
+                    if (lane == lc) x[ja] = xj;
+#pragma unroll
+                    for (int a = 0; a <= ja; ++a)
+                        if (a < ja || lane < lc) x[a] -= lo(cur[a]) * xj;
+                    const float xj1 = readlane_d(x[ja] * rd[ja], lc - 1);
+                    if (lane == lc - 1) x[ja] = xj1;
+#pragma unroll
+                    for (int a = 0; a <= ja; ++a)
+                        if (a < ja || lane < lc - 1) x[a] -= hi(cur[a]) * xj1;
+                    if (q + U < HALF) load_upper(cur, q + U, ja);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        const int row = 64 * a + lane;
+        if (row < M) z[blk * M + row] = x[a];
+    }
+}
+
 // ----------------------------------------------------------------- GMRES
 constexpr int MAXV = 32;  // restart + 1 <= 32 basis vectors per launch
 
@@ -1943,12 +2142,19 @@ hipError_t launch_bj_apply(int m, const float *factors, const int *rowsrc, const
         k_bj_apply<M, float, float, BJ_RING32><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(factors, rowsrc, r, z, nblocks, -1, 0);
     })
 }
-long long pair_factor_stride(int m) { return pair_stride(m); }
+// 2-byte factor layouts: column pairs (the default) or row pairs (kept for
+// comparison); the caller fixes the layout per factorisation
+bool two_byte_rowpairs_requested() { return std::getenv("DGH_BJ_ROWPAIRS") != nullptr; }
+long long pair_factor_stride(int m, bool rowpairs) { return rowpairs ? pair_stride(m) : colpair_stride(m); }
 hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, float *rdiag, const float *umax,
-                           int nblocks, int fp16, int *flag, hipStream_t s) {
+                           int nblocks, int fp16, bool rowpairs, int *flag, hipStream_t s) {
     if (m % 2 != 0 || (fp16 && !umax)) return hipErrorInvalidValue;
-    k_to_pairs<<<nblocks, BLOCK, 0, s>>>(square, static_cast<unsigned short *>(out), sigma, rdiag, umax, m, fp16,
-                                         flag);
+    if (!rowpairs)
+        k_to_colpairs<<<nblocks, BLOCK, 0, s>>>(square, static_cast<unsigned short *>(out), sigma, rdiag, umax, m,
+                                                fp16, flag);
+    else
+        k_to_pairs<<<nblocks, BLOCK, 0, s>>>(square, static_cast<unsigned short *>(out), sigma, rdiag, umax, m, fp16,
+                                             flag);
     return hipGetLastError();
 }
 // columns in flight per wave (measured at 256^2, k=8: FP16 2.67 / 2.77 / 3.18 ms
@@ -1959,9 +2165,20 @@ hipError_t launch_to_pairs(int m, const float *square, void *out, float *sigma, 
 #ifndef DGH_BJ_RING_BF16
 #define DGH_BJ_RING_BF16 8
 #endif
+#ifndef DGH_BJ_RING_CP
+#define DGH_BJ_RING_CP 4
+#endif
 hipError_t launch_bj_apply_pairs(int m, const void *factors, const float *sigma, const float *rdiag,
-                                 const int *rowsrc, const float *r, float *z, int nblocks, int fp16, hipStream_t s) {
+                                 const int *rowsrc, const float *r, float *z, int nblocks, int fp16, bool rowpairs, hipStream_t s) {
     const auto *f = static_cast<const unsigned short *>(factors);
+    if (!rowpairs) {
+        DGH_DISPATCH_M(m, {
+            if (fp16)
+                k_bj_apply_colpairs<M, true, DGH_BJ_RING_CP><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, sigma, rdiag, rowsrc, r, z, nblocks);
+            else
+                k_bj_apply_colpairs<M, false, DGH_BJ_RING_CP><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, sigma, rdiag, rowsrc, r, z, nblocks);
+        })
+    }
     DGH_DISPATCH_M(m, {
         if (fp16)
             k_bj_apply_pairs<M, true, DGH_BJ_RING_FP16><<<grid_for(nblocks, BLOCK / 64), BLOCK, 0, s>>>(f, sigma, rdiag, rowsrc, r, z, nblocks);
