@@ -160,3 +160,36 @@ def test_mixed_512_config(precision, torch_cuda):
         # sequential sum carries about sqrt(N / 8) eps ~ 3e-13 of rounding
         assert abs(got["residual_physical_rms"] - w) <= [5e-13, 1e-10][it] * w
         assert 0 <= got["precision_error"] <= 1e-9, got["precision_error"]
+
+
+@pytest.mark.parametrize("precision", ["mixed-fp16", "mixed-bf16"])
+@pytest.mark.parametrize("k,nx,state", [(1, 6, "seeded"), (2, 8, "seeded"), (4, 10, "diagonal"), (8, 12, "seeded")])
+def test_two_byte_layouts_agree(precision, k, nx, state, torch_cuda, monkeypatch):
+    """The column-pair layout of the 2-byte factors (the default) and the
+    row-pair layout (DGH_BJ_ROWPAIRS=1, read when the factors are built) apply
+    every row's updates in the same order, so the block-Jacobi solves are equal
+    to the last bit."""
+    from dg_hdg_euler_cpp_amd import DeviceContext
+
+    torch = torch_cuda
+    args = ["--nx", str(nx), "--ny", str(nx), "--k", str(k), "--testcase", state, "--precision", precision,
+            "--integrator", "be", "--cfl", "0.1"]
+    out = {}
+    for layout in ("rows", "columns"):
+        if layout == "rows":
+            monkeypatch.setenv("DGH_BJ_ROWPAIRS", "1")
+        else:
+            monkeypatch.delenv("DGH_BJ_ROWPAIRS", raising=False)
+        print(f"\n[case] {precision} k={k} {nx}^2 {state} {layout}", flush=True)
+        with DeviceContext(args) as dev:
+            u = dev.seeded(5) if state == "seeded" else dev.initial()
+            rate = dev.freeze(u)
+            dev.set_step(0.1 / rate, u)
+            dev.dg_refresh(u)
+            dev.bj_refresh_mixed()
+            g = torch.Generator(device="cuda")
+            g.manual_seed(11)
+            x = torch.randn(dev.size, dtype=torch.float32, device="cuda", generator=g)
+            out[layout] = dev.bj_apply_f32(x).clone()
+    assert torch.equal(out["rows"], out["columns"])
+    assert bool(torch.isfinite(out["columns"]).all())
